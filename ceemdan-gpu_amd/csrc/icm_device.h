@@ -1,0 +1,88 @@
+// Device-side helpers shared by the ICEEMDAN kernels (gfx950, wave64).
+//
+// Everything here is written for 64-wide wavefronts: ballots are 64-bit,
+// shuffles span the whole wave, and workgroup reductions combine per-wave
+// partials through LDS.  No FMA contraction is allowed in this library
+// (built with -ffp-contract=off): the FP64 results must round exactly like
+// the CPU oracle.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace icm {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// number of set bits of `mask` below this lane
+__device__ __forceinline__ int lanes_below(uint64_t mask)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ uint64_t wave_ballot(bool p) { return __ballot(p); }
+
+template <typename T>
+__device__ __forceinline__ T shfl_up(T v, int d) { return __shfl_up(v, (unsigned)d, kWave); }
+template <typename T>
+__device__ __forceinline__ T shfl_down(T v, int d) { return __shfl_down(v, (unsigned)d, kWave); }
+
+// ----------------------------------------------------------------- look-back
+// Tile status word of the single-pass knot compaction:
+//   [63:48] epoch (launch id), [47:46] flag (1 = local count, 2 = inclusive),
+//   [45:23] count of maxima, [22:0] count of minima.
+constexpr uint32_t kFlagAggregate = 1;
+constexpr uint32_t kFlagInclusive = 2;
+
+__device__ __forceinline__ uint64_t pack_status(uint32_t epoch, uint32_t flag, uint32_t cmax,
+                                                uint32_t cmin)
+{
+    return ((uint64_t)(epoch & 0xFFFFu) << 48) | ((uint64_t)flag << 46) |
+           ((uint64_t)(cmax & 0x7FFFFFu) << 23) | (uint64_t)(cmin & 0x7FFFFFu);
+}
+__device__ __forceinline__ uint32_t st_epoch(uint64_t w) { return (uint32_t)(w >> 48); }
+__device__ __forceinline__ uint32_t st_flag(uint64_t w) { return (uint32_t)(w >> 46) & 3u; }
+__device__ __forceinline__ uint32_t st_cmax(uint64_t w) { return (uint32_t)(w >> 23) & 0x7FFFFFu; }
+__device__ __forceinline__ uint32_t st_cmin(uint64_t w) { return (uint32_t)w & 0x7FFFFFu; }
+
+__device__ __forceinline__ void status_store(uint64_t* p, uint64_t w)
+{
+    __hip_atomic_store(p, w, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t status_load(const uint64_t* p)
+{
+    return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Bounded spin: a predecessor tile always finishes (tiles are taken in
+// ticket order, so every predecessor is already resident), but the wait is
+// still capped so that a logic error reports instead of hanging the device.
+constexpr uint32_t kSpinCap = 1u << 24;
+
+// ------------------------------------------------------------------ Philox
+struct u64x4 {
+    uint64_t v[4];
+};
+
+__device__ __forceinline__ u64x4 philox4x64_10(uint64_t c0, uint64_t c1, uint64_t c2, uint64_t c3,
+                                              uint64_t k0, uint64_t k1)
+{
+    const uint64_t M0 = 0xD2E7470EE14C6C93ULL, M1 = 0xCA5A826395121157ULL;
+    const uint64_t W0 = 0x9E3779B97F4A7C15ULL, W1 = 0xBB67AE8584CAA73BULL;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint64_t lo0 = M0 * c0, hi0 = __umul64hi(M0, c0);
+        uint64_t lo1 = M1 * c2, hi1 = __umul64hi(M1, c2);
+        uint64_t n0 = hi1 ^ c1 ^ k0;
+        uint64_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += W0; k1 += W1;
+    }
+    u64x4 o;
+    o.v[0] = c0; o.v[1] = c1; o.v[2] = c2; o.v[3] = c3;
+    return o;
+}
+
+}  // namespace icm

@@ -1,0 +1,136 @@
+// Kernel argument blocks and launch helpers of the ICEEMDAN HIP path.
+//
+// Data layout in HBM (one "series" = one noise realization of one channel, or
+// one noise realization of a noise group):
+//   samples   real[sum_s len_s]            series s at off[s], contiguous
+//   knots     int x[2][kstride], real y[2][kstride], real c[2][kstride]
+//             type 0 = maxima, 1 = minima; series s owns [kbase[s], kbase[s]+len_s/2+2)
+//   tiles     T samples each; series s owns global tiles [tile0[s], tile0[s]+ntl[s])
+//   kpre      int[ntiles][2]   knots before the tile (exclusive prefix in the series)
+//   nk        int[nseries][2]  knot counts of the series
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace icm {
+
+constexpr int kThreads = 256;   // workgroup size of the tile kernels (4 waves)
+
+struct SeriesDev {
+    const int* tile_series;   // [ntiles]
+    const int* tile0;         // [nseries]
+    const int* ntl;           // [nseries]
+    const int64_t* off;       // [nseries]
+    const int* len;           // [nseries]
+    const int64_t* kbase;     // [nseries]
+    const int* chan;          // [nseries] channel (signal phase) or noise group (noise phase)
+    const int64_t* eoff;      // [nseries] offset of the series' noise-mode row
+    const int* nidx;          // [nseries] noise series whose modes this series uses
+    int nseries;
+    int ntiles;
+    int64_t kstride;
+};
+
+template <typename real>
+struct KnotsDev {
+    int* x;      // [2][kstride]
+    real* y;     // [2][kstride]
+    int* pre;    // [ntiles][2]
+    int* n;      // [nseries][2]
+};
+
+struct PassCtl {
+    uint64_t* status;      // [ntiles] look-back words
+    uint32_t epoch;        // launch id (unique within one forward)
+    unsigned* tickets;     // [epochs] tile tickets
+    int* winlist;          // [2 * maxwin] window items for the next solve
+    unsigned* wincount;    // [epochs] number of window items
+    int K;                 // rows kept per window of the solve
+    int* err;              // device error word
+};
+
+enum InitMode : int {
+    INIT_COPY = 0,     // h0 = src[off + p]                       (noise residual)
+    INIT_STAGE0 = 1,   // h0 = xt[c][p] + (eps0 * E[p]) * inv[s]  (cudaICEEMDAN.cu:68-78)
+    INIT_STAGEK = 2,   // h0 = r[c][p] + (eps0 * E[p]) * sq[c]    (cudaICEEMDAN.cu:80-90)
+};
+
+template <typename real>
+struct InitArgs {
+    int mode;
+    const real* src;        // INIT_COPY: samples buffer (series offsets)
+    const real* chan_sig;   // INIT_STAGE0/K: per-channel signal (xt or r)
+    const int64_t* chan_off;  // offset of channel c in chan_sig
+    const real* E;          // noise modes of this stage (eoff offsets)
+    const double* scale;    // INIT_STAGE0: inv[s] per series; INIT_STAGEK: sq[c] per channel
+    real eps0;
+};
+
+template <typename real>
+struct SiftArgs {
+    SeriesDev sd;
+    const real* h_in;
+    real* h_out;
+    KnotsDev<real> kin;     // knots of h_in
+    const real* c;          // spline coefficients c of kin
+    KnotsDev<real> kout;    // knots of h_out (written when emit != 0)
+    int emit;               // write knots of h_out
+    PassCtl pc;
+    const int* active;      // Rilling: per-series active flag (nullptr = all)
+    int* sx_cnt;            // Rilling: per-tile count of sx > 0.05 (nullptr = off)
+    int* sx_mark;           // Rilling: per-tile any(sx > 0.5)
+};
+
+template <typename real>
+struct SolveArgs {
+    SeriesDev sd;
+    KnotsDev<real> k;       // knots (y of knots 0 and n-1 fixed up in place)
+    real* c;                // [2][kstride] out
+    const int* winlist;
+    const unsigned* wincount;
+    const int* active;
+    int* err;
+};
+
+// host-side launchers (icm_kernels.hip)
+template <typename real>
+hipError_t launch_init(const SeriesDev& sd, const InitArgs<real>& ia, real* h_out,
+                       const KnotsDev<real>& kout, const PassCtl& pc, hipStream_t st);
+template <typename real>
+hipError_t launch_sift(const SiftArgs<real>& a, hipStream_t st);
+template <typename real>
+hipError_t launch_solve(const SolveArgs<real>& a, int grid, hipStream_t st);
+
+template <typename real>
+hipError_t launch_rilling_decide(const SeriesDev& sd, const int* nk, const int* sx_cnt,
+                                 const int* sx_mark, int* active, int* stop_pass, int* subs,
+                                 int pass, int cap, unsigned* n_active, hipStream_t st);
+template <typename real>
+hipError_t launch_series_var(const real* base, const int64_t* off, int64_t stride,
+                             const int* len, int nser, real* var_out, hipStream_t st);
+template <typename real>
+hipError_t launch_noise(uint64_t seed, int I, int64_t N, real* out, int64_t row_stride,
+                        hipStream_t st);
+template <typename real>
+hipError_t launch_ensemble(int stage, int nchan, int I, int max_len, const int* chan_len,
+                           const int64_t* chan_off, const int64_t* ser_off,
+                           const int64_t* ser_eoff, const int* ser_nidx, const real* hA, const real* hB,
+                           const int* mode_parity, int fixed_parity, const real* E,
+                           const double* inv, const double* sq, real eps0, const real* xt,
+                           real* r, real* imf_base, const int64_t* chan_imf_off, hipStream_t st);
+template <typename real>
+hipError_t launch_standardize(const real* y, const real* var, real* xt, int64_t N,
+                              hipStream_t st);
+template <typename real>
+hipError_t launch_inv_sqrt(const real* var, double* out, int n, int take_sqrt, hipStream_t st);
+template <typename real>
+hipError_t launch_sub(real* w, const real* e, int64_t n, hipStream_t st);
+template <typename real>
+hipError_t launch_copy(real* dst, const real* src, int64_t n, hipStream_t st);
+template <typename real>
+hipError_t launch_select_copy(const SeriesDev& sd, int max_len, const real* hA, const real* hB,
+                              const int* parity, real* dst, hipStream_t st);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st);
+hipError_t launch_check_index(const int* d_x, int64_t N, int* err, hipStream_t st);
+
+}  // namespace icm

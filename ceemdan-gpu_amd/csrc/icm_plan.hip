@@ -1,0 +1,937 @@
+// Host orchestration of the ICEEMDAN forward (C ABI in include/iceemdan_mi355x.h).
+//
+// Replaces the reference host drivers iceemdanS/iceemdanD
+// (cudaICEEMDAN.cu:619-1338 and :1340-2059).  Differences by design:
+//  * one plan owns every buffer; nothing is allocated per call
+//  * many channels of uneven length and all their realizations are sifted by
+//    the same launches ("series" = one realization of one channel)
+//  * FIXED mode has no host round trip at all and is captured once in a
+//    hipGraph; RILLING mode masks finished series on the device and polls a
+//    single counter for the early exit (the reference syncs 5x per pass)
+//  * noise modes are computed once per distinct length and shared by every
+//    channel of that length (all channels use the same seed, as every
+//    reference call uses seed 1234, cudaICEEMDAN.cu:740)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <new>
+#include <vector>
+
+#include "../../include/iceemdan_mi355x.h"
+#include "icm_errors.h"
+#include "icm_kernels.h"
+
+namespace icm {
+
+const int64_t kMaxLen = 16000000;
+const uint32_t kEpochs = 65536;
+
+// --------------------------------------------------------------- geometry
+
+int tile_of(int) { return 512; }
+int solve_B(int dtype) { return dtype == ICM_DTYPE_F64 ? 12 : 8; }
+int solve_H(int dtype) { return dtype == ICM_DTYPE_F64 ? 64 : 32; }
+int solve_K(int dtype) { return 64 * solve_B(dtype) - 2 * solve_H(dtype); }
+
+int validate(const icm_config* c)
+{
+    if (!c) return ICM_ERR_INVALID_ARG;
+    if (c->dtype != ICM_DTYPE_F32 && c->dtype != ICM_DTYPE_F64) return ICM_ERR_INVALID_ARG;
+    if (c->num_channels < 1 || !c->lengths) return ICM_ERR_INVALID_ARG;
+    for (int i = 0; i < c->num_channels; ++i)
+        if (c->lengths[i] < 4 || c->lengths[i] > kMaxLen) return ICM_ERR_INVALID_ARG;
+    if (c->num_noise < 1 || c->num_noise > 65535) return ICM_ERR_INVALID_ARG;
+    if (c->num_imfs < 2 || c->num_imfs > 64) return ICM_ERR_INVALID_ARG;
+    if (c->stop_mode != ICM_STOP_FIXED && c->stop_mode != ICM_STOP_RILLING) return ICM_ERR_INVALID_ARG;
+    if (c->max_sifts < 1 || c->max_sifts > 100000) return ICM_ERR_INVALID_ARG;
+    if (c->noise_max_sifts < 1 || c->noise_max_sifts > 100000) return ICM_ERR_INVALID_ARG;
+    if (!std::isfinite(c->noise_strength)) return ICM_ERR_INVALID_ARG;
+    int64_t total = 0;
+    for (int i = 0; i < c->num_channels; ++i) total += c->lengths[i];
+    if (total * (int64_t)c->num_noise > (int64_t)1 << 40) return ICM_ERR_INVALID_ARG;
+    // one launch covers every tile of a phase: tiles x 256 work-items < 2^32
+    int64_t tiles = 0;
+    for (int i = 0; i < c->num_channels; ++i) tiles += (c->lengths[i] + 511) / 512;
+    if (tiles * c->num_noise * 256 >= ((int64_t)1 << 32)) return ICM_ERR_INVALID_ARG;
+    return ICM_OK;
+}
+
+// Series tables of one phase (host side), built from (length, count) groups.
+struct HostSeries {
+    std::vector<int> tile_series, tile0, ntl, len, chan, nidx;
+    std::vector<int64_t> off, kbase, eoff;
+    int64_t samples = 0, kcap = 0, maxwin = 0;
+    int ntiles = 0;
+    int max_len = 0;
+};
+
+static void build_series(HostSeries& hs, const std::vector<int64_t>& lens, int I, int T, int K,
+                         const std::vector<int64_t>* noise_off,   // per channel: noise row base
+                         const std::vector<int>* noise_grp)
+{
+    int64_t off = 0, kb = 0;
+    int tiles = 0;
+    for (size_t c = 0; c < lens.size(); ++c) {
+        const int L = (int)lens[c];
+        hs.max_len = std::max(hs.max_len, L);
+        for (int i = 0; i < I; ++i) {
+            const int s = (int)hs.len.size();
+            const int nt = (L + T - 1) / T;
+            hs.len.push_back(L);
+            hs.off.push_back(off);
+            hs.kbase.push_back(kb);
+            hs.tile0.push_back(tiles);
+            hs.ntl.push_back(nt);
+            hs.chan.push_back((int)c);
+            if (noise_off) {
+                const int g = (*noise_grp)[c];
+                hs.eoff.push_back((*noise_off)[g] + (int64_t)i * L);
+                hs.nidx.push_back(g * I + i);
+            } else {
+                hs.eoff.push_back(off);
+                hs.nidx.push_back(s);
+            }
+            for (int t = 0; t < nt; ++t) hs.tile_series.push_back(s);
+            tiles += nt;
+            off += L;
+            const int64_t cap = L / 2 + 2;
+            kb += cap;
+            hs.maxwin += 2 * ((cap + K - 1) / K);
+        }
+    }
+    hs.samples = off;
+    hs.kcap = kb;
+    hs.ntiles = tiles;
+}
+
+// ------------------------------------------------------------------ plan
+
+struct DevSeries {
+    int *tile_series = nullptr, *tile0 = nullptr, *ntl = nullptr, *len = nullptr, *chan = nullptr,
+        *nidx = nullptr;
+    int64_t *off = nullptr, *kbase = nullptr, *eoff = nullptr;
+    int nseries = 0, ntiles = 0, max_len = 0;
+    int64_t samples = 0, kcap = 0, maxwin = 0;
+    SeriesDev view(int64_t kstride) const
+    {
+        SeriesDev v;
+        v.tile_series = tile_series; v.tile0 = tile0; v.ntl = ntl; v.off = off; v.len = len;
+        v.kbase = kbase; v.chan = chan; v.eoff = eoff; v.nidx = nidx;
+        v.nseries = nseries; v.ntiles = ntiles; v.kstride = kstride;
+        return v;
+    }
+};
+
+struct PlanBase {
+    icm_config cfg{};
+    std::vector<int64_t> lens;
+    std::vector<int64_t> glen;         // noise-group lengths
+    std::vector<int> chan_group;       // channel -> noise group
+    std::vector<int64_t> group_off;    // group -> first noise sample (I rows)
+    std::vector<int64_t> chan_off;     // channel -> offset in per-channel arrays
+    std::vector<int64_t> chan_imf_off; // channel -> offset of [M][N] output
+    HostSeries hn, hsig;
+    DevSeries dn, dsig;
+    int64_t kstride = 0, maxwin = 0, ntiles_max = 0, nser_max = 0, samples_max = 0;
+    int64_t chan_total = 0;
+    std::vector<void*> allocs;
+    int64_t bytes = 0;
+    hipStream_t own = nullptr;
+    // control
+    uint64_t* status = nullptr;
+    unsigned* tickets = nullptr;
+    unsigned* wincount = nullptr;
+    int* winlist = nullptr;
+    int* err = nullptr;
+    unsigned* n_active = nullptr;
+    unsigned* h_n_active = nullptr;   // pinned
+    int* h_err = nullptr;             // pinned
+    // rilling
+    int *active = nullptr, *stop_pass = nullptr, *subs = nullptr, *sx_cnt = nullptr,
+        *sx_mark = nullptr;
+    // trace
+    int trace_passes = 0;
+    int32_t* trace[2] = {nullptr, nullptr};
+    int32_t* stops[2] = {nullptr, nullptr};
+    // timing
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    icm_timing timing{};
+    // graphs
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    bool graph_failed = false;
+    uint32_t epoch = 0;
+    int64_t* d_chan_off = nullptr;
+    int64_t* d_chan_imf_off = nullptr;
+    int* d_chan_len = nullptr;
+    int max_chan_len = 0;
+    bool noise_external = false;
+
+    virtual ~PlanBase() {}
+    virtual int forward(const void* const* sig, void* const* imfs, const void* noise,
+                        hipStream_t st) = 0;
+    virtual int noise_modes_out(void* d_out, int64_t count, hipStream_t st) = 0;
+
+    template <typename T>
+    int alloc(T** p, int64_t n)
+    {
+        *p = nullptr;
+        if (n <= 0) n = 1;
+        void* q = nullptr;
+        if (hipMalloc(&q, (size_t)n * sizeof(T)) != hipSuccess) return ICM_ERR_OUT_OF_MEMORY;
+        allocs.push_back(q);
+        bytes += n * (int64_t)sizeof(T);
+        *p = (T*)q;
+        return ICM_OK;
+    }
+    template <typename T>
+    int upload(T** p, const std::vector<T>& v)
+    {
+        int rc = alloc(p, (int64_t)v.size());
+        if (rc) return rc;
+        if (!v.empty() &&
+            hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+            return ICM_ERR_HIP;
+        return ICM_OK;
+    }
+    int upload_series(DevSeries& d, const HostSeries& h)
+    {
+        int rc;
+        if ((rc = upload(&d.tile_series, h.tile_series))) return rc;
+        if ((rc = upload(&d.tile0, h.tile0))) return rc;
+        if ((rc = upload(&d.ntl, h.ntl))) return rc;
+        if ((rc = upload(&d.len, h.len))) return rc;
+        if ((rc = upload(&d.chan, h.chan))) return rc;
+        if ((rc = upload(&d.nidx, h.nidx))) return rc;
+        if ((rc = upload(&d.off, h.off))) return rc;
+        if ((rc = upload(&d.kbase, h.kbase))) return rc;
+        if ((rc = upload(&d.eoff, h.eoff))) return rc;
+        d.nseries = (int)h.len.size();
+        d.ntiles = h.ntiles;
+        d.max_len = h.max_len;
+        d.samples = h.samples;
+        d.kcap = h.kcap;
+        d.maxwin = h.maxwin;
+        return ICM_OK;
+    }
+    void free_all()
+    {
+        for (int i = 0; i < 2; ++i)
+            if (gexec[i]) (void)hipGraphExecDestroy(gexec[i]);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (void* p : allocs) (void)hipFree(p);
+        allocs.clear();
+        if (h_n_active) (void)hipHostFree(h_n_active);
+        if (h_err) (void)hipHostFree(h_err);
+        if (own) (void)hipStreamDestroy(own);
+    }
+};
+
+void layout(PlanBase& P)
+{
+    const icm_config& c = P.cfg;
+    const int T = tile_of(c.dtype), K = solve_K(c.dtype), I = c.num_noise;
+    // noise groups = distinct lengths, in first-appearance order
+    std::map<int64_t, int> gid;
+    for (int ch = 0; ch < c.num_channels; ++ch) {
+        const int64_t L = c.lengths[ch];
+        P.lens.push_back(L);
+        auto it = gid.find(L);
+        if (it == gid.end()) {
+            const int g = (int)P.glen.size();
+            gid[L] = g;
+            P.glen.push_back(L);
+            P.chan_group.push_back(g);
+        } else {
+            P.chan_group.push_back(it->second);
+        }
+    }
+    int64_t o = 0;
+    for (int64_t L : P.glen) {
+        P.group_off.push_back(o);
+        o += L * I;
+    }
+    int64_t co = 0, io = 0;
+    for (int64_t L : P.lens) {
+        P.chan_off.push_back(co);
+        P.chan_imf_off.push_back(io);
+        co += L;
+        io += L * c.num_imfs;
+        P.max_chan_len = std::max(P.max_chan_len, (int)L);
+    }
+    P.chan_total = co;
+    build_series(P.hn, P.glen, I, T, K, nullptr, nullptr);
+    build_series(P.hsig, P.lens, I, T, K, &P.group_off, &P.chan_group);
+    P.kstride = std::max(P.hn.kcap, P.hsig.kcap);
+    P.maxwin = std::max(P.hn.maxwin, P.hsig.maxwin);
+    P.ntiles_max = std::max(P.hn.ntiles, P.hsig.ntiles);
+    P.nser_max = std::max((int64_t)P.hn.len.size(), (int64_t)P.hsig.len.size());
+    P.samples_max = std::max(P.hn.samples, P.hsig.samples);
+}
+
+int64_t plan_bytes_estimate(const icm_config* c)
+{
+    PlanBase* dummy = nullptr;
+    (void)dummy;
+    struct Tmp : PlanBase {
+        int forward(const void* const*, void* const*, const void*, hipStream_t) override { return 0; }
+        int noise_modes_out(void*, int64_t, hipStream_t) override { return 0; }
+    } P;
+    P.cfg = *c;
+    layout(P);
+    const int64_t s = c->dtype == ICM_DTYPE_F64 ? 8 : 4;
+    const int M = c->num_imfs;
+    int64_t b = 0;
+    b += 2 * P.samples_max * s;                    // h ping-pong
+    b += (int64_t)(M - 1) * P.hn.samples * s;      // noise modes
+    b += P.hn.samples * s;                         // noise residual
+    b += 2 * 2 * P.kstride * (4 + s);              // knots x,y ping-pong
+    b += 2 * P.kstride * s;                        // c
+    b += 2 * 2 * P.ntiles_max * 4 + 2 * 2 * P.nser_max * 4;
+    b += P.ntiles_max * 8 + 2 * (int64_t)kEpochs * 4 + 2 * P.maxwin * 4;
+    b += 5 * P.chan_total * s + P.chan_total * M * s;
+    return b;
+}
+
+// ------------------------------------------------------------ typed plan
+
+template <typename real>
+struct Plan : PlanBase {
+    real *hA = nullptr, *hB = nullptr, *modes = nullptr, *wres = nullptr;
+    int* kx[2] = {nullptr, nullptr};
+    real* ky[2] = {nullptr, nullptr};
+    int* kpre[2] = {nullptr, nullptr};
+    int* nk[2] = {nullptr, nullptr};
+    real* kc = nullptr;
+    real *y = nullptr, *xt = nullptr, *r = nullptr, *imf = nullptr;
+    real *var_chan = nullptr, *var_noise = nullptr;
+    double *inv_noise = nullptr, *sq_chan = nullptr;
+
+    int setup()
+    {
+        int rc;
+        const int M = cfg.num_imfs;
+        if ((rc = upload_series(dn, hn))) return rc;
+        if ((rc = upload_series(dsig, hsig))) return rc;
+        if ((rc = alloc(&hA, samples_max))) return rc;
+        if ((rc = alloc(&hB, samples_max))) return rc;
+        if ((rc = alloc(&modes, (int64_t)(M - 1) * hn.samples))) return rc;
+        if ((rc = alloc(&wres, hn.samples))) return rc;
+        for (int b = 0; b < 2; ++b) {
+            if ((rc = alloc(&kx[b], 2 * kstride))) return rc;
+            if ((rc = alloc(&ky[b], 2 * kstride))) return rc;
+            if ((rc = alloc(&kpre[b], 2 * ntiles_max))) return rc;
+            if ((rc = alloc(&nk[b], 2 * nser_max))) return rc;
+        }
+        if ((rc = alloc(&kc, 2 * kstride))) return rc;
+        if ((rc = alloc(&status, ntiles_max))) return rc;
+        if ((rc = alloc(&tickets, kEpochs))) return rc;
+        if ((rc = alloc(&wincount, kEpochs))) return rc;
+        if ((rc = alloc(&winlist, 2 * maxwin))) return rc;
+        if ((rc = alloc(&err, 1))) return rc;
+        if ((rc = alloc(&n_active, 1))) return rc;
+        if ((rc = alloc(&y, chan_total))) return rc;
+        if ((rc = alloc(&xt, chan_total))) return rc;
+        if ((rc = alloc(&r, chan_total))) return rc;
+        if ((rc = alloc(&imf, chan_total * M))) return rc;
+        if ((rc = alloc(&var_chan, cfg.num_channels))) return rc;
+        if ((rc = alloc(&sq_chan, cfg.num_channels))) return rc;
+        if ((rc = alloc(&var_noise, (int64_t)hn.len.size()))) return rc;
+        if ((rc = alloc(&inv_noise, (int64_t)hn.len.size()))) return rc;
+        if ((rc = upload(&d_chan_off, chan_off))) return rc;
+        if ((rc = upload(&d_chan_imf_off, chan_imf_off))) return rc;
+        std::vector<int> cl(lens.begin(), lens.end());
+        if ((rc = upload(&d_chan_len, cl))) return rc;
+        if (cfg.stop_mode == ICM_STOP_RILLING) {
+            if ((rc = alloc(&active, nser_max))) return rc;
+            if ((rc = alloc(&stop_pass, nser_max))) return rc;
+            if ((rc = alloc(&subs, nser_max))) return rc;
+            if ((rc = alloc(&sx_cnt, ntiles_max))) return rc;
+            if ((rc = alloc(&sx_mark, ntiles_max))) return rc;
+        }
+        if (hipHostMalloc((void**)&h_n_active, sizeof(unsigned), hipHostMallocDefault) != hipSuccess)
+            return ICM_ERR_OUT_OF_MEMORY;
+        if (hipHostMalloc((void**)&h_err, sizeof(int), hipHostMallocDefault) != hipSuccess)
+            return ICM_ERR_OUT_OF_MEMORY;
+        if (hipStreamCreateWithFlags(&own, hipStreamNonBlocking) != hipSuccess) return ICM_ERR_HIP;
+        for (auto& e : ev)
+            if (hipEventCreate(&e) != hipSuccess) return ICM_ERR_HIP;
+        return ICM_OK;
+    }
+
+    PassCtl ctl(uint32_t e)
+    {
+        PassCtl pc;
+        pc.status = status;
+        pc.epoch = e;
+        pc.tickets = tickets;
+        pc.winlist = winlist;
+        pc.wincount = wincount;
+        pc.K = solve_K(cfg.dtype);
+        pc.err = err;
+        return pc;
+    }
+    KnotsDev<real> knots(int b)
+    {
+        KnotsDev<real> k;
+        k.x = kx[b];
+        k.y = ky[b];
+        k.pre = kpre[b];
+        k.n = nk[b];
+        return k;
+    }
+
+    int solve_grid() const { return 2048; }
+
+#define ICM_TRY(x)                                     \
+    do {                                               \
+        if ((x) != hipSuccess) return ICM_ERR_HIP;     \
+    } while (0)
+
+    int reset_control(hipStream_t st)
+    {
+        ICM_TRY(launch_fill_u32((uint32_t*)tickets, 0u, kEpochs, st));
+        ICM_TRY(launch_fill_u32((uint32_t*)wincount, 0u, kEpochs, st));
+        ICM_TRY(launch_fill_u32((uint32_t*)status, 0u, 2 * ntiles_max, st));
+        epoch = 0;
+        return ICM_OK;
+    }
+    uint32_t next_epoch() { return ++epoch; }
+
+    int trace_copy(int phase, int stage, int pass, const DevSeries& d, int b, hipStream_t st)
+    {
+        if (!trace_passes || pass >= trace_passes || !trace[phase]) return ICM_OK;
+        const int64_t n = 2 * (int64_t)d.nseries;
+        int32_t* dst = trace[phase] + ((int64_t)stage * trace_passes + pass) * n;
+        ICM_TRY(launch_copy<float>((float*)dst, (const float*)nk[b], n, st));
+        return ICM_OK;
+    }
+
+    // Sift one stage of a phase, FIXED mode: init + S x (solve, sift).
+    // The stage output (h after S subtractions) ends in `out` if given
+    // (series offsets), else in buffer S % 2.
+    int stage_fixed(int phase, int stage, const DevSeries& d, const InitArgs<real>& ia, int S,
+                    real* out, hipStream_t st)
+    {
+        const SeriesDev sd = d.view(kstride);
+        int rc;
+        if (epoch + (uint32_t)S + 8 >= kEpochs - 1 && (rc = reset_control(st))) return rc;
+        uint32_t e_emit = next_epoch();
+        ICM_TRY(launch_init<real>(sd, ia, hA, knots(0), ctl(e_emit), st));
+        if ((rc = trace_copy(phase, stage, 0, d, 0, st))) return rc;
+        for (int j = 0; j < S; ++j) {
+            const int b = j & 1;
+            SolveArgs<real> sa;
+            sa.sd = sd;
+            sa.k = knots(b);
+            sa.c = kc;
+            sa.winlist = winlist;
+            sa.wincount = wincount + e_emit;
+            sa.active = nullptr;
+            sa.err = err;
+            ICM_TRY(launch_solve<real>(sa, solve_grid(), st));
+            SiftArgs<real> a;
+            a.sd = sd;
+            a.h_in = b ? hB : hA;
+            a.h_out = (j == S - 1 && out) ? out : (b ? hA : hB);
+            a.kin = knots(b);
+            a.c = kc;
+            a.kout = knots(b ^ 1);
+            a.emit = (j < S - 1) ? 1 : 0;
+            e_emit = next_epoch();
+            a.pc = ctl(e_emit);
+            a.active = nullptr;
+            a.sx_cnt = nullptr;
+            a.sx_mark = nullptr;
+            ICM_TRY(launch_sift<real>(a, st));
+            if (j < S - 1 && (rc = trace_copy(phase, stage, j + 1, d, b ^ 1, st))) return rc;
+        }
+        return ICM_OK;
+    }
+
+    // RILLING mode: passes until every series stopped or the cap is reached.
+    // Returns with stop_pass[s] = pass whose input h is the exported mode
+    // (buffer stop_pass & 1).
+    int stage_rilling(int phase, int stage, const DevSeries& d, const InitArgs<real>& ia, int cap,
+                      hipStream_t st)
+    {
+        const SeriesDev sd = d.view(kstride);
+        ICM_TRY(launch_fill_u32((uint32_t*)active, 1u, d.nseries, st));
+        ICM_TRY(launch_fill_u32((uint32_t*)subs, 0u, d.nseries, st));
+        ICM_TRY(launch_fill_u32((uint32_t*)stop_pass, 0u, d.nseries, st));
+        if (epoch + 2 * (uint32_t)cap + 8 >= kEpochs - 1) {
+            int rc = reset_control(st);
+            if (rc) return rc;
+        }
+        uint32_t e_emit = next_epoch();
+        ICM_TRY(launch_init<real>(sd, ia, hA, knots(0), ctl(e_emit), st));
+        int rc;
+        if ((rc = trace_copy(phase, stage, 0, d, 0, st))) return rc;
+        for (int j = 0; j < cap; ++j) {
+            const int b = j & 1;
+            SolveArgs<real> sa;
+            sa.sd = sd;
+            sa.k = knots(b);
+            sa.c = kc;
+            sa.winlist = winlist;
+            sa.wincount = wincount + e_emit;
+            sa.active = active;
+            sa.err = err;
+            ICM_TRY(launch_solve<real>(sa, solve_grid(), st));
+            SiftArgs<real> a;
+            a.sd = sd;
+            a.h_in = b ? hB : hA;
+            a.h_out = b ? hA : hB;
+            a.kin = knots(b);
+            a.c = kc;
+            a.kout = knots(b ^ 1);
+            a.emit = 1;
+            e_emit = next_epoch();
+            a.pc = ctl(e_emit);
+            a.active = active;
+            a.sx_cnt = sx_cnt;
+            a.sx_mark = sx_mark;
+            ICM_TRY(launch_sift<real>(a, st));
+            ICM_TRY(launch_fill_u32((uint32_t*)n_active, 0u, 1, st));
+            ICM_TRY(launch_rilling_decide<real>(sd, nk[b], sx_cnt, sx_mark, active, stop_pass, subs,
+                                                j, cap, n_active, st));
+            if ((rc = trace_copy(phase, stage, j + 1, d, b ^ 1, st))) return rc;
+            // early exit: results are identical to running every pass to the
+            // cap with finished series masked (they never change again)
+            const bool poll = (j < 16) || ((j & 3) == 3) || (j == cap - 1);
+            if (poll) {
+                ICM_TRY(hipMemcpyAsync(h_n_active, n_active, sizeof(unsigned),
+                                       hipMemcpyDeviceToHost, st));
+                ICM_TRY(hipStreamSynchronize(st));
+                if (*h_n_active == 0) break;
+            }
+        }
+        if (stops[phase]) {
+            int32_t* dst = stops[phase] + (int64_t)stage * d.nseries;
+            ICM_TRY(launch_copy<float>((float*)dst, (const float*)subs, d.nseries, st));
+        }
+        return ICM_OK;
+    }
+
+    // Noise phase: E_{k+1}(w^i) for k = 0..M-2 (cudaICEEMDAN.cu:845-994)
+    int noise_phase(hipStream_t st)
+    {
+        const int M = cfg.num_imfs;
+        int rc;
+        InitArgs<real> ia{};
+        ia.mode = INIT_COPY;
+        ia.src = wres;
+        for (int k = 0; k < M - 1; ++k) {
+            real* Ek = modes + (int64_t)k * hn.samples;
+            if (cfg.stop_mode == ICM_STOP_FIXED) {
+                if ((rc = stage_fixed(ICM_PHASE_NOISE, k, dn, ia, cfg.noise_max_sifts, Ek, st)))
+                    return rc;
+            } else {
+                if ((rc = stage_rilling(ICM_PHASE_NOISE, k, dn, ia, cfg.noise_max_sifts, st)))
+                    return rc;
+                ICM_TRY(launch_select_copy<real>(dn.view(kstride), dn.max_len, hA, hB, stop_pass,
+                                                 Ek, st));
+            }
+            ICM_TRY(launch_sub<real>(wres, Ek, hn.samples, st));
+        }
+        return ICM_OK;
+    }
+
+    // CEEMDAN stages (cudaICEEMDAN.cu:1003-1298)
+    int signal_phase(hipStream_t st)
+    {
+        const int M = cfg.num_imfs, I = cfg.num_noise, C = cfg.num_channels;
+        const real eps0 = (real)cfg.noise_strength;
+        int rc;
+        // stage-0 statistics: var(x) per channel, var(E_1(w^i)) per noise series
+        ICM_TRY(launch_series_var<real>(y, d_chan_off, 0, d_chan_len, C, var_chan, st));
+        for (int c = 0; c < C; ++c)
+            ICM_TRY(launch_standardize<real>(y + chan_off[c], var_chan + c, xt + chan_off[c],
+                                             lens[c], st));
+        ICM_TRY(launch_series_var<real>(modes, dn.off, 0, dn.len, dn.nseries, var_noise, st));
+        ICM_TRY(launch_inv_sqrt<real>(var_noise, inv_noise, dn.nseries, 0, st));
+        for (int k = 0; k < M - 1; ++k) {
+            const real* Ek = modes + (int64_t)k * hn.samples;
+            InitArgs<real> ia{};
+            ia.E = Ek;
+            ia.eps0 = eps0;
+            ia.chan_off = d_chan_off;
+            if (k == 0) {
+                ia.mode = INIT_STAGE0;
+                ia.chan_sig = xt;
+                ia.scale = inv_noise;
+            } else {
+                ICM_TRY(launch_series_var<real>(r, d_chan_off, 0, d_chan_len, C, var_chan, st));
+                ICM_TRY(launch_inv_sqrt<real>(var_chan, sq_chan, C, 1, st));
+                ia.mode = INIT_STAGEK;
+                ia.chan_sig = r;
+                ia.scale = sq_chan;
+            }
+            const int* parity = nullptr;
+            int fixed_parity = 0;
+            if (cfg.stop_mode == ICM_STOP_FIXED) {
+                if ((rc = stage_fixed(ICM_PHASE_SIGNAL, k, dsig, ia, cfg.max_sifts, nullptr, st)))
+                    return rc;
+                fixed_parity = cfg.max_sifts & 1;
+            } else {
+                if ((rc = stage_rilling(ICM_PHASE_SIGNAL, k, dsig, ia, cfg.max_sifts, st))) return rc;
+                parity = stop_pass;
+            }
+            ICM_TRY(launch_ensemble<real>(k, C, I, max_chan_len, d_chan_len, d_chan_off, dsig.off,
+                                          dsig.eoff, dsig.nidx, hA, hB, parity, fixed_parity, Ek,
+                                          inv_noise, sq_chan, eps0, xt, r, imf, d_chan_imf_off,
+                                          st));
+        }
+        // residue row M-1 (cudaICEEMDAN.cu:1298)
+        for (int c = 0; c < C; ++c)
+            ICM_TRY(launch_copy<real>(imf + chan_imf_off[c] + (int64_t)(M - 1) * lens[c],
+                                      r + chan_off[c], lens[c], st));
+        return ICM_OK;
+    }
+
+    int capture_or_run(int which, hipStream_t st)
+    {
+        auto body = [&](hipStream_t s) -> int {
+            return which == 0 ? noise_phase(s) : signal_phase(s);
+        };
+        const bool graph = cfg.use_graph && cfg.stop_mode == ICM_STOP_FIXED && !trace_passes &&
+                           !graph_failed;
+        if (!graph) return body(st);
+        if (!gexec[which]) {
+            // capture on the plan's own stream, replay on the caller's
+            const uint32_t e0 = epoch;
+            if (hipStreamBeginCapture(own, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                graph_failed = true;
+                return body(st);
+            }
+            int rc = body(own);
+            hipGraph_t g = nullptr;
+            hipError_t ce = hipStreamEndCapture(own, &g);
+            if (rc != ICM_OK || ce != hipSuccess || !g) {
+                if (g) (void)hipGraphDestroy(g);
+                (void)hipGetLastError();
+                graph_failed = true;
+                epoch = e0;
+                return body(st);
+            }
+            hipGraphExec_t ex = nullptr;
+            if (hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                (void)hipGetLastError();
+                graph_failed = true;
+                epoch = e0;
+                return body(st);
+            }
+            (void)hipGraphDestroy(g);
+            gexec[which] = ex;
+        }
+        // the graph's launches carry the epochs assigned at capture; the
+        // control words were reset before the replay
+        ICM_TRY(hipGraphLaunch(gexec[which], st));
+        return ICM_OK;
+    }
+
+    int forward(const void* const* sig, void* const* out, const void* noise, hipStream_t st) override
+    {
+        const int C = cfg.num_channels, M = cfg.num_imfs;
+        *h_err = 0;
+        ICM_TRY(hipEventRecord(ev[0], st));
+        ICM_TRY(launch_fill_u32((uint32_t*)err, 0u, 1, st));
+        for (int c = 0; c < C; ++c) {
+            if (!sig[c] || !out[c]) return ICM_ERR_INVALID_ARG;
+            ICM_TRY(launch_copy<real>(y + chan_off[c], (const real*)sig[c], lens[c], st));
+        }
+        noise_external = (noise != nullptr);
+        if (noise_external) {
+            if (glen.size() != 1) return ICM_ERR_INVALID_ARG;
+            ICM_TRY(launch_copy<real>(wres, (const real*)noise, hn.samples, st));
+        } else {
+            // white noise w^i (cudaICEEMDAN.cu:739-741), one set per distinct length
+            for (size_t g = 0; g < glen.size(); ++g)
+                ICM_TRY(launch_noise<real>(cfg.seed, cfg.num_noise, glen[g], wres + group_off[g],
+                                           glen[g], st));
+        }
+        int rc;
+        if ((rc = reset_control(st))) return rc;
+        if ((rc = capture_or_run(0, st))) return rc;
+        ICM_TRY(hipEventRecord(ev[1], st));
+        if ((rc = capture_or_run(1, st))) return rc;
+        ICM_TRY(hipEventRecord(ev[2], st));
+        for (int c = 0; c < C; ++c)
+            ICM_TRY(launch_copy<real>((real*)out[c], imf + chan_imf_off[c], (int64_t)M * lens[c], st));
+        ICM_TRY(hipEventRecord(ev[3], st));
+        ICM_TRY(hipMemcpyAsync(h_err, err, sizeof(int), hipMemcpyDeviceToHost, st));
+        ICM_TRY(hipStreamSynchronize(st));
+        float t01 = 0, t12 = 0, t03 = 0;
+        (void)hipEventElapsedTime(&t01, ev[0], ev[1]);
+        (void)hipEventElapsedTime(&t12, ev[1], ev[2]);
+        (void)hipEventElapsedTime(&t03, ev[0], ev[3]);
+        timing.noise_ms = t01;
+        timing.signal_ms = t12;
+        timing.total_ms = t03;
+        if (*h_err) return ICM_ERR_DEVICE;
+        return ICM_OK;
+    }
+
+    // one pass over the noise-phase series (I series of length N, back to back)
+    int sift_once(const real* h, real* hn, int32_t* nk_out, hipStream_t st)
+    {
+        int rc;
+        ICM_TRY(launch_fill_u32((uint32_t*)err, 0u, 1, st));
+        if ((rc = reset_control(st))) return rc;
+        InitArgs<real> ia{};
+        ia.mode = INIT_COPY;
+        ia.src = h;
+        if ((rc = stage_fixed(ICM_PHASE_NOISE, 0, dn, ia, 1, hn, st))) return rc;
+        if (nk_out) ICM_TRY(launch_copy<float>((float*)nk_out, (const float*)nk[0], 2 * dn.nseries, st));
+        ICM_TRY(hipMemcpyAsync(h_err, err, sizeof(int), hipMemcpyDeviceToHost, st));
+        ICM_TRY(hipStreamSynchronize(st));
+        return *h_err ? ICM_ERR_DEVICE : ICM_OK;
+    }
+
+    int noise_modes_out(void* d_out, int64_t count, hipStream_t st) override
+    {
+        const int64_t n = (int64_t)(cfg.num_imfs - 1) * hn.samples;
+        if (count < n) return ICM_ERR_INVALID_ARG;
+        ICM_TRY(launch_copy<real>((real*)d_out, modes, n, st));
+        ICM_TRY(hipStreamSynchronize(st));
+        return ICM_OK;
+    }
+#undef ICM_TRY
+};
+
+}  // namespace icm
+
+using namespace icm;
+
+struct icm_plan_s {
+    PlanBase* p;
+};
+
+extern "C" {
+
+const char* icm_status_string(int s)
+{
+    switch (s) {
+    case ICM_OK: return "ok";
+    case ICM_ERR_INVALID_ARG: return "invalid argument";
+    case ICM_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case ICM_ERR_HIP: return "HIP runtime error";
+    case ICM_ERR_DEVICE: return "device kernel error";
+    case ICM_ERR_INDEX: return "sample-index array is not 0..N-1";
+    case ICM_ERR_STATE: return "call not valid in this plan state";
+    default: return "unknown status";
+    }
+}
+
+int icm_config_default(icm_config* c)
+{
+    if (!c) return ICM_ERR_INVALID_ARG;
+    std::memset(c, 0, sizeof(*c));
+    c->dtype = ICM_DTYPE_F64;
+    c->num_channels = 1;
+    c->num_noise = 100;
+    c->num_imfs = 10;
+    c->stop_mode = ICM_STOP_FIXED;
+    c->max_sifts = 10;
+    c->noise_max_sifts = 10;
+    c->noise_strength = 0.2;
+    c->seed = 1234;
+    c->use_graph = 1;
+    return ICM_OK;
+}
+
+int icm_validate_config(const icm_config* c) { return validate(c); }
+
+int icm_plan_bytes(const icm_config* c, int64_t* bytes)
+{
+    int rc = validate(c);
+    if (rc) return rc;
+    if (!bytes) return ICM_ERR_INVALID_ARG;
+    *bytes = plan_bytes_estimate(c);
+    return ICM_OK;
+}
+
+int icm_plan_create(const icm_config* c, icm_plan* out)
+{
+    if (!out) return ICM_ERR_INVALID_ARG;
+    *out = nullptr;
+    int rc = validate(c);
+    if (rc) return rc;
+    PlanBase* p = nullptr;
+    if (c->dtype == ICM_DTYPE_F64) p = new (std::nothrow) Plan<double>();
+    else p = new (std::nothrow) Plan<float>();
+    if (!p) return ICM_ERR_OUT_OF_MEMORY;
+    p->cfg = *c;
+    layout(*p);               // copies the lengths; the caller's array is not kept
+    p->cfg.lengths = nullptr;
+    if (c->dtype == ICM_DTYPE_F64) rc = static_cast<Plan<double>*>(p)->setup();
+    else rc = static_cast<Plan<float>*>(p)->setup();
+    if (rc) {
+        p->free_all();
+        delete p;
+        return rc;
+    }
+    icm_plan h = new (std::nothrow) icm_plan_s;
+    if (!h) {
+        p->free_all();
+        delete p;
+        return ICM_ERR_OUT_OF_MEMORY;
+    }
+    h->p = p;
+    *out = h;
+    return ICM_OK;
+}
+
+int icm_plan_destroy(icm_plan plan)
+{
+    if (!plan) return ICM_ERR_INVALID_ARG;
+    plan->p->free_all();
+    delete plan->p;
+    delete plan;
+    return ICM_OK;
+}
+
+int icm_plan_get_info(icm_plan plan, icm_plan_info* info)
+{
+    if (!plan || !info) return ICM_ERR_INVALID_ARG;
+    PlanBase& P = *plan->p;
+    info->device_bytes = P.bytes;
+    info->noise_series = (int64_t)P.hn.len.size();
+    info->signal_series = (int64_t)P.hsig.len.size();
+    info->noise_samples = P.hn.samples;
+    info->signal_samples = P.hsig.samples;
+    info->tiles_noise = P.hn.ntiles;
+    info->tiles_signal = P.hsig.ntiles;
+    info->tile = tile_of(P.cfg.dtype);
+    info->solve_rows_per_lane = solve_B(P.cfg.dtype);
+    info->solve_halo = solve_H(P.cfg.dtype);
+    return ICM_OK;
+}
+
+int icm_forward(icm_plan plan, const void* const* d_signals, void* const* d_imfs,
+                const void* d_noise, void* stream)
+{
+    if (!plan || !d_signals || !d_imfs) return ICM_ERR_INVALID_ARG;
+    return plan->p->forward(d_signals, d_imfs, d_noise, (hipStream_t)stream);
+}
+
+int icm_last_timing(icm_plan plan, icm_timing* out)
+{
+    if (!plan || !out) return ICM_ERR_INVALID_ARG;
+    *out = plan->p->timing;
+    return ICM_OK;
+}
+
+int icm_set_trace(icm_plan plan, int passes)
+{
+    if (!plan || passes < 0) return ICM_ERR_INVALID_ARG;
+    PlanBase& P = *plan->p;
+    P.trace_passes = passes;
+    const int M = P.cfg.num_imfs;
+    if (passes > 0) {
+        const int64_t nn = (int64_t)(M - 1) * passes * 2 * (int64_t)P.hn.len.size();
+        const int64_t ns = (int64_t)(M - 1) * passes * 2 * (int64_t)P.hsig.len.size();
+        if (P.alloc(&P.trace[0], nn) || P.alloc(&P.trace[1], ns)) return ICM_ERR_OUT_OF_MEMORY;
+        (void)hipMemset(P.trace[0], 0, nn * 4);
+        (void)hipMemset(P.trace[1], 0, ns * 4);
+    }
+    if (P.cfg.stop_mode == ICM_STOP_RILLING && !P.stops[0]) {
+        if (P.alloc(&P.stops[0], (int64_t)(M - 1) * P.hn.len.size()) ||
+            P.alloc(&P.stops[1], (int64_t)(M - 1) * P.hsig.len.size()))
+            return ICM_ERR_OUT_OF_MEMORY;
+    }
+    return ICM_OK;
+}
+
+int icm_get_trace(icm_plan plan, int phase, int32_t* host_out, int64_t count)
+{
+    if (!plan || !host_out || (phase != 0 && phase != 1)) return ICM_ERR_INVALID_ARG;
+    PlanBase& P = *plan->p;
+    if (!P.trace[phase]) return ICM_ERR_STATE;
+    const int64_t nser = phase == 0 ? (int64_t)P.hn.len.size() : (int64_t)P.hsig.len.size();
+    const int64_t n = (int64_t)(P.cfg.num_imfs - 1) * P.trace_passes * 2 * nser;
+    if (count < n) return ICM_ERR_INVALID_ARG;
+    if (hipMemcpy(host_out, P.trace[phase], n * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        return ICM_ERR_HIP;
+    return ICM_OK;
+}
+
+int icm_get_stops(icm_plan plan, int phase, int32_t* host_out, int64_t count)
+{
+    if (!plan || !host_out || (phase != 0 && phase != 1)) return ICM_ERR_INVALID_ARG;
+    PlanBase& P = *plan->p;
+    if (!P.stops[phase]) return ICM_ERR_STATE;
+    const int64_t nser = phase == 0 ? (int64_t)P.hn.len.size() : (int64_t)P.hsig.len.size();
+    const int64_t n = (int64_t)(P.cfg.num_imfs - 1) * nser;
+    if (count < n) return ICM_ERR_INVALID_ARG;
+    if (hipMemcpy(host_out, P.stops[phase], n * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        return ICM_ERR_HIP;
+    return ICM_OK;
+}
+
+int icm_get_noise_modes(icm_plan plan, void* d_out, int64_t count, void* stream)
+{
+    if (!plan || !d_out) return ICM_ERR_INVALID_ARG;
+    return plan->p->noise_modes_out(d_out, count, (hipStream_t)stream);
+}
+
+int icm_sift_pass(int dtype, const void* d_h, int64_t N, int64_t I, void* d_h_new, int32_t* d_nk,
+                  void* stream)
+{
+    if (!d_h || !d_h_new) return ICM_ERR_INVALID_ARG;
+    icm_config c;
+    icm_config_default(&c);
+    c.dtype = dtype;
+    c.num_channels = 1;
+    int64_t len = N;
+    c.lengths = &len;
+    c.num_noise = (int)I;
+    c.num_imfs = 2;
+    c.max_sifts = 1;
+    c.noise_max_sifts = 1;
+    c.use_graph = 0;
+    if (I < 1 || I > (1 << 20)) return ICM_ERR_INVALID_ARG;
+    icm_plan plan = nullptr;
+    int rc = icm_plan_create(&c, &plan);
+    if (rc) return rc;
+    if (dtype == ICM_DTYPE_F64)
+        rc = static_cast<Plan<double>*>(plan->p)->sift_once((const double*)d_h, (double*)d_h_new,
+                                                            d_nk, (hipStream_t)stream);
+    else
+        rc = static_cast<Plan<float>*>(plan->p)->sift_once((const float*)d_h, (float*)d_h_new,
+                                                           d_nk, (hipStream_t)stream);
+    icm_plan_destroy(plan);
+    return rc;
+}
+
+int icm_noise(int dtype, uint64_t seed, int64_t I, int64_t N, void* d_out, void* stream)
+{
+    if (!d_out || I < 1 || N < 1 || I > 65535 || N > kMaxLen) return ICM_ERR_INVALID_ARG;
+    hipError_t e;
+    if (dtype == ICM_DTYPE_F64)
+        e = launch_noise<double>(seed, (int)I, N, (double*)d_out, N, (hipStream_t)stream);
+    else if (dtype == ICM_DTYPE_F32)
+        e = launch_noise<float>(seed, (int)I, N, (float*)d_out, N, (hipStream_t)stream);
+    else
+        return ICM_ERR_INVALID_ARG;
+    if (e != hipSuccess) return ICM_ERR_HIP;
+    if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return ICM_ERR_HIP;
+    return ICM_OK;
+}
+
+int icm_backward(icm_plan plan, const void* const* d_signals, const void* const* d_grad_imfs,
+                 void* const* d_grad_signals, const void* d_noise, void* stream)
+{
+    (void)d_signals; (void)d_grad_imfs; (void)d_grad_signals; (void)d_noise; (void)stream;
+    if (!plan) return ICM_ERR_INVALID_ARG;
+    return ICM_ERR_STATE;
+}
+
+}  // extern "C"

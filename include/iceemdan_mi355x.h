@@ -1,0 +1,137 @@
+/*
+ * C ABI of the MI355X-native Improved CEEMDAN library (libiceemdan_mi355x.so).
+ *
+ * All data pointers named d_* are device pointers owned by the caller.
+ * Streams are passed as `void*` (a hipStream_t; NULL = the null stream).
+ * Every entry point returns an ICM_* status code (0 = success).
+ *
+ * Reference interfaces replaced (EEGLab-Pannon/CEEMDAN-GPU):
+ *   icm_plan_create/icm_forward/icm_plan_destroy
+ *       one call of iceemdanS / iceemdanD (cudaICEEMDAN.cuh:25-26, defined
+ *       cudaICEEMDAN.cu:619 and :1340), generalised to many channels of
+ *       uneven length and to a reusable plan (allocation hoisted out of the
+ *       call, cudaICEEMDAN.cu:621-765).
+ *   icm_noise         curandGenerateNormal(Double) (cudaICEEMDAN.cu:739-741,
+ *                     :1460-1462), replaced by Philox4x64-10 + Box-Muller.
+ *   icm_sift_pass     one iteration of the sifting loop body
+ *                     (cudaICEEMDAN.cu:1031-1137) for I series of length N.
+ *   icm_backward      no reference counterpart (adjoint of the decomposition).
+ * The drop-in C++ entry points iceemdanS/iceemdanD are declared in
+ * include/cudaICEEMDAN.h and the statistics kernels in include/statistics.h.
+ */
+#ifndef ICEEMDAN_MI355X_H
+#define ICEEMDAN_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ICM_OK 0
+#define ICM_ERR_INVALID_ARG (-1)
+#define ICM_ERR_OUT_OF_MEMORY (-2)
+#define ICM_ERR_HIP (-3)
+#define ICM_ERR_DEVICE (-4)      /* a kernel raised its device error word */
+#define ICM_ERR_INDEX (-5)       /* sample-index array is not 0..N-1 */
+#define ICM_ERR_STATE (-6)       /* call not valid in the plan's state */
+
+#define ICM_DTYPE_F32 0
+#define ICM_DTYPE_F64 1
+
+#define ICM_STOP_FIXED 0         /* exactly max_sifts subtractions per stage */
+#define ICM_STOP_RILLING 1       /* reference stop rule, capped at max_sifts */
+
+#define ICM_PHASE_NOISE 0
+#define ICM_PHASE_SIGNAL 1
+
+typedef struct icm_config {
+    int dtype;                  /* ICM_DTYPE_F32 / ICM_DTYPE_F64 */
+    int num_channels;           /* C >= 1 */
+    const int64_t* lengths;     /* host array [C]: samples per channel (4 .. 16,000,000) */
+    int num_noise;              /* I >= 1: noise realizations per channel */
+    int num_imfs;               /* M >= 2: output rows (M-1 sifted stages + residue) */
+    int stop_mode;              /* ICM_STOP_FIXED / ICM_STOP_RILLING */
+    int max_sifts;              /* FIXED: sifts per stage; RILLING: cap (reference max_iter) */
+    int noise_max_sifts;        /* the same for the noise-mode stages (reference: 2000) */
+    double noise_strength;      /* eps0 */
+    uint64_t seed;              /* Philox key of the white noise */
+    int use_graph;              /* FIXED only: replay the forward as a captured hipGraph */
+} icm_config;
+
+typedef struct icm_plan_s* icm_plan;
+
+typedef struct icm_plan_info {
+    int64_t device_bytes;       /* bytes allocated by the plan */
+    int64_t noise_series;       /* noise-phase series (noise groups x I) */
+    int64_t signal_series;      /* signal-phase series (C x I) */
+    int64_t noise_samples;      /* samples per noise-phase pass */
+    int64_t signal_samples;     /* samples per signal-phase pass */
+    int64_t tiles_noise;
+    int64_t tiles_signal;
+    int32_t tile;               /* samples per tile */
+    int32_t solve_rows_per_lane;/* B of the windowed solve */
+    int32_t solve_halo;         /* H of the windowed solve */
+} icm_plan_info;
+
+typedef struct icm_timing {
+    double noise_ms;            /* noise generation + noise-mode stages */
+    double signal_ms;           /* CEEMDAN stages (reference timer, cudaICEEMDAN.cu:1002-1297) */
+    double total_ms;            /* whole forward, input copy to output copy */
+} icm_timing;
+
+const char* icm_status_string(int status);
+
+/* Host-only checks (no device calls). */
+int icm_config_default(icm_config* cfg);
+int icm_validate_config(const icm_config* cfg);
+int icm_plan_bytes(const icm_config* cfg, int64_t* bytes);
+
+/* Plan lifecycle.  A plan owns all scratch; it is reentrant across plans but
+ * a single plan must not run two forwards concurrently. */
+int icm_plan_create(const icm_config* cfg, icm_plan* out);
+int icm_plan_destroy(icm_plan plan);
+int icm_plan_get_info(icm_plan plan, icm_plan_info* info);
+
+/* Forward decomposition.
+ * d_signals[c]: channel c, lengths[c] samples of dtype.
+ * d_imfs[c]:    channel c output, [num_imfs][lengths[c]] row-major; rows
+ *               0..M-2 are the modes, row M-1 the residue, all in units of
+ *               x / std(x) (cudaICEEMDAN.cu:1013-1020, :1298).
+ * d_noise:      optional [I][N] standard normals (all channels the same
+ *               length N); NULL = generate from cfg.seed.
+ * Synchronous: returns after the result is complete. */
+int icm_forward(icm_plan plan, const void* const* d_signals, void* const* d_imfs,
+                const void* d_noise, void* stream);
+int icm_last_timing(icm_plan plan, icm_timing* out);
+
+/* Per-pass knot counts (debug/parity): enable before a forward with the
+ * number of passes to keep per stage; read back [M-1][passes][series][2]
+ * (n_max, n_min of each pass input) of a phase. */
+int icm_set_trace(icm_plan plan, int passes);
+int icm_get_trace(icm_plan plan, int phase, int32_t* host_out, int64_t count);
+/* RILLING: subtractions performed per [M-1][series] of a phase. */
+int icm_get_stops(icm_plan plan, int phase, int32_t* host_out, int64_t count);
+/* noise modes E_1..E_{M-1} of the last forward: [M-1][noise series][N] */
+int icm_get_noise_modes(icm_plan plan, void* d_out, int64_t count, void* stream);
+
+/* Standard-normal generator: out[i][p], i < I, p < N (Philox4x64-10, key
+ * (seed, 0), counter (p/4, i, 0, 0), Box-Muller). */
+int icm_noise(int dtype, uint64_t seed, int64_t I, int64_t N, void* d_out, void* stream);
+
+/* One sift pass over I series of length N stored back to back:
+ * d_h_new = d_h - mean envelope; d_nk[2*i] / d_nk[2*i+1] = knot counts
+ * (maxima, minima) of series i of d_h (may be NULL). */
+int icm_sift_pass(int dtype, const void* d_h, int64_t N, int64_t I, void* d_h_new, int32_t* d_nk,
+                  void* stream);
+
+/* Adjoint of the FIXED-mode forward (no reference counterpart): given the
+ * forward's inputs and dL/dIMF, writes dL/dx for every channel.  Extrema
+ * sets are those of the forward (recomputed, not stored). */
+int icm_backward(icm_plan plan, const void* const* d_signals, const void* const* d_grad_imfs,
+                 void* const* d_grad_signals, const void* d_noise, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,102 @@
+"""GPU parity of the HIP path against the FP64 CPU oracle.
+
+FP64: bit-identical results are expected (same arithmetic order, no FMA
+contraction on either side); the assertions use exact equality for knot
+counts and a 1e-12 relative bound for samples (the north-star bar is 1e-9).
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _eeg(name, n=None):
+    x = np.fromfile(os.path.join(GOLD, name), dtype="<f4").astype(np.float64)
+    return x if n is None else x[:n]
+
+
+def _sig():
+    import importlib
+    return importlib.import_module("ceemdan-gpu_amd.signals")
+
+
+def relerr(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def test_noise_matches_oracle(icm, orc, dev):
+    import torch
+    print("case: noise I=7 N=1001 seed=1", flush=True)
+    w = icm.noise(7, 1001, seed=1, dtype=torch.float64, device=dev).cpu().numpy()
+    ref = orc.noise(1, 7, 1001)
+    # identical Philox bits; Box-Muller's log/sin/cos come from different libms
+    assert np.max(np.abs(w - ref)) < 1e-13
+    w32 = icm.noise(7, 1001, seed=1, dtype=torch.float32, device=dev).cpu().numpy()
+    assert np.array_equal(w32, ref.astype(np.float32)) or np.max(np.abs(w32 - ref)) < 1e-6
+
+
+@pytest.mark.parametrize("case", ["white", "eeg", "sine", "short", "plateau"])
+def test_sift_pass_fp64_bitexact(icm, orc, dev, case):
+    import torch
+    rng = np.random.default_rng(11)
+    if case == "white":
+        h = rng.standard_normal((5, 3000))
+    elif case == "eeg":
+        h = np.stack([_eeg("eegSampleDataCH4.f32", 30504)[i * 5000:(i + 1) * 5000 + 77]
+                      for i in range(4)])
+    elif case == "sine":
+        n = np.arange(2600)
+        h = np.stack([np.sin(2 * np.pi * n / p) + 0.01 * rng.standard_normal(n.size)
+                      for p in (7.3, 40.0, 900.0)])
+    elif case == "short":
+        h = rng.standard_normal((6, 9))
+    else:
+        h = np.round(rng.standard_normal((3, 2000)) * 2) / 2   # many equal neighbours
+    print(f"case: sift_pass fp64 {case} shape={h.shape}", flush=True)
+    hd = torch.tensor(h, dtype=torch.float64, device=dev)
+    hn, nk = icm.sift_pass(hd)
+    hn = hn.cpu().numpy()
+    nk = nk.cpu().numpy()
+    for i in range(h.shape[0]):
+        ref, nmax, nmin, _, _ = orc.sift_pass(h[i])
+        assert (nk[i, 0], nk[i, 1]) == (nmax, nmin), i
+        assert np.array_equal(hn[i], ref), (i, relerr(hn[i], ref))
+
+
+def test_sift_pass_long_series(icm, orc, dev):
+    """Many tiles and several solve windows per series (EEG CH1, 244032 samples)."""
+    import torch
+    x = _eeg("eegSampleDataCH1_244032.f32")
+    h = np.stack([x, x[::-1].copy()])
+    print(f"case: sift_pass fp64 long shape={h.shape}", flush=True)
+    hn, nk = icm.sift_pass(torch.tensor(h, dtype=torch.float64, device=dev))
+    hn = hn.cpu().numpy()
+    nk = nk.cpu().numpy()
+    for i in range(2):
+        ref, nmax, nmin, _, _ = orc.sift_pass(h[i])
+        assert (nk[i, 0], nk[i, 1]) == (nmax, nmin)
+        assert np.array_equal(hn[i], ref), relerr(hn[i], ref)
+
+
+def test_iceemdan_fp64_small_bitexact(icm, orc, dev):
+    """Three-tone signal, N=4096, I=12, M=6, FIXED S=10, noise from the GPU."""
+    import torch
+    N, I, M, S = 4096, 12, 6, 10
+    x = _sig().three_tone(N, seed=0)
+    print(f"case: iceemdan fp64 N={N} I={I} M={M} S={S}", flush=True)
+    w = icm.noise(I, N, seed=1, dtype=torch.float64, device=dev)
+    plan = icm.Plan([N], I, M, dtype=torch.float64, max_sifts=S, use_graph=False, device=dev)
+    plan.enable_trace(S)
+    out = plan.forward([torch.tensor(x, dtype=torch.float64, device=dev)], noise=w)[0]
+    got = out.cpu().numpy()
+    ref = orc.iceemdan(x, w.cpu().numpy(), M, S_or_cap=S, trace_passes=S)
+    tr_sig = plan.trace(icm.PHASE_SIGNAL)     # [M-1][S][I][2]
+    tr_noise = plan.trace(icm.PHASE_NOISE)
+    assert np.array_equal(tr_noise, ref["trace"][0])
+    assert np.array_equal(tr_sig, ref["trace"][1])
+    assert np.array_equal(got, ref["imfs"]), relerr(got, ref["imfs"])
+    plan.close()

@@ -1,0 +1,326 @@
+"""Pins of the FP64 CPU oracle (CPU only).
+
+Each check is chosen so that a plausible mistake in the oracle (dropped
+term, wrong sign or index, transposed operand) fails it: independent
+brute force, scipy's natural cubic spline, numpy statistics, numpy's
+Philox, published known-answer vectors, and invariants of the
+decomposition.
+"""
+import os
+
+import numpy as np
+import pytest
+from scipy.interpolate import CubicSpline
+from scipy.linalg import solve_banded
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def eeg(name):
+    return np.fromfile(os.path.join(GOLD, name), dtype="<f4").astype(np.float64)
+
+
+# ------------------------------------------------------------------ extrema
+
+def brute_extrema(h, is_max):
+    N = h.size
+    out = []
+    for p in range(N):
+        if p == 0 or p == N - 1:
+            out.append(p)
+        elif is_max and h[p] > h[p - 1] and h[p] > h[p + 1]:
+            out.append(p)
+        elif not is_max and h[p] < h[p - 1] and h[p] < h[p + 1]:
+            out.append(p)
+    return np.array(out)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_extrema_brute_force(orc, seed):
+    rng = np.random.default_rng(seed)
+    for N in (4, 5, 17, 1000):
+        h = rng.standard_normal(N)
+        h[rng.integers(0, N, N // 5)] = 0.25            # plateaus / ties
+        for is_max in (True, False):
+            pos, val = orc.extrema(h, is_max)
+            ref = brute_extrema(h, is_max)
+            assert np.array_equal(pos, ref)
+            assert np.array_equal(val, h[ref])
+
+
+def test_extrema_nan_and_plateau(orc):
+    h = np.array([0.0, 1.0, np.nan, 1.0, 2.0, 2.0, 1.0, 3.0, 0.0])
+    pos, _ = orc.extrema(h, True)
+    # NaN is never an extremum, a flat top (2,2) is not strict, endpoints forced
+    assert list(pos) == [0, 7, 8]
+    pos, _ = orc.extrema(h, False)
+    assert list(pos) == [0, 6, 8]
+
+
+# ------------------------------------------------------------- boundary fix
+
+def test_boundary_fix_hand_cases(orc):
+    pos = np.array([0, 2, 4, 7, 9, 12], np.int32)
+    val = np.array([0.0, 1.0, 3.0, 2.0, 2.0, 1.0])
+    # left: line through (2,1),(4,3) -> slope 1 -> at 0: -1 ; max keeps 0.0
+    # right: line through (7,2),(9,2) -> 2 at 12 ; max replaces 1.0 by 2.0
+    out = orc.boundary_fix(pos, val, True)
+    assert out[0] == 0.0 and out[-1] == 2.0
+    out = orc.boundary_fix(pos, val, False)
+    assert out[0] == -1.0 and out[-1] == 1.0
+    # n < 6: untouched
+    out = orc.boundary_fix(pos[:5], val[:5], True)
+    assert np.array_equal(out, val[:5])
+
+
+# ------------------------------------------------------------------ spline
+
+def natural_system(pos, val):
+    n = pos.size
+    ab = np.zeros((3, n))
+    f = np.zeros(n)
+    ab[1, :] = 1.0
+    for g in range(1, n - 1):
+        hl = pos[g] - pos[g - 1]
+        hr = pos[g + 1] - pos[g]
+        ab[2, g - 1] = hl          # sub-diagonal
+        ab[1, g] = 2 * (hl + hr)
+        ab[0, g + 1] = hr          # super-diagonal
+        f[g] = 3 * ((val[g + 1] - val[g]) / hr - (val[g] - val[g - 1]) / hl)
+    return ab, f
+
+
+@pytest.mark.parametrize("src", ["white", "eeg", "sparse"])
+def test_spline_solve_vs_scipy(orc, src):
+    rng = np.random.default_rng(5)
+    if src == "white":
+        h = rng.standard_normal(30000)
+    elif src == "eeg":
+        h = eeg("eegSampleDataCH4.f32")
+    else:
+        h = np.cumsum(rng.standard_normal(20000))       # few, uneven knots
+    pos, val = orc.extrema(h, True)
+    c = orc.spline_solve(pos, val)
+    ab, f = natural_system(pos, val)
+    c_banded = solve_banded((1, 1), ab, f)
+    cs = CubicSpline(pos.astype(float), val, bc_type="natural")
+    scale = np.max(np.abs(c_banded))
+    assert np.max(np.abs(c - c_banded)) <= 1e-13 * scale
+    assert np.max(np.abs(c[:-1] - cs.c[1])) <= 1e-13 * scale
+    assert c[0] == 0.0 and c[-1] == 0.0
+    # residual of the tridiagonal system
+    Ac = ab[1] * c
+    Ac[1:] += ab[2, :-1] * c[:-1]
+    Ac[:-1] += ab[0, 1:] * c[1:]
+    assert np.max(np.abs(Ac - f)) <= 1e-12 * np.max(np.abs(f))
+
+
+def test_spline_window_truncation_negligible(orc):
+    """The windowed solve equals a global Thomas solve to round-off on EEG data
+    (several windows: n >> 640)."""
+    h = eeg("eegSampleDataCH1_244032.f32")
+    for is_max in (True, False):
+        pos, val = orc.extrema(h, is_max)
+        assert pos.size > 10 * 640
+        c = orc.spline_solve(pos, val)
+        ct = orc.spline_solve_thomas(pos, val)
+        assert np.max(np.abs(c - ct)) <= 1e-14 * np.max(np.abs(ct))
+
+
+def test_spline_window_single_window_is_global(orc):
+    rng = np.random.default_rng(9)
+    pos = np.cumsum(rng.integers(1, 9, 300)).astype(np.int32)
+    pos[0] = 0
+    val = rng.standard_normal(pos.size)
+    c = orc.spline_solve(pos, val)
+    ct = orc.spline_solve_thomas(pos, val)
+    assert np.max(np.abs(c - ct)) <= 1e-15 * np.max(np.abs(ct)) * 10
+
+
+def test_envelope_vs_scipy_and_interpolation(orc):
+    rng = np.random.default_rng(3)
+    h = rng.standard_normal(5000)
+    pos, val = orc.extrema(h, False)
+    c = orc.spline_solve(pos, val)
+    env = orc.envelope(h.size, pos, val, c)
+    cs = CubicSpline(pos.astype(float), val, bc_type="natural")
+    ref = cs(np.arange(h.size))
+    assert np.max(np.abs(env - ref)) <= 1e-12 * np.max(np.abs(ref))
+    assert np.allclose(env[pos], val, rtol=0, atol=1e-12)
+    assert env[0] == val[0]
+
+
+def test_envelope_exact_on_linear_data(orc):
+    pos = np.array([0, 3, 7, 8, 15, 20], np.int32)
+    val = 2.0 + 0.5 * pos
+    c = orc.spline_solve(pos, val)
+    env = orc.envelope(21, pos, val, c)
+    assert np.max(np.abs(c)) < 1e-15
+    assert np.allclose(env, 2.0 + 0.5 * np.arange(21), rtol=0, atol=1e-13)
+
+
+# --------------------------------------------------------------- sift pass
+
+def test_sift_pass_independent(orc):
+    """h_new = h - (U+L)/2 with U, L from scipy natural splines through the
+    boundary-fixed extrema."""
+    rng = np.random.default_rng(4)
+    h = np.sin(np.arange(3000) / 9.0) + 0.3 * rng.standard_normal(3000)
+    hn, nmax, nmin, cnt, mark = orc.sift_pass(h)
+    env = []
+    for is_max in (True, False):
+        p = brute_extrema(h, is_max)
+        v = h[p].copy()
+        n = p.size
+        if n >= 6:
+            t = v[1] - (v[2] - v[1]) / (p[2] - p[1]) * (p[1] - p[0])
+            tr = v[n - 2] + (v[n - 2] - v[n - 3]) / (p[n - 2] - p[n - 3]) * (p[n - 1] - p[n - 2])
+            if is_max:
+                v[0], v[-1] = max(v[0], t), max(v[-1], tr)
+            else:
+                v[0], v[-1] = min(v[0], t), min(v[-1], tr)
+        env.append(CubicSpline(p.astype(float), v, bc_type="natural")(np.arange(h.size)))
+    U, L = env
+    assert (nmax, nmin) == (brute_extrema(h, True).size, brute_extrema(h, False).size)
+    assert np.max(np.abs(hn - (h - (U + L) / 2))) < 1e-11
+    sx = np.abs((U + L) / (U - L))
+    assert abs(cnt - int(np.sum(sx > 0.05))) <= 2
+    assert mark == int(np.any(sx > 0.5))
+
+
+def test_sift_pass_few_knots_no_subtraction(orc):
+    h = np.linspace(0.0, 1.0, 100) ** 2            # monotone: only the endpoints
+    hn, nmax, nmin, cnt, mark = orc.sift_pass(h)
+    assert nmax == 2 and nmin == 2
+    assert np.array_equal(hn, h)
+
+
+def test_rilling_truth_table(orc):
+    N = 1000
+    # frac = cnt/N > 0.05 continues regardless of the rest
+    assert orc.rilling_continue(51, 0, 2, 2, N)
+    assert not orc.rilling_continue(50, 0, 100, 100, N)
+    # any(sx > 0.5) continues only with more than 6 extrema in total
+    assert orc.rilling_continue(0, 1, 4, 3, N)
+    assert not orc.rilling_continue(0, 1, 3, 3, N)
+    assert not orc.rilling_continue(0, 0, 50, 50, N)
+
+
+# ---------------------------------------------------------------- statistics
+
+def test_canonical_sum_and_var(orc):
+    rng = np.random.default_rng(8)
+    for N in (2, 5, 1023, 1024, 1025, 244032):
+        x = rng.standard_normal(N) * 3 + 1
+        assert abs(orc.sum_canon(x) - np.sum(x)) <= 1e-12 * np.sum(np.abs(x))
+        assert abs(orc.var_canon(x) - np.var(x, ddof=1)) <= 1e-12 * np.var(x)
+    # exact order check against a hand restatement of the canonical order
+    x = rng.standard_normal(3000)
+    acc = np.zeros(1024)
+    for p in range(3000):
+        acc[p % 1024] += x[p]
+    s = 512
+    while s >= 1:
+        acc[:s] = acc[:s] + acc[s:2 * s]
+        s //= 2
+    assert orc.sum_canon(x) == acc[0]
+
+
+# ------------------------------------------------------------------- Philox
+
+KAT = [  # Random123 philox4x64_10 known-answer vectors
+    ([0, 0, 0, 0], [0, 0],
+     [0x16554d9eca36314c, 0xdb20fe9d672d0fdc, 0xd7e772cee186176b, 0x7e68b68aec7ba23b]),
+    ([2**64 - 1] * 4, [2**64 - 1] * 2,
+     [0x87b092c3013fe90b, 0x438c3c67be8d0224, 0x9cc7d7c69cd777b6, 0xa09caebf594f0ba0]),
+    ([0x243f6a8885a308d3, 0x13198a2e03707344, 0xa4093822299f31d0, 0x082efa98ec4e6c89],
+     [0x452821e638d01377, 0xbe5466cf34e90c6c],
+     [0xa528f45403e61d95, 0x38c72dbd566e9788, 0xa5a1610e72fd18b5, 0x57bd43b5e52b7fe6]),
+]
+
+
+@pytest.mark.parametrize("ctr,key,out", KAT)
+def test_philox_known_answers(orc, ctr, key, out):
+    assert orc.philox4x64(ctr, key) == out
+
+
+def test_philox_matches_numpy(orc):
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        ctr = [int(v) for v in rng.integers(0, 2**63, 4, dtype=np.int64)]
+        key = [int(v) for v in rng.integers(0, 2**63, 2, dtype=np.int64)]
+        # numpy increments the counter before producing the first block
+        c = list(ctr)
+        c[0] = (c[0] - 1) % 2**64
+        if ctr[0] == 0:
+            c[1] = (c[1] - 1) % 2**64
+        bg = np.random.Philox(key=np.array(key, dtype=np.uint64),
+                              counter=np.array(c, dtype=np.uint64))
+        assert [int(v) for v in bg.random_raw(4)] == orc.philox4x64(ctr, key)
+
+
+def test_noise_statistics(orc):
+    w = orc.noise(1, 8, 50000)
+    assert abs(w.mean()) < 0.01
+    assert abs(w.std() - 1.0) < 0.01
+    # counter depends on (sample, realization) only: prefix-stable in N
+    w2 = orc.noise(1, 8, 777)
+    assert np.array_equal(w2, w[:, :777])
+    assert not np.array_equal(orc.noise(2, 1, 100), w[:1, :100])
+
+
+# ----------------------------------------------------------------- ICEEMDAN
+
+def three_tone(N, seed=0):
+    t = np.arange(N) / 1000.0
+    rng = np.random.default_rng(seed)
+    return (np.sin(2 * np.pi * 5 * t) + 0.5 * np.sin(2 * np.pi * 20 * t)
+            + 0.25 * np.sin(2 * np.pi * 60 * t) + 0.1 * rng.standard_normal(N))
+
+
+def test_iceemdan_completeness(orc):
+    N, I, M = 4096, 20, 8
+    x = three_tone(N)
+    w = orc.noise(1, I, N)
+    imfs = orc.iceemdan(x, w, M, S_or_cap=10)["imfs"]
+    xt = x / np.sqrt(np.var(x, ddof=1))
+    assert np.max(np.abs(imfs.sum(0) - xt)) < 1e-12
+    # the 60 Hz tone lands in the first modes, the 5 Hz tone in a later one
+    power = (imfs[:-1] ** 2).sum(1)
+    assert np.argmax(power) >= 2
+
+
+def test_iceemdan_eps0_zero_is_noise_free(orc):
+    """eps0 = 0: every realization sifts x~ itself, so the result does not
+    depend on I or on the noise (cudaICEEMDAN.cu:76,88 with eps0 = 0)."""
+    N, M = 2048, 5
+    x = three_tone(N, seed=3)
+    a = orc.iceemdan(x, orc.noise(1, 3, N), M, S_or_cap=10, eps0=0.0)["imfs"]
+    b = orc.iceemdan(x, orc.noise(7, 9, N), M, S_or_cap=10, eps0=0.0)["imfs"]
+    assert np.max(np.abs(a - b)) < 1e-13
+
+
+def test_iceemdan_rilling_stops(orc):
+    N, I, M = 1000, 10, 3
+    n = np.arange(N)
+    x = np.sin(2 * np.pi * 0.065 * n) + np.where((n >= 500) & (n <= 748),
+                                                  np.sin(2 * np.pi * 0.255 * (n - 500)), 0)
+    r = orc.iceemdan(x, orc.noise(1234, I, N), M, stop_mode=orc.STOP_RILLING, S_or_cap=100)
+    stops = r["stops"]
+    assert stops.min() >= 0 and stops.max() <= 99
+    imfs = r["imfs"]
+    xt = x / np.sqrt(np.var(x, ddof=1))
+    assert np.max(np.abs(imfs.sum(0) - xt)) < 1e-12
+
+
+def test_golden_fixtures(orc):
+    """The oracle reproduces its committed outputs bit for bit."""
+    path = os.path.join(GOLD, "oracle_fixtures.npz")
+    if not os.path.exists(path):
+        pytest.skip("fixtures not generated")
+    from tests.golden.make_fixtures import CASES, run_case
+    data = np.load(path, allow_pickle=False)
+    for name in CASES:
+        got = run_case(orc, name)
+        for k, v in got.items():
+            assert np.array_equal(data[f"{name}/{k}"], v), (name, k)
