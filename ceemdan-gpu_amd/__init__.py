@@ -66,6 +66,9 @@ _SIGS = {
                               C.c_void_p, C.c_void_p]),
     "icm_last_timing": (C.c_int, [C.c_void_p, C.POINTER(icm_timing)]),
     "icm_set_trace": (C.c_int, [C.c_void_p, C.c_int]),
+    "icm_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "icm_get_kernel_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double),
+                                       C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "icm_get_trace": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int32), C.c_int64]),
     "icm_get_stops": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int32), C.c_int64]),
     "icm_get_noise_modes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
@@ -263,6 +266,18 @@ class Plan:
         _check(lib().icm_forward(self._h, sp, op,
                                  C.c_void_p(noise.data_ptr()) if noise is not None else None, st),
                "icm_forward")
+        return out
+
+    def set_profiling(self, on):
+        _check(lib().icm_set_profiling(self._h, 1 if on else 0), "icm_set_profiling")
+
+    def kernel_stats(self):
+        """{kind: (total_ms, launches, algorithmic_bytes)} of the last forward."""
+        out = {}
+        for k, name in ((0, "init"), (1, "solve"), (2, "sift")):
+            ms, n, b = C.c_double(), C.c_int64(), C.c_int64()
+            _check(lib().icm_get_kernel_stats(self._h, k, C.byref(ms), C.byref(n), C.byref(b)))
+            out[name] = (ms.value, n.value, b.value)
         return out
 
     def timing(self):

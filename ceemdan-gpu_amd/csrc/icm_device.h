@@ -29,38 +29,6 @@ __device__ __forceinline__ T shfl_up(T v, int d) { return __shfl_up(v, (unsigned
 template <typename T>
 __device__ __forceinline__ T shfl_down(T v, int d) { return __shfl_down(v, (unsigned)d, kWave); }
 
-// ----------------------------------------------------------------- look-back
-// Tile status word of the single-pass knot compaction:
-//   [63:48] epoch (launch id), [47:46] flag (1 = local count, 2 = inclusive),
-//   [45:23] count of maxima, [22:0] count of minima.
-constexpr uint32_t kFlagAggregate = 1;
-constexpr uint32_t kFlagInclusive = 2;
-
-__device__ __forceinline__ uint64_t pack_status(uint32_t epoch, uint32_t flag, uint32_t cmax,
-                                                uint32_t cmin)
-{
-    return ((uint64_t)(epoch & 0xFFFFu) << 48) | ((uint64_t)flag << 46) |
-           ((uint64_t)(cmax & 0x7FFFFFu) << 23) | (uint64_t)(cmin & 0x7FFFFFu);
-}
-__device__ __forceinline__ uint32_t st_epoch(uint64_t w) { return (uint32_t)(w >> 48); }
-__device__ __forceinline__ uint32_t st_flag(uint64_t w) { return (uint32_t)(w >> 46) & 3u; }
-__device__ __forceinline__ uint32_t st_cmax(uint64_t w) { return (uint32_t)(w >> 23) & 0x7FFFFFu; }
-__device__ __forceinline__ uint32_t st_cmin(uint64_t w) { return (uint32_t)w & 0x7FFFFFu; }
-
-__device__ __forceinline__ void status_store(uint64_t* p, uint64_t w)
-{
-    __hip_atomic_store(p, w, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ uint64_t status_load(const uint64_t* p)
-{
-    return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Bounded spin: a predecessor tile always finishes (tiles are taken in
-// ticket order, so every predecessor is already resident), but the wait is
-// still capped so that a logic error reports instead of hanging the device.
-constexpr uint32_t kSpinCap = 1u << 24;
-
 // ------------------------------------------------------------------ Philox
 struct u64x4 {
     uint64_t v[4];

@@ -17,32 +17,49 @@ namespace icm {
 constexpr int kThreads = 256;   // workgroup size of the tile kernels (4 waves)
 
 struct SeriesDev {
+    const int* tile_order;    // [ntiles] launch order -> global tile (tile-major across series)
     const int* tile_series;   // [ntiles]
     const int* tile0;         // [nseries]
     const int* ntl;           // [nseries]
     const int64_t* off;       // [nseries]
     const int* len;           // [nseries]
-    const int64_t* kbase;     // [nseries]
+    const int64_t* kbase;     // [nseries] offset of the series in the compact c array
     const int* chan;          // [nseries] channel (signal phase) or noise group (noise phase)
     const int64_t* eoff;      // [nseries] offset of the series' noise-mode row
     const int* nidx;          // [nseries] noise series whose modes this series uses
     int nseries;
     int ntiles;
-    int64_t kstride;
+    int64_t kstride;          // per-type stride of the compact c array
+    int64_t sstride;          // per-type stride of the knot slots (tiles x cap)
+    int cap;                  // knot slots per tile and type
 };
 
+// Knots of one sift-pass input, kept where the tile that found them wrote
+// them (no cross-workgroup compaction): tile gt's maxima are slots
+// [gt*cap, gt*cap + cnt[2*gt]) of type 0, its minima the same of type 1.
 template <typename real>
 struct KnotsDev {
-    int* x;      // [2][kstride]
-    real* y;     // [2][kstride]
-    int* pre;    // [ntiles][2]
-    int* n;      // [nseries][2]
+    int* x;      // [2][sstride] sample index in the series
+    real* y;     // [2][sstride] sample value
+    int* cnt;    // [ntiles][2]  knots found in the tile
+    int* pre;    // [ntiles][2]  knots before the tile in its series (k_knot_prefix)
+    int* n;      // [nseries][2] knots of the series (k_knot_prefix)
+    real* fix;   // [nseries][2][2] values of knots 0 and n-1 after the endpoint fix (k_solve)
+    int* nbx;    // [ntiles][2][3] positions of the knots pre-1, pre-2 and pre+cnt of the
+    real* nby;   //  tile (-1 = none) and their values (k_knot_prefix)
+};
+
+// A run of consecutive tiles of one series, sifted by one workgroup that
+// prefetches tile i+1 while it computes tile i.
+struct Segment {
+    int s;        // series
+    int t_first;  // first tile (within the series)
+    int count;    // tiles
+    int pad;
 };
 
 struct PassCtl {
-    uint64_t* status;      // [ntiles] look-back words
     uint32_t epoch;        // launch id (unique within one forward)
-    unsigned* tickets;     // [epochs] tile tickets
     int* winlist;          // [2 * maxwin] window items for the next solve
     unsigned* wincount;    // [epochs] number of window items
     int K;                 // rows kept per window of the solve
@@ -69,6 +86,7 @@ struct InitArgs {
 template <typename real>
 struct SiftArgs {
     SeriesDev sd;
+    const Segment* segs;    // [nsegs] one workgroup each
     const real* h_in;
     real* h_out;
     KnotsDev<real> kin;     // knots of h_in
@@ -97,10 +115,13 @@ template <typename real>
 hipError_t launch_init(const SeriesDev& sd, const InitArgs<real>& ia, real* h_out,
                        const KnotsDev<real>& kout, const PassCtl& pc, hipStream_t st);
 template <typename real>
-hipError_t launch_sift(const SiftArgs<real>& a, hipStream_t st);
+hipError_t launch_sift(const SiftArgs<real>& a, int nsegs, hipStream_t st);
 template <typename real>
 hipError_t launch_solve(const SolveArgs<real>& a, int grid, hipStream_t st);
 
+template <typename real>
+hipError_t launch_knot_prefix(const SeriesDev& sd, const KnotsDev<real>& k, const PassCtl& pc,
+                              const int* active, hipStream_t st);
 template <typename real>
 hipError_t launch_rilling_decide(const SeriesDev& sd, const int* nk, const int* sx_cnt,
                                  const int* sx_mark, int* active, int* stop_pass, int* subs,

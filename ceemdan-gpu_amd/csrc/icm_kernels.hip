@@ -32,16 +32,23 @@ struct TileGeom {
     int gt, s, t, N, t0, nt;
 };
 
-template <int T>
-__device__ __forceinline__ int take_ticket(unsigned* tickets, uint32_t epoch, int* s_slot)
+template <typename real, int T>
+__device__ __forceinline__ TileGeom<real, T> tile_geom(const SeriesDev& sd)
 {
-    if (threadIdx.x == 0) *s_slot = (int)atomicAdd(&tickets[epoch], 1u);
-    __syncthreads();
-    return *s_slot;
+    TileGeom<real, T> g;
+    g.gt = sd.tile_order[blockIdx.x];
+    g.s = sd.tile_series[g.gt];
+    g.t = g.gt - sd.tile0[g.s];
+    g.N = sd.len[g.s];
+    g.t0 = g.t * T;
+    g.nt = min(T, g.N - g.t0);
+    return g;
 }
 
 // Extrema flags over the tile samples u = p - t0 in [0, nt), from LDS
 // samples sh[u + 2] (sh covers p in [t0-2, t0+nt]).  mask[type*NCH + chunk].
+// Strict comparison with both neighbours, endpoints always knots
+// (find_extrema_shfl_max/min, cudaICEEMDAN.cu:92-148).
 template <typename real, int T>
 __device__ __forceinline__ void tile_flags(const real* sh, int N, int t0, int nt, uint64_t* mask)
 {
@@ -97,12 +104,13 @@ __device__ __forceinline__ int rank_in_tile(const uint64_t* mask, const int* pre
     return pre[type * (NCH + 1) + ch] + __popcll(below);
 }
 
-// Compact the extrema of the new samples (sh over [t0-2, t0+nt], new values
-// valid on [t0-1, t0+nt]) into the knot arrays of the next pass.
+// Knots of the new samples (sh over [t0-2, t0+nt], new values valid on
+// [t0-1, t0+nt]) into the tile's own slots: select_extrema_max/min
+// (cudaICEEMDAN.cu:150-207) without the global scan; k_knot_prefix turns the
+// per-tile counts into series offsets afterwards.
 template <typename real, int T>
-__device__ void emit_tile(const SeriesDev& sd, const KnotsDev<real>& kout, const PassCtl& pc,
-                          const TileGeom<real, T>& g, const real* sh, uint64_t* mask, int* cpre,
-                          int* s_pre)
+__device__ void emit_tile(const SeriesDev& sd, const KnotsDev<real>& kout,
+                          const TileGeom<real, T>& g, const real* sh, uint64_t* mask, int* cpre)
 {
     constexpr int NCH = T / 64;
     const int tid = threadIdx.x;
@@ -110,41 +118,7 @@ __device__ void emit_tile(const SeriesDev& sd, const KnotsDev<real>& kout, const
     __syncthreads();
     chunk_prefix<T>(mask, cpre);
     __syncthreads();
-    const uint32_t cmx = (uint32_t)cpre[NCH];
-    const uint32_t cmn = (uint32_t)cpre[(NCH + 1) + NCH];
-    if (tid == 0) {
-        const bool first = (g.t == 0);
-        status_store(&pc.status[g.gt],
-                     pack_status(pc.epoch, first ? kFlagInclusive : kFlagAggregate, cmx, cmn));
-        uint32_t pmx = 0, pmn = 0;
-        if (!first) {
-            int q = g.gt - 1;
-            const int qfirst = g.gt - g.t;
-            uint32_t spins = 0;
-            while (true) {
-                const uint64_t w = status_load(&pc.status[q]);
-                if (st_epoch(w) != (pc.epoch & 0xFFFFu) || st_flag(w) == 0u) {
-                    if (++spins > kSpinCap) {
-                        atomicOr(pc.err, ICM_DEVERR_SPIN);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                pmx += st_cmax(w);
-                pmn += st_cmin(w);
-                if (st_flag(w) == kFlagInclusive || q == qfirst) break;
-                --q;
-            }
-            status_store(&pc.status[g.gt],
-                         pack_status(pc.epoch, kFlagInclusive, pmx + cmx, pmn + cmn));
-        }
-        s_pre[0] = (int)pmx;
-        s_pre[1] = (int)pmn;
-    }
-    __syncthreads();
-    const int pmx = s_pre[0], pmn = s_pre[1];
-    const int64_t kb = sd.kbase[g.s];
+    const int64_t slot0 = (int64_t)g.gt * sd.cap;
 #pragma unroll
     for (int r = 0; r < T / kThreads; ++r) {
         const int u = r * kThreads + tid;
@@ -153,50 +127,154 @@ __device__ void emit_tile(const SeriesDev& sd, const KnotsDev<real>& kout, const
             const int bit = u & 63;
             const real v = sh[u + 2];
             if ((mask[ch] >> bit) & 1ull) {
-                const int64_t o = kb + pmx + rank_in_tile<T>(mask, cpre, 0, u);
+                const int64_t o = slot0 + rank_in_tile<T>(mask, cpre, 0, u);
                 kout.x[o] = g.t0 + u;
                 kout.y[o] = v;
             }
             if ((mask[NCH + ch] >> bit) & 1ull) {
-                const int64_t o = sd.kstride + kb + pmn + rank_in_tile<T>(mask, cpre, 1, u);
+                const int64_t o = sd.sstride + slot0 + rank_in_tile<T>(mask, cpre, 1, u);
                 kout.x[o] = g.t0 + u;
                 kout.y[o] = v;
             }
         }
     }
     if (tid == 0) {
-        kout.pre[2 * g.gt] = pmx;
-        kout.pre[2 * g.gt + 1] = pmn;
-        if (g.t == sd.ntl[g.s] - 1) {
-            const int nmx = pmx + (int)cmx, nmn = pmn + (int)cmn;
-            kout.n[2 * g.s] = nmx;
-            kout.n[2 * g.s + 1] = nmn;
-            const int wmx = (nmx + pc.K - 1) / pc.K;
-            const int wmn = (nmn + pc.K - 1) / pc.K;
-            const unsigned base = atomicAdd(&pc.wincount[pc.epoch], (unsigned)(wmx + wmn));
-            for (int w = 0; w < wmx; ++w) {
-                pc.winlist[2 * (base + w)] = g.s;
-                pc.winlist[2 * (base + w) + 1] = (w << 1);
-            }
-            for (int w = 0; w < wmn; ++w) {
-                pc.winlist[2 * (base + wmx + w)] = g.s;
-                pc.winlist[2 * (base + wmx + w) + 1] = (w << 1) | 1;
-            }
-        }
+        kout.cnt[2 * g.gt] = cpre[NCH];
+        kout.cnt[2 * g.gt + 1] = cpre[(NCH + 1) + NCH];
     }
 }
 
-template <typename real, int T>
-__device__ __forceinline__ TileGeom<real, T> tile_geom(const SeriesDev& sd, int gt)
+// ------------------------------------------------------------- knot prefix
+
+// Summary of the knots seen so far along a series: how many (capped at 2)
+// and the slots of the last two.  combine(A, B) = A followed by B.
+struct LastTwo {
+    int c;
+    int64_t l1, l2;
+};
+
+__device__ __forceinline__ LastTwo lt_combine(const LastTwo& A, const LastTwo& B)
 {
-    TileGeom<real, T> g;
-    g.gt = gt;
-    g.s = sd.tile_series[gt];
-    g.t = gt - sd.tile0[g.s];
-    g.N = sd.len[g.s];
-    g.t0 = g.t * T;
-    g.nt = min(T, g.N - g.t0);
-    return g;
+    if (B.c >= 2) return B;
+    if (B.c == 1) {
+        LastTwo r;
+        r.c = min(A.c + 1, 2);
+        r.l1 = B.l1;
+        r.l2 = A.l1;
+        return r;
+    }
+    return A;
+}
+
+__device__ __forceinline__ LastTwo lt_shfl_up(const LastTwo& v, int o)
+{
+    LastTwo r;
+    r.c = shfl_up(v.c, o);
+    r.l1 = shfl_up(v.l1, o);
+    r.l2 = shfl_up(v.l2, o);
+    return r;
+}
+
+// Per-series exclusive scan of the tile knot counts (the CUB ExclusiveSum of
+// cudaICEEMDAN.cu:859-883, over tiles instead of samples), the series knot
+// counts, the solve's window items, and for every tile the slots of the two
+// knots before it and of the first knot after it.  One wavefront per series.
+template <typename real>
+__global__ void k_knot_prefix(SeriesDev sd, KnotsDev<real> k, PassCtl pc, const int* active)
+{
+    const int s = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (active && !active[s]) return;
+    const int tb = sd.tile0[s], nt = sd.ntl[s];
+    const int cap = sd.cap;
+    int run[2] = {0, 0};
+    LastTwo carry[2];
+    for (int ty = 0; ty < 2; ++ty) {
+        carry[ty].c = 0;
+        carry[ty].l1 = -1;
+        carry[ty].l2 = -1;
+    }
+    for (int base = 0; base < nt; base += kWave) {
+        const int t = base + lane;
+#pragma unroll
+        for (int ty = 0; ty < 2; ++ty) {
+            const int c = t < nt ? k.cnt[2 * (tb + t) + ty] : 0;
+            int inc = c;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const int v = shfl_up(inc, o);
+                if (lane >= o) inc += v;
+            }
+            LastTwo own;
+            own.c = min(c, 2);
+            own.l1 = c >= 1 ? (int64_t)(tb + t) * cap + (c - 1) : -1;
+            own.l2 = c >= 2 ? (int64_t)(tb + t) * cap + (c - 2) : -1;
+            LastTwo scan = own;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const LastTwo v = lt_shfl_up(scan, o);
+                if (lane >= o) scan = lt_combine(v, scan);
+            }
+            LastTwo ex = lt_shfl_up(scan, 1);
+            if (lane == 0) {
+                ex.c = 0;
+                ex.l1 = -1;
+                ex.l2 = -1;
+            }
+            const LastTwo before = lt_combine(carry[ty], ex);
+            if (t < nt) {
+                k.pre[2 * (tb + t) + ty] = run[ty] + inc - c;
+                const int64_t o = 6 * (int64_t)(tb + t) + 3 * ty;
+                const int64_t so = ty ? sd.sstride : 0;
+                k.nbx[o] = before.l1 >= 0 ? k.x[so + before.l1] : -1;
+                k.nby[o] = before.l1 >= 0 ? k.y[so + before.l1] : (real)0;
+                k.nbx[o + 1] = before.l2 >= 0 ? k.x[so + before.l2] : -1;
+                k.nby[o + 1] = before.l2 >= 0 ? k.y[so + before.l2] : (real)0;
+            }
+            run[ty] += __shfl(inc, kWave - 1, kWave);
+            LastTwo last;
+            last.c = __shfl(scan.c, kWave - 1, kWave);
+            last.l1 = __shfl(scan.l1, kWave - 1, kWave);
+            last.l2 = __shfl(scan.l2, kWave - 1, kWave);
+            carry[ty] = lt_combine(carry[ty], last);
+        }
+    }
+    // first knot after each tile: suffix minimum over non-empty tiles
+    for (int ty = 0; ty < 2; ++ty) {
+        int nxt = 0x7FFFFFFF;
+        for (int top = ((nt - 1) / kWave) * kWave; top >= 0; top -= kWave) {
+            const int t = top + lane;
+            const int c = t < nt ? k.cnt[2 * (tb + t) + ty] : 0;
+            int key = c > 0 ? t : 0x7FFFFFFF;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const int v = shfl_down(key, o);
+                if (lane + o < kWave) key = min(key, v);
+            }
+            int ex = shfl_down(key, 1);
+            if (lane == kWave - 1) ex = 0x7FFFFFFF;
+            const int after = min(ex, nxt);
+            if (t < nt) {
+                const int64_t o = 6 * (int64_t)(tb + t) + 3 * ty + 2;
+                const int64_t sl = (ty ? sd.sstride : 0) + (int64_t)(tb + after) * cap;
+                k.nbx[o] = after == 0x7FFFFFFF ? -1 : k.x[sl];
+                k.nby[o] = after == 0x7FFFFFFF ? (real)0 : k.y[sl];
+            }
+            nxt = min(nxt, __shfl(key, 0, kWave));
+        }
+    }
+    const int wmx = (run[0] + pc.K - 1) / pc.K;
+    const int wmn = (run[1] + pc.K - 1) / pc.K;
+    unsigned wbase = 0;
+    if (lane == 0) {
+        k.n[2 * s] = run[0];
+        k.n[2 * s + 1] = run[1];
+        wbase = atomicAdd(&pc.wincount[pc.epoch], (unsigned)(wmx + wmn));
+    }
+    wbase = __shfl(wbase, 0, kWave);
+    for (int w = lane; w < wmx + wmn; w += kWave) {
+        const int type = w >= wmx;
+        const int wi = type ? w - wmx : w;
+        pc.winlist[2 * (wbase + w)] = s;
+        pc.winlist[2 * (wbase + w) + 1] = (wi << 1) | type;
+    }
 }
 
 // ------------------------------------------------------------- stage init
@@ -220,15 +298,13 @@ __device__ __forceinline__ real init_value(const InitArgs<real>& ia, const Serie
 
 template <typename real, int T>
 __global__ __launch_bounds__(kThreads) void k_init(SeriesDev sd, InitArgs<real> ia, real* h_out,
-                                                   KnotsDev<real> kout, PassCtl pc)
+                                                   KnotsDev<real> kout)
 {
     constexpr int NCH = T / 64;
     __shared__ real sh[T + 3];
     __shared__ uint64_t mask[2 * NCH];
     __shared__ int cpre[2 * (NCH + 1)];
-    __shared__ int s_misc[4];
-    const int gt = take_ticket<T>(pc.tickets, pc.epoch, &s_misc[0]);
-    const TileGeom<real, T> g = tile_geom<real, T>(sd, gt);
+    const TileGeom<real, T> g = tile_geom<real, T>(sd);
     real* out = h_out + sd.off[g.s];
     for (int v = threadIdx.x; v < g.nt + 2; v += kThreads) {
         const int p = g.t0 - 1 + v;
@@ -239,137 +315,324 @@ __global__ __launch_bounds__(kThreads) void k_init(SeriesDev sd, InitArgs<real> 
         }
     }
     __syncthreads();
-    emit_tile<real, T>(sd, kout, pc, g, sh, mask, cpre, &s_misc[2]);
+    emit_tile<real, T>(sd, kout, g, sh, mask, cpre);
 }
 
 // ---------------------------------------------------------------- sift pass
 
+// Per-tile inputs that depend only on the tile index (prefetched one tile
+// ahead of the samples, two ahead of use).
+struct TileMeta {
+    int pre[2];
+    int cnt[2];
+};
+
+// Interval coefficients S(t) = a + (b + (c + d t) t) t, t = p - pos.
+template <typename real>
+struct alignas(2 * sizeof(real)) Coef {
+    real a, b, c, d;
+};
+
+template <typename real>
+__device__ __forceinline__ real horner(const Coef<real>& q, real t)
+{
+    return q.a + (q.b + (q.c + q.d * t) * t) * t;
+}
+
+// One workgroup sifts a segment of consecutive tiles of one series.  Lane
+// mapping: main sample u = r*256 + tid of the tile, so the extrema ballots of
+// a wave are the knot flags of its lanes' samples and a sample's interval is
+// (knots before its chunk) + mbcnt(ballot) - 1.
 template <typename real, int T>
 __global__ __launch_bounds__(kThreads) void k_sift(SiftArgs<real> a)
 {
-    constexpr int NCH = T / 64;
-    constexpr int KC = T / 2 + 8;
-    __shared__ real sh[T + 3];
-    __shared__ uint64_t mask[2 * NCH];
-    __shared__ int cpre[2 * (NCH + 1)];
-    __shared__ int skp[2][KC];
-    __shared__ real ska[2][KC], skc[2][KC], skb[2][KC], skd[2][KC];
-    __shared__ int s_misc[8];
+    constexpr int NR = T / kThreads;                          // main samples per thread
+    constexpr int NCH = T / kWave;                            // 64-sample chunks per tile
+    constexpr int KC = T / 2 + 8;                             // knot slots per type
+    constexpr int NH = (T + 3 + kThreads - 1) / kThreads;     // samples prefetched per thread
+    constexpr int NC = (2 * KC + kThreads - 1) / kThreads;    // c values prefetched per thread
+    __shared__ real sh[T + 3];                 // h over [t0-2, t1]; h_new after the update
+    __shared__ int s_cnt[2][NCH];              // knots per chunk
+    __shared__ int skp[2][KC];                 // knot positions of [klo, khi]
+    __shared__ Coef<real> sco[2][KC];          // knot value (a), c, then b and d
+    __shared__ real scr[2 * KC];               // prefetched c of [lo, hi] of both types
+    __shared__ int snbx[6];
+    __shared__ real snby[6];
+    __shared__ int s_flag[2];
     __shared__ int s_red[2 * (kThreads / kWave)];
 
     const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
     const SeriesDev& sd = a.sd;
-    const int gt = take_ticket<T>(a.pc.tickets, a.pc.epoch, &s_misc[0]);
-    const TileGeom<real, T> g = tile_geom<real, T>(sd, gt);
-    if (a.active && !a.active[g.s]) return;
-
-    const real* h = a.h_in + sd.off[g.s];
-    real* hout = a.h_out + sd.off[g.s];
-    for (int q = tid; q < g.nt + 3; q += kThreads) {
-        const int p = g.t0 - 2 + q;
-        sh[q] = (p >= 0 && p < g.N) ? h[p] : (real)0;
-    }
-    __syncthreads();
-    tile_flags<real, T>(sh, g.N, g.t0, g.nt, mask);
-    if (tid == 0) {
-        // extrema flags of sample t0-1 (needed for its interval)
-        int fmx = 0, fmn = 0;
-        const int p = g.t0 - 1;
-        if (p == 0) {
-            fmx = fmn = 1;
-        } else if (p > 0) {
-            fmx = (sh[1] > sh[0]) && (sh[1] > sh[2]);
-            fmn = (sh[1] < sh[0]) && (sh[1] < sh[2]);
-        }
-        s_misc[2] = fmx;
-        s_misc[3] = fmn;
-    }
-    __syncthreads();
-    chunk_prefix<T>(mask, cpre);
-    __syncthreads();
-
-    const int pre0 = a.kin.pre[2 * gt], pre1 = a.kin.pre[2 * gt + 1];
-    const int n0 = a.kin.n[2 * g.s], n1 = a.kin.n[2 * g.s + 1];
-    const int fprev0 = s_misc[2], fprev1 = s_misc[3];
-    // interval k of sample p: pos[k] < p <= pos[k+1], p = 0 -> 0 (cudaICEEMDAN.cu:418-433)
-    auto interval_of = [&](int type, int p) -> int {
-        if (p == 0) return 0;
-        const int pre = type ? pre1 : pre0;
-        int cl;
-        if (p == g.t0 - 1) {
-            cl = pre - (type ? fprev1 : fprev0);
-        } else {
-            const int u = p - g.t0;
-            if (u >= g.nt) cl = pre + cpre[type * (NCH + 1) + NCH];
-            else cl = pre + rank_in_tile<T>(mask, cpre, type, u);
-        }
-        return cl - 1;
-    };
-    const int p_first = max(g.t0 - 1, 0);
-    const int p_last = min(g.t0 + g.nt, g.N - 1);
-    const int klo0 = interval_of(0, p_first), khi0 = interval_of(0, p_last) + 1;
-    const int klo1 = interval_of(1, p_first), khi1 = interval_of(1, p_last) + 1;
-    int cnt0 = khi0 - klo0 + 1, cnt1 = khi1 - klo1 + 1;
-    if (cnt0 > KC || cnt1 > KC || cnt0 < 2 || cnt1 < 2 || khi0 >= n0 || khi1 >= n1) {
-        if (tid == 0) atomicOr(a.pc.err, ICM_DEVERR_KNOTCAP);
-        return;
-    }
-    const int64_t kb = sd.kbase[g.s];
-    for (int i = tid; i < cnt0 + cnt1; i += kThreads) {
-        const int type = (i >= cnt0);
-        const int idx = type ? i - cnt0 : i;
-        const int64_t gi = kb + (type ? sd.kstride : 0) + (type ? klo1 : klo0) + idx;
-        skp[type][idx] = a.kin.x[gi];
-        ska[type][idx] = a.kin.y[gi];
-        skc[type][idx] = a.c[gi];
-    }
-    __syncthreads();
-    // interval coefficients (cudaICEEMDAN.cu:394-398)
-    for (int i = tid; i < cnt0 + cnt1 - 2; i += kThreads) {
-        const int type = (i >= cnt0 - 1);
-        const int k = type ? i - (cnt0 - 1) : i;
-        const real hk = (real)(skp[type][k + 1] - skp[type][k]);
-        const real c0 = skc[type][k], c1 = skc[type][k + 1];
-        skb[type][k] = (ska[type][k + 1] - ska[type][k]) / hk - hk * ((real)2 * c0 + c1) / (real)3;
-        skd[type][k] = (c1 - c0) / ((real)3 * hk);
-    }
-    __syncthreads();
-
+    const Segment seg = a.segs[blockIdx.x];
+    const int s = seg.s;
+    if (a.active && !a.active[s]) return;
+    const int N = sd.len[s];
+    const int tb = sd.tile0[s];
+    const int64_t kb = sd.kbase[s];
+    const real* h = a.h_in + sd.off[s];
+    real* hout = a.h_out + sd.off[s];
+    const int n0 = a.kin.n[2 * s], n1 = a.kin.n[2 * s + 1];
+    const real fix00 = a.kin.fix[4 * s], fix01 = a.kin.fix[4 * s + 1];
+    const real fix10 = a.kin.fix[4 * s + 2], fix11 = a.kin.fix[4 * s + 3];
     const bool use_mean = (n0 > 3) && (n1 > 3);
-    int sxc = 0, sxm = 0;
-    for (int v = tid; v < g.nt + 2; v += kThreads) {
-        const int p = g.t0 - 1 + v;
-        if (p < 0 || p >= g.N) continue;
-        const int ku = interval_of(0, p) - klo0;
-        const int kl = interval_of(1, p) - klo1;
-        const real tu = (real)(p - skp[0][ku]);
-        const real U = ska[0][ku] + (skb[0][ku] + (skc[0][ku] + skd[0][ku] * tu) * tu) * tu;
-        const real tl = (real)(p - skp[1][kl]);
-        const real L = ska[1][kl] + (skb[1][kl] + (skc[1][kl] + skd[1][kl] * tl) * tl) * tl;
-        const real m = use_mean ? (U + L) / (real)2 : (real)0;
-        const real hn = sh[v + 1] - m;
-        sh[v + 1] = hn;
-        if (v >= 1 && v <= g.nt) {
-            hout[p] = hn;
-            if (a.sx_cnt) {
-                const real sx = fabs((U + L) / (U - L));
-                sxc += (sx > (real)0.05) ? 1 : 0;
-                sxm |= (sx > (real)0.5) ? 1 : 0;
-            }
+    const real* c0p = a.c + kb;
+    const real* c1p = a.c + sd.kstride + kb;
+
+    auto load_meta = [&](int t, TileMeta& m) {
+        const int gt = tb + t;
+        m.pre[0] = a.kin.pre[2 * gt];
+        m.pre[1] = a.kin.pre[2 * gt + 1];
+        m.cnt[0] = a.kin.cnt[2 * gt];
+        m.cnt[1] = a.kin.cnt[2 * gt + 1];
+    };
+    auto load_tile = [&](int t, const TileMeta& m, real* hr, real* cr, int* nbx, real* nby) {
+        const int t0 = t * T;
+        const int nt = min(T, N - t0);
+#pragma unroll
+        for (int k = 0; k < NH; ++k) {
+            const int q = tid + k * kThreads;
+            const int p = t0 - 2 + q;
+            hr[k] = (q < nt + 3 && p >= 0 && p < N) ? h[p] : (real)0;
         }
-    }
-    if (a.sx_cnt) {
-        // workgroup sums of the Rilling partials (integers: order-free)
-        for (int o = 32; o >= 1; o >>= 1) {
-            sxc += __shfl_xor(sxc, o, kWave);
-            sxm |= __shfl_xor(sxm, o, kWave);
+        // c of knots [max(pre-2,0), min(pre+cnt, n-1)] per type, back to back
+        const int lo0 = max(m.pre[0] - 2, 0);
+        const int len0 = min(m.pre[0] + m.cnt[0], n0 - 1) - lo0 + 1;
+        const int lo1 = max(m.pre[1] - 2, 0);
+        const int len1 = min(m.pre[1] + m.cnt[1], n1 - 1) - lo1 + 1;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            const int j = tid + k * kThreads;
+            real v = (real)0;
+            if (j < len0) v = c0p[lo0 + j];
+            else if (j - len0 < len1) v = c1p[lo1 + (j - len0)];
+            cr[k] = v;
         }
-        if ((tid & 63) == 0) {
-            s_red[2 * (tid >> 6)] = sxc;
-            s_red[2 * (tid >> 6) + 1] = sxm;
+        if (tid < 6) {
+            const int64_t o = 6 * (int64_t)(tb + t) + tid;
+            *nbx = a.kin.nbx[o];
+            *nby = a.kin.nby[o];
+        }
+    };
+
+    real hcur[NH], hnext[NH], ccur[NC], cnext[NC];
+    int nbx_cur = 0, nbx_next = 0;
+    real nby_cur = (real)0, nby_next = (real)0;
+    TileMeta mcur, mnext;
+    load_meta(seg.t_first, mcur);
+    if (seg.count > 1) load_meta(seg.t_first + 1, mnext);
+    load_tile(seg.t_first, mcur, hcur, ccur, &nbx_cur, &nby_cur);
+
+    for (int i = 0; i < seg.count; ++i) {
+        const int t = seg.t_first + i;
+        const int gt = tb + t;
+        const int t0 = t * T;
+        const int nt = min(T, N - t0);
+        // stage the prefetched tile, then start fetching the next one
+#pragma unroll
+        for (int k = 0; k < NH; ++k) {
+            const int q = tid + k * kThreads;
+            if (q < T + 3) sh[q] = hcur[k];
+        }
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            const int j = tid + k * kThreads;
+            if (j < 2 * KC) scr[j] = ccur[k];
+        }
+        if (tid < 6) {
+            snbx[tid] = nbx_cur;
+            snby[tid] = nby_cur;
+        }
+        const TileMeta m = mcur;
+        TileMeta mnext2 = mnext;
+        if (i + 1 < seg.count) {
+            load_tile(t + 1, mnext, hnext, cnext, &nbx_next, &nby_next);
+            if (i + 2 < seg.count) load_meta(t + 2, mnext2);
         }
         __syncthreads();
+
+        // ---- extrema of h over the tile (find_extrema_shfl_*, :92-148)
+        uint64_t bm[2][NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const int u = r * kThreads + tid;
+            const int p = t0 + u;
+            bool fmx = false, fmn = false;
+            if (u < nt) {
+                if (p == 0 || p == N - 1) {
+                    fmx = fmn = true;
+                } else {
+                    const real v = sh[u + 2], lft = sh[u + 1], rgt = sh[u + 3];
+                    fmx = (v > lft) && (v > rgt);
+                    fmn = (v < lft) && (v < rgt);
+                }
+            }
+            bm[0][r] = wave_ballot(fmx);
+            bm[1][r] = wave_ballot(fmn);
+            if (lane == 0) {
+                s_cnt[0][r * (kThreads / kWave) + wv] = __popcll(bm[0][r]);
+                s_cnt[1][r * (kThreads / kWave) + wv] = __popcll(bm[1][r]);
+            }
+        }
         if (tid == 0) {
+            int fmx = 0, fmn = 0;
+            const int p = t0 - 1;
+            if (p == 0) {
+                fmx = fmn = 1;
+            } else if (p > 0) {
+                fmx = (sh[1] > sh[0]) && (sh[1] > sh[2]);
+                fmn = (sh[1] < sh[0]) && (sh[1] < sh[2]);
+            }
+            s_flag[0] = fmx;
+            s_flag[1] = fmn;
+        }
+        __syncthreads();
+
+        // knots before each of this thread's chunks, and the tile totals
+        int cb[2][NR];
+        int tot[2] = {0, 0};
+#pragma unroll
+        for (int r = 0; r < NR; ++r) cb[0][r] = cb[1][r] = 0;
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+            const int x0 = s_cnt[0][ch], x1 = s_cnt[1][ch];
+#pragma unroll
+            for (int r = 0; r < NR; ++r)
+                if (ch < r * (kThreads / kWave) + wv) {
+                    cb[0][r] += x0;
+                    cb[1][r] += x1;
+                }
+            tot[0] += x0;
+            tot[1] += x1;
+        }
+        const int fprev[2] = {s_flag[0], s_flag[1]};
+        int klo[2], khi[2], lo[2], len[2];
+        const int nn[2] = {n0, n1};
+#pragma unroll
+        for (int ty = 0; ty < 2; ++ty) {
+            // interval of sample p: pos[k] < p <= pos[k+1], p = 0 -> 0 (:418-433)
+            klo[ty] = t0 >= 1 ? m.pre[ty] - fprev[ty] - 1 : 0;
+            khi[ty] = t0 + nt < N ? m.pre[ty] + tot[ty] : nn[ty] - 1;
+            lo[ty] = max(m.pre[ty] - 2, 0);
+            len[ty] = min(m.pre[ty] + m.cnt[ty], nn[ty] - 1) - lo[ty] + 1;
+        }
+        const int cnt0 = khi[0] - klo[0] + 1, cnt1 = khi[1] - klo[1] + 1;
+        if (cnt0 > KC || cnt1 > KC || cnt0 < 2 || cnt1 < 2 || khi[0] >= n0 || khi[1] >= n1 ||
+            klo[0] < lo[0] || klo[1] < lo[1] || khi[0] >= lo[0] + len[0] ||
+            khi[1] >= lo[1] + len[1] || tot[0] != m.cnt[0] || tot[1] != m.cnt[1]) {
+            if (tid == 0) atomicOr(a.pc.err, ICM_DEVERR_KNOTCAP);
+            return;
+        }
+        // ---- knot table of [klo, khi]: the lanes owning the tile's knots,
+        // then the (at most three) knots outside it
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const int u = r * kThreads + tid;
+#pragma unroll
+            for (int ty = 0; ty < 2; ++ty) {
+                if ((bm[ty][r] >> lane) & 1ull) {
+                    const int gk = m.pre[ty] + cb[ty][r] + lanes_below(bm[ty][r]);
+                    const int idx = gk - klo[ty];
+                    real y = sh[u + 2];
+                    if (gk == 0) y = ty ? fix10 : fix00;
+                    if (gk == nn[ty] - 1) y = ty ? fix11 : fix01;
+                    skp[ty][idx] = t0 + u;
+                    sco[ty][idx].a = y;
+                    sco[ty][idx].c = scr[(ty ? len[0] : 0) + gk - lo[ty]];
+                }
+            }
+        }
+        if (tid < 6) {
+            const int ty = tid / 3, which = tid % 3;
+            const int gk = which == 0 ? m.pre[ty] - 1
+                                      : (which == 1 ? m.pre[ty] - 2 : m.pre[ty] + tot[ty]);
+            if (gk >= klo[ty] && gk <= khi[ty] && snbx[tid] >= 0) {
+                real y = snby[tid];
+                if (gk == 0) y = ty ? fix10 : fix00;
+                if (gk == nn[ty] - 1) y = ty ? fix11 : fix01;
+                skp[ty][gk - klo[ty]] = snbx[tid];
+                sco[ty][gk - klo[ty]].a = y;
+                sco[ty][gk - klo[ty]].c = scr[(ty ? len[0] : 0) + gk - lo[ty]];
+            }
+        }
+        __syncthreads();
+        // ---- interval coefficients (spline_coefficients, :394-398)
+        for (int j = tid; j < cnt0 + cnt1 - 2; j += kThreads) {
+            const int ty = (j >= cnt0 - 1);
+            const int k = ty ? j - (cnt0 - 1) : j;
+            const real hk = (real)(skp[ty][k + 1] - skp[ty][k]);
+            const real a0 = sco[ty][k].a, a1 = sco[ty][k + 1].a;
+            const real c0 = sco[ty][k].c, c1 = sco[ty][k + 1].c;
+            sco[ty][k].b = (a1 - a0) / hk - hk * ((real)2 * c0 + c1) / (real)3;
+            sco[ty][k].d = (c1 - c0) / ((real)3 * hk);
+        }
+        __syncthreads();
+
+        // ---- envelopes, mean, h_new = h - m (interpolate :403-439,
+        // averageUppperLower :441-454, updateRealizations :527-544)
+        int sxc = 0, sxm = 0;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const int u = r * kThreads + tid;
+            if (u < nt) {
+                const int p = t0 + u;
+                int ku = 0, kl = 0;
+                if (p > 0) {
+                    ku = m.pre[0] + cb[0][r] + lanes_below(bm[0][r]) - 1;
+                    kl = m.pre[1] + cb[1][r] + lanes_below(bm[1][r]) - 1;
+                }
+                ku -= klo[0];
+                kl -= klo[1];
+                const real U = horner(sco[0][ku], (real)(p - skp[0][ku]));
+                const real L = horner(sco[1][kl], (real)(p - skp[1][kl]));
+                const real mm = use_mean ? (U + L) / (real)2 : (real)0;
+                const real hn = sh[u + 2] - mm;
+                sh[u + 2] = hn;
+                hout[p] = hn;
+                if (a.sx_cnt) {
+                    const real sx = fabs((U + L) / (U - L));
+                    sxc += (sx > (real)0.05) ? 1 : 0;
+                    sxm |= (sx > (real)0.5) ? 1 : 0;
+                }
+            }
+        }
+        // the samples just outside the tile (t0-1 and t1), needed for the
+        // extrema of h_new at the tile edges
+        if (tid < 2) {
+            const int p = tid == 0 ? t0 - 1 : t0 + nt;
+            if (p >= 0 && p < N) {
+                int ku, kl;
+                if (p == 0) {
+                    ku = kl = 0;
+                } else if (tid == 0) {
+                    ku = m.pre[0] - fprev[0] - 1;
+                    kl = m.pre[1] - fprev[1] - 1;
+                } else {
+                    ku = m.pre[0] + tot[0] - 1;
+                    kl = m.pre[1] + tot[1] - 1;
+                }
+                ku -= klo[0];
+                kl -= klo[1];
+                const real U = horner(sco[0][ku], (real)(p - skp[0][ku]));
+                const real L = horner(sco[1][kl], (real)(p - skp[1][kl]));
+                const real mm = use_mean ? (U + L) / (real)2 : (real)0;
+                const int q = tid == 0 ? 1 : nt + 2;
+                sh[q] = sh[q] - mm;
+            }
+        }
+        if (a.sx_cnt) {
+            // workgroup sums of the Rilling partials (integers: order-free)
+            for (int o = 32; o >= 1; o >>= 1) {
+                sxc += __shfl_xor(sxc, o, kWave);
+                sxm |= __shfl_xor(sxm, o, kWave);
+            }
+            if (lane == 0) {
+                s_red[2 * wv] = sxc;
+                s_red[2 * wv + 1] = sxm;
+            }
+        }
+        __syncthreads();
+        if (a.sx_cnt && tid == 0) {
             int c = 0, mk = 0;
             for (int w = 0; w < kThreads / kWave; ++w) {
                 c += s_red[2 * w];
@@ -378,10 +641,75 @@ __global__ __launch_bounds__(kThreads) void k_sift(SiftArgs<real> a)
             a.sx_cnt[gt] = c;
             a.sx_mark[gt] = mk;
         }
+
+        if (a.emit) {
+            // ---- knots of h_new into the tile's slots (select_extrema_*, :150-207)
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const int u = r * kThreads + tid;
+                const int p = t0 + u;
+                bool fmx = false, fmn = false;
+                if (u < nt) {
+                    if (p == 0 || p == N - 1) {
+                        fmx = fmn = true;
+                    } else {
+                        const real v = sh[u + 2], lft = sh[u + 1], rgt = sh[u + 3];
+                        fmx = (v > lft) && (v > rgt);
+                        fmn = (v < lft) && (v < rgt);
+                    }
+                }
+                bm[0][r] = wave_ballot(fmx);
+                bm[1][r] = wave_ballot(fmn);
+                if (lane == 0) {
+                    s_cnt[0][r * (kThreads / kWave) + wv] = __popcll(bm[0][r]);
+                    s_cnt[1][r * (kThreads / kWave) + wv] = __popcll(bm[1][r]);
+                }
+            }
+            __syncthreads();
+            int tot2[2] = {0, 0};
+#pragma unroll
+            for (int r = 0; r < NR; ++r) cb[0][r] = cb[1][r] = 0;
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch) {
+                const int x0 = s_cnt[0][ch], x1 = s_cnt[1][ch];
+#pragma unroll
+                for (int r = 0; r < NR; ++r)
+                    if (ch < r * (kThreads / kWave) + wv) {
+                        cb[0][r] += x0;
+                        cb[1][r] += x1;
+                    }
+                tot2[0] += x0;
+                tot2[1] += x1;
+            }
+            const int64_t slot0 = (int64_t)gt * sd.cap;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const int u = r * kThreads + tid;
+#pragma unroll
+                for (int ty = 0; ty < 2; ++ty) {
+                    if ((bm[ty][r] >> lane) & 1ull) {
+                        const int64_t o = (ty ? sd.sstride : 0) + slot0 + cb[ty][r] +
+                                          lanes_below(bm[ty][r]);
+                        a.kout.x[o] = t0 + u;
+                        a.kout.y[o] = sh[u + 2];
+                    }
+                }
+            }
+            if (tid == 0) {
+                a.kout.cnt[2 * gt] = tot2[0];
+                a.kout.cnt[2 * gt + 1] = tot2[1];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NH; ++k) hcur[k] = hnext[k];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) ccur[k] = cnext[k];
+        nbx_cur = nbx_next;
+        nby_cur = nby_next;
+        mcur = mnext;
+        mnext = mnext2;
     }
-    if (!a.emit) return;
-    __syncthreads();
-    emit_tile<real, T>(sd, a.kout, a.pc, g, sh, mask, cpre, &s_misc[4]);
 }
 
 // ------------------------------------------------------- windowed spline solve
@@ -405,11 +733,61 @@ __device__ __forceinline__ void spline_row(int n, long g, int i, const int* sp, 
     f = (real)3 * ((sy[i + 1] - sy[i]) / (real)hr - (sy[i] - sy[i - 1]) / (real)hl);
 }
 
-template <typename real>
 __device__ __forceinline__ void wave_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
+}
+
+// Copy knots [glo, ghi) of (series s, type) from the tile slots into LDS
+// (sp/sy index = g - lo).  One wavefront; tpre is 64 ints of LDS scratch.
+template <typename real>
+__device__ void gather_knots(const SeriesDev& sd, const KnotsDev<real>& k, int s, int type,
+                             long lo, int glo, int ghi, int* sp, real* sy, int* tpre)
+{
+    const int lane = threadIdx.x & 63;
+    const int tb = sd.tile0[s], ntl = sd.ntl[s];
+    // 64-ary search for the tile holding knot glo
+    int a = 0, b = ntl;
+    while (b - a > 1) {
+        const int step = (b - a + kWave - 1) / kWave;
+        const int idx = a + lane * step;
+        const bool ok = idx < b && k.pre[2 * (tb + idx) + type] <= glo;
+        const uint64_t m = wave_ballot(ok);
+        const int L = 63 - __builtin_clzll(m);
+        a = a + L * step;
+        b = min(a + step, b);
+    }
+    int g = glo;
+    int tile = a;
+    while (g < ghi) {
+        const int tl = tile + lane;
+        const bool valid = tl < ntl;
+        const int p = valid ? k.pre[2 * (tb + tl) + type] : 0x7FFFFFFF;
+        const int c = valid ? k.cnt[2 * (tb + tl) + type] : 0;
+        tpre[lane] = p;
+        const uint64_t vm = wave_ballot(valid);
+        if (vm == 0ull) break;   // counts inconsistent with n: leave the rest
+        const int last = 63 - __builtin_clzll(vm);
+        const int pend = __shfl(p + c, last, kWave);
+        wave_sync();
+        const int gend = min(ghi, pend);
+        for (int gi = g + lane; gi < gend; gi += kWave) {
+            int x = 0, y = last;
+            while (x < y) {
+                const int mid = (x + y + 1) >> 1;
+                if (tpre[mid] <= gi) x = mid;
+                else y = mid - 1;
+            }
+            const int64_t slot = (type ? sd.sstride : 0) + (int64_t)(tb + tile + x) * sd.cap +
+                                 (gi - tpre[x]);
+            sp[gi - lo] = k.x[slot];
+            sy[gi - lo] = k.y[slot];
+        }
+        g = gend;
+        tile += kWave;
+        wave_sync();
+    }
 }
 
 template <typename real, int B, int H>
@@ -420,6 +798,7 @@ __global__ __launch_bounds__(kThreads) void k_solve(SolveArgs<real> a)
     constexpr int NW = kThreads / kWave;
     __shared__ int s_pos[NW][W + 2];
     __shared__ real s_y[NW][W + 2];
+    __shared__ int s_tpre[NW][kWave];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     int* sp = s_pos[wv];
     real* sy = s_y[wv];
@@ -434,35 +813,37 @@ __global__ __launch_bounds__(kThreads) void k_solve(SolveArgs<real> a)
         const int64_t kb = a.sd.kbase[s] + (type ? a.sd.kstride : 0);
         const long g0 = (long)w * K - H;
         const long lo = g0 - 1;
-        for (int i = lane; i < W + 2; i += kWave) {
-            const long gg = lo + i;
-            if (gg >= 0 && gg < n) {
-                sp[i] = a.k.x[kb + gg];
-                sy[i] = a.k.y[kb + gg];
-            }
-        }
-        wave_sync<real>();
-        // endpoint fix-up when n >= 6 (cudaICEEMDAN.cu:209-291); idempotent,
-        // so windows that overlap an end may all apply it
-        if (lane == 0 && n >= 6) {
+        const int glo = (int)max(lo, 0L);
+        const int ghi = (int)min(lo + W + 2, (long)n);
+        gather_knots<real>(a.sd, a.k, s, type, lo, glo, ghi, sp, sy, s_tpre[wv]);
+        wave_sync();
+        // endpoint fix-up when n >= 6 (setBoundaryMax/Min, cudaICEEMDAN.cu:209-291);
+        // idempotent, so every window that holds an end applies it
+        if (lane == 0) {
+            const long keep0 = (long)w * K;
             if (lo <= 0) {
                 const int i0 = (int)(-lo);
-                const real slope = (sy[i0 + 2] - sy[i0 + 1]) / (real)(sp[i0 + 2] - sp[i0 + 1]);
-                const real t = sy[i0 + 1] - slope * (real)(sp[i0 + 1] - sp[i0]);
-                if (type == 0 ? (t > sy[i0]) : (t < sy[i0])) sy[i0] = t;
-                if (w == 0) a.k.y[kb] = sy[i0];
+                if (n >= 6) {
+                    const real slope =
+                        (sy[i0 + 2] - sy[i0 + 1]) / (real)(sp[i0 + 2] - sp[i0 + 1]);
+                    const real t = sy[i0 + 1] - slope * (real)(sp[i0 + 1] - sp[i0]);
+                    if (type == 0 ? (t > sy[i0]) : (t < sy[i0])) sy[i0] = t;
+                }
+                if (w == 0) a.k.fix[4 * s + 2 * type] = sy[i0];
             }
             const long ie = (long)(n - 1) - lo;
             if (ie >= 0 && ie < W + 2) {
                 const int i1 = (int)ie;
-                const real slope = (sy[i1 - 1] - sy[i1 - 2]) / (real)(sp[i1 - 1] - sp[i1 - 2]);
-                const real t = sy[i1 - 1] + slope * (real)(sp[i1] - sp[i1 - 1]);
-                if (type == 0 ? (t > sy[i1]) : (t < sy[i1])) sy[i1] = t;
-                const long keep0 = (long)w * K;
-                if (n - 1 >= keep0 && n - 1 < keep0 + K) a.k.y[kb + n - 1] = sy[i1];
+                if (n >= 6) {
+                    const real slope =
+                        (sy[i1 - 1] - sy[i1 - 2]) / (real)(sp[i1 - 1] - sp[i1 - 2]);
+                    const real t = sy[i1 - 1] + slope * (real)(sp[i1] - sp[i1 - 1]);
+                    if (type == 0 ? (t > sy[i1]) : (t < sy[i1])) sy[i1] = t;
+                }
+                if (n - 1 >= keep0 && n - 1 < keep0 + K) a.k.fix[4 * s + 2 * type + 1] = sy[i1];
             }
         }
-        wave_sync<real>();
+        wave_sync();
 
         // per-lane elimination of the interior rows q = 0..B-2:
         // x_q = al_q + be_q * z_{lane-1} + ga_q * z_lane
@@ -544,7 +925,7 @@ __global__ __launch_bounds__(kThreads) void k_solve(SolveArgs<real> a)
                 cout[gg] = x;
             }
         }
-        wave_sync<real>();
+        wave_sync();
     }
 }
 
@@ -775,22 +1156,50 @@ __global__ void k_check_index(const int* x, int64_t N, int* err)
 static inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 template <typename real> struct Geom;
-template <> struct Geom<double> { static constexpr int T = 512, B = 12, H = 64; };
-template <> struct Geom<float> { static constexpr int T = 512, B = 8, H = 32; };
+template <> struct Geom<double> { static constexpr int B = 12, H = 64; };
+template <> struct Geom<float> { static constexpr int B = 8, H = 32; };
 
+// tile size (samples per tile) is a plan parameter: sd.cap = T/2 + 2
 template <typename real>
 hipError_t launch_init(const SeriesDev& sd, const InitArgs<real>& ia, real* h_out,
-                       const KnotsDev<real>& kout, const PassCtl& pc, hipStream_t st)
+                       const KnotsDev<real>& kout, const PassCtl&, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_init<real, Geom<real>::T>), dim3(sd.ntiles), dim3(kThreads), 0, st, sd,
-                       ia, h_out, kout, pc);
+    const int T = 2 * (sd.cap - 2);
+    if (T == 256)
+        hipLaunchKernelGGL((k_init<real, 256>), dim3(sd.ntiles), dim3(kThreads), 0, st, sd, ia,
+                           h_out, kout);
+    else if (T == 512)
+        hipLaunchKernelGGL((k_init<real, 512>), dim3(sd.ntiles), dim3(kThreads), 0, st, sd, ia,
+                           h_out, kout);
+    else if (T == 1024)
+        hipLaunchKernelGGL((k_init<real, 1024>), dim3(sd.ntiles), dim3(kThreads), 0, st, sd, ia,
+                           h_out, kout);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
 template <typename real>
-hipError_t launch_sift(const SiftArgs<real>& a, hipStream_t st)
+hipError_t launch_knot_prefix(const SeriesDev& sd, const KnotsDev<real>& k, const PassCtl& pc,
+                              const int* active, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_sift<real, Geom<real>::T>), dim3(a.sd.ntiles), dim3(kThreads), 0, st, a);
+    hipLaunchKernelGGL((k_knot_prefix<real>), dim3(sd.nseries), dim3(kWave), 0, st, sd, k, pc,
+                       active);
+    return hipGetLastError();
+}
+
+template <typename real>
+hipError_t launch_sift(const SiftArgs<real>& a, int nsegs, hipStream_t st)
+{
+    const int T = 2 * (a.sd.cap - 2);
+    if (T == 256)
+        hipLaunchKernelGGL((k_sift<real, 256>), dim3(nsegs), dim3(kThreads), 0, st, a);
+    else if (T == 512)
+        hipLaunchKernelGGL((k_sift<real, 512>), dim3(nsegs), dim3(kThreads), 0, st, a);
+    else if (T == 1024)
+        hipLaunchKernelGGL((k_sift<real, 1024>), dim3(nsegs), dim3(kThreads), 0, st, a);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
@@ -900,7 +1309,9 @@ hipError_t launch_check_index(const int* d_x, int64_t N, int* err, hipStream_t s
 #define ICM_INSTANTIATE(real)                                                                      \
     template hipError_t launch_init<real>(const SeriesDev&, const InitArgs<real>&, real*,          \
                                           const KnotsDev<real>&, const PassCtl&, hipStream_t);     \
-    template hipError_t launch_sift<real>(const SiftArgs<real>&, hipStream_t);                     \
+    template hipError_t launch_sift<real>(const SiftArgs<real>&, int, hipStream_t);                \
+    template hipError_t launch_knot_prefix<real>(const SeriesDev&, const KnotsDev<real>&,          \
+                                                 const PassCtl&, const int*, hipStream_t);         \
     template hipError_t launch_solve<real>(const SolveArgs<real>&, int, hipStream_t);              \
     template hipError_t launch_rilling_decide<real>(const SeriesDev&, const int*, const int*,      \
                                                     const int*, int*, int*, int*, int, int,        \

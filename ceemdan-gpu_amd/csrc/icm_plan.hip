@@ -17,6 +17,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -34,7 +35,13 @@ const uint32_t kEpochs = 65536;
 
 // --------------------------------------------------------------- geometry
 
-int tile_of(int) { return 512; }
+// samples per tile of the tile kernels (256, 512 or 1024)
+int tile_of(int)
+{
+    const char* env = std::getenv("ICM_TILE");
+    const int t = env ? std::atoi(env) : 512;
+    return (t == 256 || t == 1024) ? t : 512;
+}
 int solve_B(int dtype) { return dtype == ICM_DTYPE_F64 ? 12 : 8; }
 int solve_H(int dtype) { return dtype == ICM_DTYPE_F64 ? 64 : 32; }
 int solve_K(int dtype) { return 64 * solve_B(dtype) - 2 * solve_H(dtype); }
@@ -57,14 +64,16 @@ int validate(const icm_config* c)
     if (total * (int64_t)c->num_noise > (int64_t)1 << 40) return ICM_ERR_INVALID_ARG;
     // one launch covers every tile of a phase: tiles x 256 work-items < 2^32
     int64_t tiles = 0;
-    for (int i = 0; i < c->num_channels; ++i) tiles += (c->lengths[i] + 511) / 512;
+    const int T = tile_of(c->dtype);
+    for (int i = 0; i < c->num_channels; ++i) tiles += (c->lengths[i] + T - 1) / T;
     if (tiles * c->num_noise * 256 >= ((int64_t)1 << 32)) return ICM_ERR_INVALID_ARG;
     return ICM_OK;
 }
 
 // Series tables of one phase (host side), built from (length, count) groups.
 struct HostSeries {
-    std::vector<int> tile_series, tile0, ntl, len, chan, nidx;
+    std::vector<int> tile_order, tile_series, tile0, ntl, len, chan, nidx;
+    std::vector<Segment> segs;
     std::vector<int64_t> off, kbase, eoff;
     int64_t samples = 0, kcap = 0, maxwin = 0;
     int ntiles = 0;
@@ -108,20 +117,46 @@ static void build_series(HostSeries& hs, const std::vector<int64_t>& lens, int I
     hs.samples = off;
     hs.kcap = kb;
     hs.ntiles = tiles;
+    // segments of consecutive tiles, one workgroup each: long enough to
+    // amortize the prefetch pipeline, short enough to fill every CU
+    int seg = 1;
+    const char* env = std::getenv("ICM_SEGMENT_TILES");
+    if (env && std::atoi(env) > 0) seg = std::atoi(env);
+    else seg = std::max(1, std::min(32, tiles / 6144));
+    for (size_t s = 0; s < hs.ntl.size(); ++s)
+        for (int t = 0; t < hs.ntl[s]; t += seg) {
+            Segment g;
+            g.s = (int)s;
+            g.t_first = t;
+            g.count = std::min(seg, hs.ntl[s] - t);
+            g.pad = 0;
+            hs.segs.push_back(g);
+        }
+    // launch order of the tile kernels: tile t of every series before tile t+1
+    int maxt = 0;
+    for (int nt : hs.ntl) maxt = std::max(maxt, nt);
+    hs.tile_order.reserve(tiles);
+    for (int t = 0; t < maxt; ++t)
+        for (size_t s = 0; s < hs.ntl.size(); ++s)
+            if (t < hs.ntl[s]) hs.tile_order.push_back(hs.tile0[s] + t);
 }
 
 // ------------------------------------------------------------------ plan
 
 struct DevSeries {
-    int *tile_series = nullptr, *tile0 = nullptr, *ntl = nullptr, *len = nullptr, *chan = nullptr,
+    Segment* segs = nullptr;
+    int nsegs = 0;
+    int *tile_order = nullptr, *tile_series = nullptr, *tile0 = nullptr, *ntl = nullptr, *len = nullptr, *chan = nullptr,
         *nidx = nullptr;
     int64_t *off = nullptr, *kbase = nullptr, *eoff = nullptr;
     int nseries = 0, ntiles = 0, max_len = 0;
     int64_t samples = 0, kcap = 0, maxwin = 0;
-    SeriesDev view(int64_t kstride) const
+    SeriesDev view(int64_t kstride, int64_t sstride, int cap) const
     {
         SeriesDev v;
-        v.tile_series = tile_series; v.tile0 = tile0; v.ntl = ntl; v.off = off; v.len = len;
+        v.sstride = sstride;
+        v.cap = cap;
+        v.tile_order = tile_order; v.tile_series = tile_series; v.tile0 = tile0; v.ntl = ntl; v.off = off; v.len = len;
         v.kbase = kbase; v.chan = chan; v.eoff = eoff; v.nidx = nidx;
         v.nseries = nseries; v.ntiles = ntiles; v.kstride = kstride;
         return v;
@@ -139,13 +174,13 @@ struct PlanBase {
     HostSeries hn, hsig;
     DevSeries dn, dsig;
     int64_t kstride = 0, maxwin = 0, ntiles_max = 0, nser_max = 0, samples_max = 0;
+    int64_t sstride = 0;   // knot slots per type (tiles x cap)
+    int cap = 0;           // knot slots per tile and type
     int64_t chan_total = 0;
     std::vector<void*> allocs;
     int64_t bytes = 0;
     hipStream_t own = nullptr;
     // control
-    uint64_t* status = nullptr;
-    unsigned* tickets = nullptr;
     unsigned* wincount = nullptr;
     int* winlist = nullptr;
     int* err = nullptr;
@@ -171,6 +206,49 @@ struct PlanBase {
     int* d_chan_len = nullptr;
     int max_chan_len = 0;
     bool noise_external = false;
+    // per-kernel timing (HIP events around each init / solve / sift launch)
+    int prof = 0;
+    std::vector<hipEvent_t> prof_ev;
+    std::vector<int> prof_kind;
+    std::vector<int64_t> prof_bytes;
+    size_t prof_used = 0;
+    double kern_ms[3] = {0, 0, 0};
+    int64_t kern_n[3] = {0, 0, 0};
+    int64_t kern_bytes[3] = {0, 0, 0};
+
+    int prof_begin(int kind, int64_t bytes, hipStream_t st)
+    {
+        if (!prof) return ICM_OK;
+        while (prof_ev.size() < 2 * (prof_used + 1)) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return ICM_ERR_HIP;
+            prof_ev.push_back(e);
+        }
+        prof_kind.resize(prof_used + 1);
+        prof_bytes.resize(prof_used + 1);
+        prof_kind[prof_used] = kind;
+        prof_bytes[prof_used] = bytes;
+        return hipEventRecord(prof_ev[2 * prof_used], st) == hipSuccess ? ICM_OK : ICM_ERR_HIP;
+    }
+    int prof_end(hipStream_t st)
+    {
+        if (!prof) return ICM_OK;
+        const hipError_t e = hipEventRecord(prof_ev[2 * prof_used + 1], st);
+        ++prof_used;
+        return e == hipSuccess ? ICM_OK : ICM_ERR_HIP;
+    }
+    void prof_collect()
+    {
+        for (int k = 0; k < 3; ++k) { kern_ms[k] = 0; kern_n[k] = 0; kern_bytes[k] = 0; }
+        for (size_t i = 0; i < prof_used; ++i) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, prof_ev[2 * i], prof_ev[2 * i + 1]);
+            kern_ms[prof_kind[i]] += ms;
+            kern_n[prof_kind[i]] += 1;
+            kern_bytes[prof_kind[i]] += prof_bytes[i];
+        }
+        prof_used = 0;
+    }
 
     virtual ~PlanBase() {}
     virtual int forward(const void* const* sig, void* const* imfs, const void* noise,
@@ -202,6 +280,9 @@ struct PlanBase {
     int upload_series(DevSeries& d, const HostSeries& h)
     {
         int rc;
+        if ((rc = upload(&d.segs, h.segs))) return rc;
+        d.nsegs = (int)h.segs.size();
+        if ((rc = upload(&d.tile_order, h.tile_order))) return rc;
         if ((rc = upload(&d.tile_series, h.tile_series))) return rc;
         if ((rc = upload(&d.tile0, h.tile0))) return rc;
         if ((rc = upload(&d.ntl, h.ntl))) return rc;
@@ -225,6 +306,7 @@ struct PlanBase {
             if (gexec[i]) (void)hipGraphExecDestroy(gexec[i]);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
+        for (auto& e : prof_ev) (void)hipEventDestroy(e);
         for (void* p : allocs) (void)hipFree(p);
         allocs.clear();
         if (h_n_active) (void)hipHostFree(h_n_active);
@@ -273,6 +355,8 @@ void layout(PlanBase& P)
     P.ntiles_max = std::max(P.hn.ntiles, P.hsig.ntiles);
     P.nser_max = std::max((int64_t)P.hn.len.size(), (int64_t)P.hsig.len.size());
     P.samples_max = std::max(P.hn.samples, P.hsig.samples);
+    P.cap = T / 2 + 2;   // strict extrema are never adjacent, plus the two ends
+    P.sstride = P.ntiles_max * P.cap;
 }
 
 int64_t plan_bytes_estimate(const icm_config* c)
@@ -291,10 +375,10 @@ int64_t plan_bytes_estimate(const icm_config* c)
     b += 2 * P.samples_max * s;                    // h ping-pong
     b += (int64_t)(M - 1) * P.hn.samples * s;      // noise modes
     b += P.hn.samples * s;                         // noise residual
-    b += 2 * 2 * P.kstride * (4 + s);              // knots x,y ping-pong
+    b += 2 * 2 * P.sstride * (4 + s);              // knot slots x,y ping-pong
     b += 2 * P.kstride * s;                        // c
-    b += 2 * 2 * P.ntiles_max * 4 + 2 * 2 * P.nser_max * 4;
-    b += P.ntiles_max * 8 + 2 * (int64_t)kEpochs * 4 + 2 * P.maxwin * 4;
+    b += 2 * 2 * 2 * P.ntiles_max * 4 + 2 * 2 * P.nser_max * (4 + 2 * s);
+    b += (int64_t)kEpochs * 4 + 2 * P.maxwin * 4;
     b += 5 * P.chan_total * s + P.chan_total * M * s;
     return b;
 }
@@ -306,8 +390,12 @@ struct Plan : PlanBase {
     real *hA = nullptr, *hB = nullptr, *modes = nullptr, *wres = nullptr;
     int* kx[2] = {nullptr, nullptr};
     real* ky[2] = {nullptr, nullptr};
+    int* kcnt[2] = {nullptr, nullptr};
     int* kpre[2] = {nullptr, nullptr};
     int* nk[2] = {nullptr, nullptr};
+    real* kfix[2] = {nullptr, nullptr};
+    int* knbx[2] = {nullptr, nullptr};
+    real* knby[2] = {nullptr, nullptr};
     real* kc = nullptr;
     real *y = nullptr, *xt = nullptr, *r = nullptr, *imf = nullptr;
     real *var_chan = nullptr, *var_noise = nullptr;
@@ -324,14 +412,16 @@ struct Plan : PlanBase {
         if ((rc = alloc(&modes, (int64_t)(M - 1) * hn.samples))) return rc;
         if ((rc = alloc(&wres, hn.samples))) return rc;
         for (int b = 0; b < 2; ++b) {
-            if ((rc = alloc(&kx[b], 2 * kstride))) return rc;
-            if ((rc = alloc(&ky[b], 2 * kstride))) return rc;
+            if ((rc = alloc(&kx[b], 2 * sstride))) return rc;
+            if ((rc = alloc(&ky[b], 2 * sstride))) return rc;
+            if ((rc = alloc(&kcnt[b], 2 * ntiles_max))) return rc;
             if ((rc = alloc(&kpre[b], 2 * ntiles_max))) return rc;
             if ((rc = alloc(&nk[b], 2 * nser_max))) return rc;
+            if ((rc = alloc(&kfix[b], 4 * nser_max))) return rc;
+            if ((rc = alloc(&knbx[b], 6 * ntiles_max))) return rc;
+            if ((rc = alloc(&knby[b], 6 * ntiles_max))) return rc;
         }
         if ((rc = alloc(&kc, 2 * kstride))) return rc;
-        if ((rc = alloc(&status, ntiles_max))) return rc;
-        if ((rc = alloc(&tickets, kEpochs))) return rc;
         if ((rc = alloc(&wincount, kEpochs))) return rc;
         if ((rc = alloc(&winlist, 2 * maxwin))) return rc;
         if ((rc = alloc(&err, 1))) return rc;
@@ -368,9 +458,7 @@ struct Plan : PlanBase {
     PassCtl ctl(uint32_t e)
     {
         PassCtl pc;
-        pc.status = status;
         pc.epoch = e;
-        pc.tickets = tickets;
         pc.winlist = winlist;
         pc.wincount = wincount;
         pc.K = solve_K(cfg.dtype);
@@ -382,8 +470,12 @@ struct Plan : PlanBase {
         KnotsDev<real> k;
         k.x = kx[b];
         k.y = ky[b];
+        k.cnt = kcnt[b];
         k.pre = kpre[b];
         k.n = nk[b];
+        k.fix = kfix[b];
+        k.nbx = knbx[b];
+        k.nby = knby[b];
         return k;
     }
 
@@ -396,9 +488,7 @@ struct Plan : PlanBase {
 
     int reset_control(hipStream_t st)
     {
-        ICM_TRY(launch_fill_u32((uint32_t*)tickets, 0u, kEpochs, st));
         ICM_TRY(launch_fill_u32((uint32_t*)wincount, 0u, kEpochs, st));
-        ICM_TRY(launch_fill_u32((uint32_t*)status, 0u, 2 * ntiles_max, st));
         epoch = 0;
         return ICM_OK;
     }
@@ -419,11 +509,15 @@ struct Plan : PlanBase {
     int stage_fixed(int phase, int stage, const DevSeries& d, const InitArgs<real>& ia, int S,
                     real* out, hipStream_t st)
     {
-        const SeriesDev sd = d.view(kstride);
+        const SeriesDev sd = d.view(kstride, sstride, cap);
         int rc;
         if (epoch + (uint32_t)S + 8 >= kEpochs - 1 && (rc = reset_control(st))) return rc;
+        const int64_t pass_bytes = 2 * (int64_t)sizeof(real) * d.samples;
         uint32_t e_emit = next_epoch();
+        if ((rc = prof_begin(0, pass_bytes, st))) return rc;
         ICM_TRY(launch_init<real>(sd, ia, hA, knots(0), ctl(e_emit), st));
+        if ((rc = prof_end(st))) return rc;
+        ICM_TRY(launch_knot_prefix<real>(sd, knots(0), ctl(e_emit), nullptr, st));
         if ((rc = trace_copy(phase, stage, 0, d, 0, st))) return rc;
         for (int j = 0; j < S; ++j) {
             const int b = j & 1;
@@ -435,9 +529,12 @@ struct Plan : PlanBase {
             sa.wincount = wincount + e_emit;
             sa.active = nullptr;
             sa.err = err;
+            if ((rc = prof_begin(1, 0, st))) return rc;
             ICM_TRY(launch_solve<real>(sa, solve_grid(), st));
+            if ((rc = prof_end(st))) return rc;
             SiftArgs<real> a;
             a.sd = sd;
+            a.segs = d.segs;
             a.h_in = b ? hB : hA;
             a.h_out = (j == S - 1 && out) ? out : (b ? hA : hB);
             a.kin = knots(b);
@@ -449,8 +546,13 @@ struct Plan : PlanBase {
             a.active = nullptr;
             a.sx_cnt = nullptr;
             a.sx_mark = nullptr;
-            ICM_TRY(launch_sift<real>(a, st));
-            if (j < S - 1 && (rc = trace_copy(phase, stage, j + 1, d, b ^ 1, st))) return rc;
+            if ((rc = prof_begin(2, pass_bytes, st))) return rc;
+            ICM_TRY(launch_sift<real>(a, d.nsegs, st));
+            if ((rc = prof_end(st))) return rc;
+            if (j < S - 1) {
+                ICM_TRY(launch_knot_prefix<real>(sd, knots(b ^ 1), ctl(e_emit), nullptr, st));
+                if ((rc = trace_copy(phase, stage, j + 1, d, b ^ 1, st))) return rc;
+            }
         }
         return ICM_OK;
     }
@@ -461,7 +563,7 @@ struct Plan : PlanBase {
     int stage_rilling(int phase, int stage, const DevSeries& d, const InitArgs<real>& ia, int cap,
                       hipStream_t st)
     {
-        const SeriesDev sd = d.view(kstride);
+        const SeriesDev sd = d.view(kstride, sstride, cap);
         ICM_TRY(launch_fill_u32((uint32_t*)active, 1u, d.nseries, st));
         ICM_TRY(launch_fill_u32((uint32_t*)subs, 0u, d.nseries, st));
         ICM_TRY(launch_fill_u32((uint32_t*)stop_pass, 0u, d.nseries, st));
@@ -471,6 +573,7 @@ struct Plan : PlanBase {
         }
         uint32_t e_emit = next_epoch();
         ICM_TRY(launch_init<real>(sd, ia, hA, knots(0), ctl(e_emit), st));
+        ICM_TRY(launch_knot_prefix<real>(sd, knots(0), ctl(e_emit), nullptr, st));
         int rc;
         if ((rc = trace_copy(phase, stage, 0, d, 0, st))) return rc;
         for (int j = 0; j < cap; ++j) {
@@ -486,6 +589,7 @@ struct Plan : PlanBase {
             ICM_TRY(launch_solve<real>(sa, solve_grid(), st));
             SiftArgs<real> a;
             a.sd = sd;
+            a.segs = d.segs;
             a.h_in = b ? hB : hA;
             a.h_out = b ? hA : hB;
             a.kin = knots(b);
@@ -497,10 +601,11 @@ struct Plan : PlanBase {
             a.active = active;
             a.sx_cnt = sx_cnt;
             a.sx_mark = sx_mark;
-            ICM_TRY(launch_sift<real>(a, st));
+            ICM_TRY(launch_sift<real>(a, d.nsegs, st));
             ICM_TRY(launch_fill_u32((uint32_t*)n_active, 0u, 1, st));
             ICM_TRY(launch_rilling_decide<real>(sd, nk[b], sx_cnt, sx_mark, active, stop_pass, subs,
                                                 j, cap, n_active, st));
+            ICM_TRY(launch_knot_prefix<real>(sd, knots(b ^ 1), ctl(e_emit), active, st));
             if ((rc = trace_copy(phase, stage, j + 1, d, b ^ 1, st))) return rc;
             // early exit: results are identical to running every pass to the
             // cap with finished series masked (they never change again)
@@ -535,7 +640,7 @@ struct Plan : PlanBase {
             } else {
                 if ((rc = stage_rilling(ICM_PHASE_NOISE, k, dn, ia, cfg.noise_max_sifts, st)))
                     return rc;
-                ICM_TRY(launch_select_copy<real>(dn.view(kstride), dn.max_len, hA, hB, stop_pass,
+                ICM_TRY(launch_select_copy<real>(dn.view(kstride, sstride, cap), dn.max_len, hA, hB, stop_pass,
                                                  Ek, st));
             }
             ICM_TRY(launch_sub<real>(wres, Ek, hn.samples, st));
@@ -601,7 +706,7 @@ struct Plan : PlanBase {
             return which == 0 ? noise_phase(s) : signal_phase(s);
         };
         const bool graph = cfg.use_graph && cfg.stop_mode == ICM_STOP_FIXED && !trace_passes &&
-                           !graph_failed;
+                           !prof && !graph_failed;
         if (!graph) return body(st);
         if (!gexec[which]) {
             // capture on the plan's own stream, replay on the caller's
@@ -675,6 +780,7 @@ struct Plan : PlanBase {
         timing.noise_ms = t01;
         timing.signal_ms = t12;
         timing.total_ms = t03;
+        prof_collect();
         if (*h_err) return ICM_ERR_DEVICE;
         return ICM_OK;
     }
@@ -826,6 +932,24 @@ int icm_last_timing(icm_plan plan, icm_timing* out)
 {
     if (!plan || !out) return ICM_ERR_INVALID_ARG;
     *out = plan->p->timing;
+    return ICM_OK;
+}
+
+int icm_set_profiling(icm_plan plan, int on)
+{
+    if (!plan) return ICM_ERR_INVALID_ARG;
+    plan->p->prof = on ? 1 : 0;
+    return ICM_OK;
+}
+
+int icm_get_kernel_stats(icm_plan plan, int kind, double* total_ms, int64_t* launches,
+                         int64_t* alg_bytes)
+{
+    if (!plan || kind < 0 || kind > 2 || !total_ms || !launches || !alg_bytes)
+        return ICM_ERR_INVALID_ARG;
+    *total_ms = plan->p->kern_ms[kind];
+    *launches = plan->p->kern_n[kind];
+    *alg_bytes = plan->p->kern_bytes[kind];
     return ICM_OK;
 }
 

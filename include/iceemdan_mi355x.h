@@ -105,6 +105,17 @@ int icm_forward(icm_plan plan, const void* const* d_signals, void* const* d_imfs
                 const void* d_noise, void* stream);
 int icm_last_timing(icm_plan plan, icm_timing* out);
 
+/* Per-kernel timing with HIP events on the forward's stream (disables the
+ * hipGraph replay while on).  kind: 0 = stage init (extrema of the stage
+ * input), 1 = spline solve, 2 = sift pass.  Totals of the last forward;
+ * alg_bytes = algorithmic bytes (2 * sizeof(real) per sample per launch). */
+#define ICM_KERNEL_INIT 0
+#define ICM_KERNEL_SOLVE 1
+#define ICM_KERNEL_SIFT 2
+int icm_set_profiling(icm_plan plan, int on);
+int icm_get_kernel_stats(icm_plan plan, int kind, double* total_ms, int64_t* launches,
+                         int64_t* alg_bytes);
+
 /* Per-pass knot counts (debug/parity): enable before a forward with the
  * number of passes to keep per stage; read back [M-1][passes][series][2]
  * (n_max, n_min of each pass input) of a phase. */
