@@ -30,6 +30,8 @@ class ICMError(RuntimeError):
     def __init__(self, status, what=""):
         self.status = status
         msg = lib().icm_status_string(status).decode()
+        if status == -3:
+            msg += f" [{lib().icm_last_error().decode()}]"
         super().__init__(f"{what}: {msg} ({status})" if what else f"{msg} ({status})")
 
 
@@ -56,6 +58,7 @@ class icm_timing(C.Structure):
 # exported C symbols and their signatures (kept in step with include/*.h)
 _SIGS = {
     "icm_status_string": (C.c_char_p, [C.c_int]),
+    "icm_last_error": (C.c_char_p, []),
     "icm_config_default": (C.c_int, [C.POINTER(icm_config)]),
     "icm_validate_config": (C.c_int, [C.POINTER(icm_config)]),
     "icm_plan_bytes": (C.c_int, [C.POINTER(icm_config), C.POINTER(C.c_int64)]),

@@ -30,6 +30,16 @@
 
 namespace icm {
 
+// text of the last HIP failure seen by an entry point on this thread
+thread_local char g_last_error[256] = "";
+
+int hip_fail(hipError_t e, int line)
+{
+    std::snprintf(g_last_error, sizeof g_last_error, "icm_plan.hip:%d: %s", line,
+                  hipGetErrorString(e));
+    return ICM_ERR_HIP;
+}
+
 const int64_t kMaxLen = 16000000;
 const uint32_t kEpochs = 65536;
 
@@ -483,7 +493,8 @@ struct Plan : PlanBase {
 
 #define ICM_TRY(x)                                     \
     do {                                               \
-        if ((x) != hipSuccess) return ICM_ERR_HIP;     \
+        const hipError_t e_ = (x);                     \
+        if (e_ != hipSuccess) return hip_fail(e_, __LINE__); \
     } while (0)
 
     int reset_control(hipStream_t st)
@@ -560,14 +571,14 @@ struct Plan : PlanBase {
     // RILLING mode: passes until every series stopped or the cap is reached.
     // Returns with stop_pass[s] = pass whose input h is the exported mode
     // (buffer stop_pass & 1).
-    int stage_rilling(int phase, int stage, const DevSeries& d, const InitArgs<real>& ia, int cap,
+    int stage_rilling(int phase, int stage, const DevSeries& d, const InitArgs<real>& ia, int max_pass,
                       hipStream_t st)
     {
         const SeriesDev sd = d.view(kstride, sstride, cap);
         ICM_TRY(launch_fill_u32((uint32_t*)active, 1u, d.nseries, st));
         ICM_TRY(launch_fill_u32((uint32_t*)subs, 0u, d.nseries, st));
         ICM_TRY(launch_fill_u32((uint32_t*)stop_pass, 0u, d.nseries, st));
-        if (epoch + 2 * (uint32_t)cap + 8 >= kEpochs - 1) {
+        if (epoch + 2 * (uint32_t)max_pass + 8 >= kEpochs - 1) {
             int rc = reset_control(st);
             if (rc) return rc;
         }
@@ -576,7 +587,7 @@ struct Plan : PlanBase {
         ICM_TRY(launch_knot_prefix<real>(sd, knots(0), ctl(e_emit), nullptr, st));
         int rc;
         if ((rc = trace_copy(phase, stage, 0, d, 0, st))) return rc;
-        for (int j = 0; j < cap; ++j) {
+        for (int j = 0; j < max_pass; ++j) {
             const int b = j & 1;
             SolveArgs<real> sa;
             sa.sd = sd;
@@ -604,12 +615,12 @@ struct Plan : PlanBase {
             ICM_TRY(launch_sift<real>(a, d.nsegs, st));
             ICM_TRY(launch_fill_u32((uint32_t*)n_active, 0u, 1, st));
             ICM_TRY(launch_rilling_decide<real>(sd, nk[b], sx_cnt, sx_mark, active, stop_pass, subs,
-                                                j, cap, n_active, st));
+                                                j, max_pass, n_active, st));
             ICM_TRY(launch_knot_prefix<real>(sd, knots(b ^ 1), ctl(e_emit), active, st));
             if ((rc = trace_copy(phase, stage, j + 1, d, b ^ 1, st))) return rc;
             // early exit: results are identical to running every pass to the
             // cap with finished series masked (they never change again)
-            const bool poll = (j < 16) || ((j & 3) == 3) || (j == cap - 1);
+            const bool poll = (j < 16) || ((j & 3) == 3) || (j == max_pass - 1);
             if (poll) {
                 ICM_TRY(hipMemcpyAsync(h_n_active, n_active, sizeof(unsigned),
                                        hipMemcpyDeviceToHost, st));
@@ -821,6 +832,8 @@ struct icm_plan_s {
 };
 
 extern "C" {
+
+const char* icm_last_error(void) { return g_last_error; }
 
 const char* icm_status_string(int s)
 {

@@ -81,6 +81,8 @@ typedef struct icm_timing {
 } icm_timing;
 
 const char* icm_status_string(int status);
+/* Detail of the last HIP runtime failure on the calling thread ("" if none). */
+const char* icm_last_error(void);
 
 /* Host-only checks (no device calls). */
 int icm_config_default(icm_config* cfg);
