@@ -52,7 +52,8 @@ class icm_plan_info(C.Structure):
 
 
 class icm_timing(C.Structure):
-    _fields_ = [("noise_ms", C.c_double), ("signal_ms", C.c_double), ("total_ms", C.c_double)]
+    _fields_ = [("noise_ms", C.c_double), ("signal_ms", C.c_double), ("total_ms", C.c_double),
+                ("backward_ms", C.c_double)]
 
 
 # exported C symbols and their signatures (kept in step with include/*.h)
@@ -79,7 +80,7 @@ _SIGS = {
     "icm_sift_pass": (C.c_int, [C.c_int, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p,
                                 C.c_void_p, C.c_void_p]),
     "icm_backward": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
-                               C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p]),
+                               C.c_void_p]),
     "icm_stats_mean": (C.c_int, [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "icm_stats_variance": (C.c_int, [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
                                      C.c_void_p, C.c_void_p]),
@@ -269,6 +270,32 @@ class Plan:
         _check(lib().icm_forward(self._h, sp, op,
                                  C.c_void_p(noise.data_ptr()) if noise is not None else None, st),
                "icm_forward")
+        self.forward_count = getattr(self, "forward_count", 0) + 1
+        return out
+
+    def backward(self, grad_imfs, out=None):
+        """dL/dx of the last forward from dL/dIMF (list of [num_imfs, N_c]
+        device tensors), FIXED stop rule, extrema sets frozen at the
+        forward's.  Returns a list of [N_c] tensors."""
+        torch = _torch()
+        if isinstance(grad_imfs, torch.Tensor):
+            grad_imfs = [grad_imfs] if grad_imfs.dim() == 2 else list(grad_imfs)
+        if len(grad_imfs) != len(self.lengths):
+            raise ValueError("number of channels differs from the plan")
+        gs = []
+        for g, n in zip(grad_imfs, self.lengths):
+            g = g.contiguous()
+            if g.numel() != n * self.num_imfs or g.dtype != self.dtype:
+                raise ValueError("gradient shape/dtype differs from the plan")
+            _require_cuda(g)
+            gs.append(g)
+        if out is None:
+            out = [torch.empty(n, dtype=self.dtype, device=gs[0].device) for n in self.lengths]
+        for o in out:
+            _require_cuda(o)
+        gp = (C.c_void_p * len(gs))(*[g.data_ptr() for g in gs])
+        op = (C.c_void_p * len(out))(*[o.data_ptr() for o in out])
+        _check(lib().icm_backward(self._h, gp, op, _stream_ptr(gs[0].device)), "icm_backward")
         return out
 
     def set_profiling(self, on):
@@ -286,7 +313,8 @@ class Plan:
     def timing(self):
         t = icm_timing()
         _check(lib().icm_last_timing(self._h, C.byref(t)))
-        return {"noise_ms": t.noise_ms, "signal_ms": t.signal_ms, "total_ms": t.total_ms}
+        return {"noise_ms": t.noise_ms, "signal_ms": t.signal_ms, "total_ms": t.total_ms,
+                "backward_ms": t.backward_ms}
 
     def noise_modes(self):
         torch = _torch()
@@ -335,6 +363,44 @@ def iceemdan(x, num_noise, num_imfs, max_sifts=10, stop_mode=STOP_FIXED, noise_s
         return plan.forward([x.contiguous()], noise=noise)[0]
     finally:
         plan.close()
+
+
+def _autograd_function():
+    torch = _torch()
+
+    class ICEEMDANFunction(torch.autograd.Function):
+        """x (1-D device tensor) -> [num_imfs, N] with a HIP backward.
+
+        The extrema sets (and endpoint-fix branches) of every sift pass are
+        held at the forward's values, so the backward is the exact gradient
+        of the piecewise-linear map the forward evaluated."""
+
+        @staticmethod
+        def forward(ctx, x, plan, noise):
+            out = plan.forward([x.detach().contiguous()], noise=noise)[0]
+            ctx.plan = plan
+            ctx.count = plan.forward_count
+            return out
+
+        @staticmethod
+        def backward(ctx, g):
+            if ctx.plan.forward_count != ctx.count:
+                raise RuntimeError("the plan ran another forward before this backward")
+            return ctx.plan.backward([g.contiguous()])[0], None, None
+
+    return ICEEMDANFunction
+
+
+_FN = None
+
+
+def iceemdan_autograd(x, plan, noise=None):
+    """Differentiable decomposition of one channel with a FIXED-mode plan
+    whose single channel has x's length (torch.autograd)."""
+    global _FN
+    if _FN is None:
+        _FN = _autograd_function()
+    return _FN.apply(x, plan, noise)
 
 
 # ---- mirror of the reference entry points (cudaICEEMDAN.cuh:25-26) ----

@@ -108,7 +108,49 @@ struct SolveArgs {
     const unsigned* wincount;
     const int* active;
     int* err;
+    int* px;                // optional [2][kstride]: knot positions, compact (for the adjoint)
+    int* fixf;              // optional [nseries][2][2]: endpoint fix applied (for the adjoint)
 };
+
+// ---------------------------------------------------------------- adjoint
+// Recorded structure of one forward pass (the knots of its input samples).
+struct PassRec {
+    int* px;     // [2][kstride] knot positions by knot index
+    int* cnt;    // [ntiles][2]
+    int* pre;    // [ntiles][2]
+    int* n;      // [nseries][2]
+    int* fixf;   // [nseries][2][2]
+};
+
+template <typename real>
+struct AdjArgs {
+    SeriesDev sd;
+    PassRec ps;
+    real* mu;             // gradient w.r.t. the pass output; becomes the pass input's
+    real* mom;            // [2][kstride][4] per-interval moments of mu
+    real* edge;           // [ntiles][2][2][4] moments of the runs at tile edges
+    real* ga;             // [2][kstride] gradient w.r.t. the knot values
+    const int* winlist;
+    const unsigned* wincount;
+    int* err;
+};
+
+template <typename real>
+hipError_t launch_adjoint_pass(const AdjArgs<real>& a, int solve_grid, hipStream_t st);
+template <typename real>
+hipError_t launch_adj_seed(const real* gimf, int M, int64_t N, int64_t stride, real* gr,
+                           hipStream_t st);
+template <typename real>
+hipError_t launch_adj_broadcast(const SeriesDev& sd, const real* gr_next, const int64_t* chan_off,
+                                real* mu, int max_len, hipStream_t st);
+template <typename real>
+hipError_t launch_adj_stage(int stage, int nchan, int I, int max_len, const int* chan_len,
+                            const int64_t* chan_off, const SeriesDev& sd, const real* gr_next,
+                            const real* mu, const real* E, const real* r_k, real eps0, real* gr_k,
+                            real* gacc, real* dots, hipStream_t st);
+template <typename real>
+hipError_t launch_adj_input(int nchan, const int* chan_len, const int64_t* chan_off, const real* y,
+                            const real* gxt, real* gy, hipStream_t st);
 
 // host-side launchers (icm_kernels.hip)
 template <typename real>

@@ -790,6 +790,87 @@ __device__ void gather_knots(const SeriesDev& sd, const KnotsDev<real>& k, int s
     }
 }
 
+// Solve one window of W = 64*B rows of a diagonally dominant tridiagonal
+// system, one wavefront, lane l owning rows [l*B, l*B+B): per-lane
+// elimination of the B-1 interior rows (x_q = al_q + be_q z_{l-1} + ga_q z_l),
+// a reduced tridiagonal system in the lanes' last rows z_l solved by parallel
+// cyclic reduction across the wave, then back substitution.  Coupling beyond
+// the window's first and last rows is dropped.  row(r, dl, d, du, f) gives
+// row r of the window.  x[q] = solution of row l*B + q.
+template <typename real, int B, class RowFn>
+__device__ __forceinline__ void partition_pcr(int lane, RowFn row, real (&x)[B])
+{
+    real cp[B], al[B], be[B], ga[B];
+    const int rb = lane * B;
+#pragma unroll
+    for (int q = 0; q <= B - 2; ++q) {
+        real dl, d, du, f;
+        row(rb + q, dl, d, du, f);
+        real inv;
+        if (q == 0) {
+            const real dl0 = (lane == 0) ? (real)0 : dl;
+            inv = (real)1 / d;
+            al[0] = f * inv;
+            be[0] = (-dl0) * inv;
+        } else {
+            inv = (real)1 / (d - dl * cp[q - 1]);
+            al[q] = (f - dl * al[q - 1]) * inv;
+            be[q] = (-dl) * be[q - 1] * inv;
+        }
+        cp[q] = du * inv;
+        ga[q] = (q == B - 2) ? (-du) * inv : (real)0;
+    }
+#pragma unroll
+    for (int q = B - 3; q >= 0; --q) {
+        al[q] = al[q] - cp[q] * al[q + 1];
+        be[q] = be[q] - cp[q] * be[q + 1];
+        ga[q] = ga[q] - cp[q] * ga[q + 1];
+    }
+    // interface row q = B-1 -> reduced tridiagonal system in z_lane
+    real dl, d, du, f;
+    row(rb + B - 1, dl, d, du, f);
+    real na0 = shfl_down(al[0], 1), nb0 = shfl_down(be[0], 1), ng0 = shfl_down(ga[0], 1);
+    if (lane == kWave - 1) {
+        du = (real)0;
+        na0 = (real)0;
+        nb0 = (real)0;
+        ng0 = (real)0;
+    }
+    real A = dl * be[B - 2];
+    real D = (d + dl * ga[B - 2]) + du * nb0;
+    real C = du * ng0;
+    real R = (f - dl * al[B - 2]) - du * na0;
+#pragma unroll
+    for (int st = 1; st < kWave; st <<= 1) {
+        const real Am = shfl_up(A, st), Dm = shfl_up(D, st), Cm = shfl_up(C, st),
+                   Rm = shfl_up(R, st);
+        const real Ap = shfl_down(A, st), Dp = shfl_down(D, st), Cp = shfl_down(C, st),
+                   Rp = shfl_down(R, st);
+        real a2 = (real)0, c2 = (real)0, d2 = D, r2 = R;
+        if (lane - st >= 0) {
+            const real k1 = A / Dm;
+            a2 = -(Am * k1);
+            d2 = d2 - Cm * k1;
+            r2 = r2 - Rm * k1;
+        }
+        if (lane + st < kWave) {
+            const real k2 = C / Dp;
+            c2 = -(Cp * k2);
+            d2 = d2 - Ap * k2;
+            r2 = r2 - Rp * k2;
+        }
+        A = a2;
+        C = c2;
+        D = d2;
+        R = r2;
+    }
+    const real z = R / D;
+    real zl = shfl_up(z, 1);
+    if (lane == 0) zl = (real)0;
+#pragma unroll
+    for (int q = 0; q < B; ++q) x[q] = (q == B - 1) ? z : (al[q] + be[q] * zl) + ga[q] * z;
+}
+
 template <typename real, int B, int H>
 __global__ __launch_bounds__(kThreads) void k_solve(SolveArgs<real> a)
 {
@@ -823,106 +904,58 @@ __global__ __launch_bounds__(kThreads) void k_solve(SolveArgs<real> a)
             const long keep0 = (long)w * K;
             if (lo <= 0) {
                 const int i0 = (int)(-lo);
+                int replaced = 0;
                 if (n >= 6) {
                     const real slope =
                         (sy[i0 + 2] - sy[i0 + 1]) / (real)(sp[i0 + 2] - sp[i0 + 1]);
                     const real t = sy[i0 + 1] - slope * (real)(sp[i0 + 1] - sp[i0]);
-                    if (type == 0 ? (t > sy[i0]) : (t < sy[i0])) sy[i0] = t;
+                    if (type == 0 ? (t > sy[i0]) : (t < sy[i0])) {
+                        sy[i0] = t;
+                        replaced = 1;
+                    }
                 }
-                if (w == 0) a.k.fix[4 * s + 2 * type] = sy[i0];
+                if (w == 0) {
+                    a.k.fix[4 * s + 2 * type] = sy[i0];
+                    if (a.fixf) a.fixf[4 * s + 2 * type] = replaced;
+                }
             }
             const long ie = (long)(n - 1) - lo;
             if (ie >= 0 && ie < W + 2) {
                 const int i1 = (int)ie;
+                int replaced = 0;
                 if (n >= 6) {
                     const real slope =
                         (sy[i1 - 1] - sy[i1 - 2]) / (real)(sp[i1 - 1] - sp[i1 - 2]);
                     const real t = sy[i1 - 1] + slope * (real)(sp[i1] - sp[i1 - 1]);
-                    if (type == 0 ? (t > sy[i1]) : (t < sy[i1])) sy[i1] = t;
+                    if (type == 0 ? (t > sy[i1]) : (t < sy[i1])) {
+                        sy[i1] = t;
+                        replaced = 1;
+                    }
                 }
-                if (n - 1 >= keep0 && n - 1 < keep0 + K) a.k.fix[4 * s + 2 * type + 1] = sy[i1];
+                if (n - 1 >= keep0 && n - 1 < keep0 + K) {
+                    a.k.fix[4 * s + 2 * type + 1] = sy[i1];
+                    if (a.fixf) a.fixf[4 * s + 2 * type + 1] = replaced;
+                }
             }
         }
         wave_sync();
 
-        // per-lane elimination of the interior rows q = 0..B-2:
-        // x_q = al_q + be_q * z_{lane-1} + ga_q * z_lane
-        real cp[B], al[B], be[B], ga[B];
-        const int rb = lane * B;
-#pragma unroll
-        for (int q = 0; q <= B - 2; ++q) {
-            real dl, d, du, f;
-            spline_row<real>(n, g0 + rb + q, rb + q + 1, sp, sy, dl, d, du, f);
-            real inv;
-            if (q == 0) {
-                const real dl0 = (lane == 0) ? (real)0 : dl;
-                inv = (real)1 / d;
-                al[0] = f * inv;
-                be[0] = (-dl0) * inv;
-            } else {
-                inv = (real)1 / (d - dl * cp[q - 1]);
-                al[q] = (f - dl * al[q - 1]) * inv;
-                be[q] = (-dl) * be[q - 1] * inv;
-            }
-            cp[q] = du * inv;
-            ga[q] = (q == B - 2) ? (-du) * inv : (real)0;
-        }
-#pragma unroll
-        for (int q = B - 3; q >= 0; --q) {
-            al[q] = al[q] - cp[q] * al[q + 1];
-            be[q] = be[q] - cp[q] * be[q + 1];
-            ga[q] = ga[q] - cp[q] * ga[q + 1];
-        }
-        // interface row q = B-1 -> reduced tridiagonal system in z_lane
-        real dl, d, du, f;
-        spline_row<real>(n, g0 + rb + B - 1, rb + B, sp, sy, dl, d, du, f);
-        real na0 = shfl_down(al[0], 1), nb0 = shfl_down(be[0], 1), ng0 = shfl_down(ga[0], 1);
-        if (lane == kWave - 1) {
-            du = (real)0;
-            na0 = (real)0;
-            nb0 = (real)0;
-            ng0 = (real)0;
-        }
-        real A = dl * be[B - 2];
-        real D = (d + dl * ga[B - 2]) + du * nb0;
-        real C = du * ng0;
-        real R = (f - dl * al[B - 2]) - du * na0;
-        // parallel cyclic reduction across the wave
-#pragma unroll
-        for (int st = 1; st < kWave; st <<= 1) {
-            const real Am = shfl_up(A, st), Dm = shfl_up(D, st), Cm = shfl_up(C, st),
-                       Rm = shfl_up(R, st);
-            const real Ap = shfl_down(A, st), Dp = shfl_down(D, st), Cp = shfl_down(C, st),
-                       Rp = shfl_down(R, st);
-            real a2 = (real)0, c2 = (real)0, d2 = D, r2 = R;
-            if (lane - st >= 0) {
-                const real k1 = A / Dm;
-                a2 = -(Am * k1);
-                d2 = d2 - Cm * k1;
-                r2 = r2 - Rm * k1;
-            }
-            if (lane + st < kWave) {
-                const real k2 = C / Dp;
-                c2 = -(Cp * k2);
-                d2 = d2 - Ap * k2;
-                r2 = r2 - Rp * k2;
-            }
-            A = a2;
-            C = c2;
-            D = d2;
-            R = r2;
-        }
-        const real z = R / D;
-        real zl = shfl_up(z, 1);
-        if (lane == 0) zl = (real)0;
+        real xs[B];
+        partition_pcr<real, B>(
+            lane,
+            [&](int r, real& dl, real& d, real& du, real& f) {
+                spline_row<real>(n, g0 + r, r + 1, sp, sy, dl, d, du, f);
+            },
+            xs);
         real* cout = a.c + kb;
+        const int rb = lane * B;
 #pragma unroll
         for (int q = 0; q < B; ++q) {
             const int r = rb + q;
             const long gg = g0 + r;
             if (r >= H && r < H + K && gg < n) {
-                const real x = (q == B - 1) ? z : (al[q] + be[q] * zl) + ga[q] * z;
-                cout[gg] = x;
+                cout[gg] = xs[q];
+                if (a.px) a.px[kb + gg] = sp[r + 1];
             }
         }
         wave_sync();
@@ -1004,6 +1037,420 @@ __global__ __launch_bounds__(1024) void k_series_var(const real* base, const int
     }
     const real ss = canon_sum_block<real>(acc, sred);
     if (threadIdx.x == 0) var_out[s] = ss * (real)(1.0 / (double)(N - 1));
+}
+
+// ================================================================ adjoint
+// With the extrema sets of every pass frozen (recorded by the forward), one
+// sift pass is linear: h' = h - (U + L) / 2.  Its transpose maps the
+// gradient mu of h' to that of h: mu - (G_max^T mu + G_min^T mu) / 2, where
+// G^T is: per-interval moments of mu (transposed evaluation), transposed
+// coefficient formulas, a solve with the transposed spline matrix (its
+// interior block is the symmetric interior block of the forward matrix, so
+// the forward's windowed solver is reused), transposed right-hand side and
+// endpoint fix, and a scatter to the knot samples.
+
+// 1. moments sum_p mu_p t^m, m = 0..3, of the runs of samples of each knot
+// interval inside a tile (interval k = (x_k, x_{k+1}], sample 0 in interval 0).
+// Runs bounded by two knots of the tile are complete intervals; the first
+// and last run of a tile go to the edge array for k_adj_span.
+template <typename real, int T>
+__global__ __launch_bounds__(kThreads) void k_adj_mom(AdjArgs<real> a)
+{
+    constexpr int KC = T / 2 + 2;
+    __shared__ real sterm[T][4];
+    __shared__ int skp[KC];
+    const int tid = threadIdx.x;
+    const SeriesDev& sd = a.sd;
+    const int gt = blockIdx.x;
+    const int s = sd.tile_series[gt];
+    const int t = gt - sd.tile0[s];
+    const int N = sd.len[s];
+    const int t0 = t * T;
+    const int nt = min(T, N - t0);
+    const int n0 = a.ps.n[2 * s], n1 = a.ps.n[2 * s + 1];
+    if (!(n0 > 3 && n1 > 3)) return;   // the pass did not subtract: identity
+    const real* mu = a.mu + sd.off[s];
+    for (int ty = 0; ty < 2; ++ty) {
+        const int64_t kb = sd.kbase[s] + (ty ? sd.kstride : 0);
+        const int cnt = a.ps.cnt[2 * gt + ty];
+        const int pre = a.ps.pre[2 * gt + ty];
+        for (int i = tid; i < cnt; i += kThreads) skp[i] = a.ps.px[kb + pre + i];
+        const int left = pre > 0 ? a.ps.px[kb + pre - 1] : 0;
+        __syncthreads();
+        // run of each sample = knots of the tile strictly before it
+        for (int u = tid; u < nt; u += kThreads) {
+            int lo = 0, hi = cnt;           // first knot >= sample
+            const int p = t0 + u;
+            while (lo < hi) {
+                const int m = (lo + hi) >> 1;
+                if (skp[m] < p) lo = m + 1;
+                else hi = m;
+            }
+            const int r = lo;
+            int lp = r >= 1 ? skp[r - 1] : left;
+            if (t0 == 0 && r == 0) lp = 0;   // sample 0 belongs to interval 0
+            const real tt = (real)(p - lp);
+            const real v = mu[p];
+            sterm[u][0] = v;
+            sterm[u][1] = v * tt;
+            sterm[u][2] = v * tt * tt;
+            sterm[u][3] = v * tt * tt * tt;
+        }
+        __syncthreads();
+        for (int r = tid; r <= cnt; r += kThreads) {
+            int start = r == 0 ? 0 : skp[r - 1] - t0 + 1;
+            const int end = r == cnt ? nt - 1 : skp[r] - t0;
+            if (t0 == 0 && r == 1) start = 0;     // sample 0 joins interval 0
+            if (t0 == 0 && r == 0) continue;
+            real m0 = (real)0, m1 = (real)0, m2 = (real)0, m3 = (real)0;
+            for (int u = start; u <= end; ++u) {
+                m0 += sterm[u][0];
+                m1 += sterm[u][1];
+                m2 += sterm[u][2];
+                m3 += sterm[u][3];
+            }
+            real* dst;
+            if (r >= 1 && r < cnt) dst = a.mom + 4 * (kb + pre + r - 1);
+            else dst = a.edge + 4 * (4 * (int64_t)gt + 2 * ty + (r == 0 ? 0 : 1));
+            dst[0] = m0;
+            dst[1] = m1;
+            dst[2] = m2;
+            dst[3] = m3;
+        }
+        __syncthreads();
+    }
+}
+
+// 2. intervals that leave their tile: the last run of the tile holding the
+// left knot plus the first runs of the following tiles up to the right knot
+template <typename real>
+__global__ void k_adj_span(AdjArgs<real> a)
+{
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const SeriesDev& sd = a.sd;
+    if (gt >= sd.ntiles) return;
+    const int s = sd.tile_series[gt];
+    const int n0 = a.ps.n[2 * s], n1 = a.ps.n[2 * s + 1];
+    if (!(n0 > 3 && n1 > 3)) return;
+    for (int ty = 0; ty < 2; ++ty) {
+        const int cnt = a.ps.cnt[2 * gt + ty];
+        if (cnt == 0) continue;
+        const int g = a.ps.pre[2 * gt + ty] + cnt - 1;
+        if (g >= (ty ? n1 : n0) - 1) continue;
+        const real* e = a.edge + 4 * (4 * (int64_t)gt + 2 * ty + 1);
+        real m0 = e[0], m1 = e[1], m2 = e[2], m3 = e[3];
+        const int last = sd.tile0[s] + sd.ntl[s] - 1;
+        for (int tau = gt + 1; tau <= last; ++tau) {
+            const real* f = a.edge + 4 * (4 * (int64_t)tau + 2 * ty);
+            m0 += f[0];
+            m1 += f[1];
+            m2 += f[2];
+            m3 += f[3];
+            if (a.ps.cnt[2 * tau + ty] > 0) break;
+        }
+        real* dst = a.mom + 4 * (sd.kbase[s] + (ty ? sd.kstride : 0) + g);
+        dst[0] = m0;
+        dst[1] = m1;
+        dst[2] = m2;
+        dst[3] = m3;
+    }
+}
+
+// 3. knot gradients: right-hand side gc of the transposed solve from the
+// moments (transposed coefficient formulas, cudaICEEMDAN.cu:394-398), the
+// solve, the transposed right-hand side f(a) (:333-341) and endpoint fix
+// (:209-291).  One wavefront per window, the forward's window geometry.
+template <typename real, int B, int H>
+__global__ __launch_bounds__(kThreads) void k_adj_knot(AdjArgs<real> a)
+{
+    constexpr int W = kWave * B;
+    constexpr int K = W - 2 * H;
+    constexpr int NW = kThreads / kWave;
+    __shared__ int s_pos[NW][W + 4];       // knots lo-1 .. lo+W+2 (index + 1)
+    __shared__ real s_m[NW][W + 4][4];     // moments of the interval starting at the knot
+    __shared__ real s_f[NW][W + 2];        // gc, then the solution gf
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int* sp = s_pos[wv];
+    real(*sm)[4] = s_m[wv];
+    real* sf = s_f[wv];
+    const unsigned total = *a.wincount;
+    for (unsigned item = blockIdx.x * NW + wv; item < total; item += gridDim.x * NW) {
+        const int s = a.winlist[2 * item];
+        const int code = a.winlist[2 * item + 1];
+        const int type = code & 1;
+        const int w = code >> 1;
+        const int n0 = a.ps.n[2 * s], n1 = a.ps.n[2 * s + 1];
+        if (!(n0 > 3 && n1 > 3)) continue;
+        const int n = type ? n1 : n0;
+        const int64_t kb = a.sd.kbase[s] + (type ? a.sd.kstride : 0);
+        const long g0 = (long)w * K - H;
+        const long lo = g0 - 2;                // LDS index i <-> knot lo + i
+        for (int i = lane; i < W + 4; i += kWave) {
+            const long g = lo + i;
+            if (g >= 0 && g < n) {
+                sp[i] = a.ps.px[kb + g];
+                const real* m = a.mom + 4 * (kb + g);
+                if (g < n - 1) {
+                    sm[i][0] = m[0];
+                    sm[i][1] = m[1];
+                    sm[i][2] = m[2];
+                    sm[i][3] = m[3];
+                } else {
+                    sm[i][0] = sm[i][1] = sm[i][2] = sm[i][3] = (real)0;
+                }
+            }
+        }
+        wave_sync();
+        // gc of window rows r (knot g = g0 + r, LDS index r + 2)
+        for (int r = lane; r < W; r += kWave) {
+            const long g = g0 + r;
+            real gc = (real)0;
+            if (g >= 1 && g <= n - 2) {
+                const int i = r + 2;
+                const real hr = (real)(sp[i + 1] - sp[i]);
+                const real hl = (real)(sp[i] - sp[i - 1]);
+                gc = sm[i][2] - (real)2 * hr / (real)3 * sm[i][1] - sm[i][3] / ((real)3 * hr);
+                gc = gc - hl / (real)3 * sm[i - 1][1] + sm[i - 1][3] / ((real)3 * hl);
+            }
+            sf[r] = gc;
+        }
+        wave_sync();
+        real xs[B];
+        partition_pcr<real, B>(
+            lane,
+            [&](int r, real& dl, real& d, real& du, real& f) {
+                const long g = g0 + r;
+                if (g <= 0 || g >= n - 1) {
+                    dl = (real)0; d = (real)1; du = (real)0; f = (real)0;
+                    return;
+                }
+                const int i = r + 2;
+                const int hl = sp[i] - sp[i - 1];
+                const int hr = sp[i + 1] - sp[i];
+                dl = (real)hl;
+                du = (real)hr;
+                d = (real)(2 * (hl + hr));
+                f = sf[r];
+            },
+            xs);
+        wave_sync();
+#pragma unroll
+        for (int q = 0; q < B; ++q) sf[lane * B + q] = xs[q];
+        wave_sync();
+        // base knot gradient of every row with both neighbours in the window
+        real gav[B];
+#pragma unroll
+        for (int q = 0; q < B; ++q) {
+            const int r = lane * B + q;
+            const long g = g0 + r;
+            real v = (real)0;
+            if (g >= 0 && g < n && r >= 1 && r <= W - 2) {
+                const int i = r + 2;
+                if (g <= n - 2) {
+                    const real hr = (real)(sp[i + 1] - sp[i]);
+                    v = sm[i][0] - sm[i][1] / hr;
+                }
+                if (g >= 1) {
+                    const real hl = (real)(sp[i] - sp[i - 1]);
+                    v = v + sm[i - 1][1] / hl;
+                }
+                if (g - 1 >= 1 && g - 1 <= n - 2) {
+                    const real hl = (real)(sp[i] - sp[i - 1]);
+                    v = v + (real)3 * sf[r - 1] / hl;
+                }
+                if (g >= 1 && g <= n - 2) {
+                    const real hl = (real)(sp[i] - sp[i - 1]);
+                    const real hr = (real)(sp[i + 1] - sp[i]);
+                    v = v - ((real)3 * sf[r] / hr + (real)3 * sf[r] / hl);
+                }
+                if (g + 1 >= 1 && g + 1 <= n - 2) {
+                    const real hr = (real)(sp[i + 1] - sp[i]);
+                    v = v + (real)3 * sf[r + 1] / hr;
+                }
+            }
+            gav[q] = v;
+        }
+        wave_sync();
+#pragma unroll
+        for (int q = 0; q < B; ++q) sf[lane * B + q] = gav[q];
+        wave_sync();
+        // endpoint fix: a_0 = a_1 (1 + rho) - a_2 rho when it replaced a_0
+        if (lane == 0 && n >= 6) {
+            const long r0 = 0 - g0;
+            if (r0 >= 1 && r0 + 2 <= W - 2 && a.ps.fixf[4 * s + 2 * type]) {
+                const int i = (int)r0 + 2;
+                const real rho = (real)(sp[i + 1] - sp[i]) / (real)(sp[i + 2] - sp[i + 1]);
+                sf[r0 + 1] = sf[r0 + 1] + sf[r0] * ((real)1 + rho);
+                sf[r0 + 2] = sf[r0 + 2] - sf[r0] * rho;
+                sf[r0] = (real)0;
+            }
+            const long r1 = (long)(n - 1) - g0;
+            if (r1 - 2 >= 1 && r1 <= W - 2 && a.ps.fixf[4 * s + 2 * type + 1]) {
+                const int i = (int)r1 + 2;
+                const real rho = (real)(sp[i] - sp[i - 1]) / (real)(sp[i - 1] - sp[i - 2]);
+                sf[r1 - 1] = sf[r1 - 1] + sf[r1] * ((real)1 + rho);
+                sf[r1 - 2] = sf[r1 - 2] - sf[r1] * rho;
+                sf[r1] = (real)0;
+            }
+        }
+        wave_sync();
+        for (int r = H + lane; r < H + K; r += kWave) {
+            const long g = g0 + r;
+            if (g < n) a.ga[kb + g] = sf[r];
+        }
+        wave_sync();
+    }
+}
+
+// 4. mu <- mu - (scatter of ga_max + scatter of ga_min) / 2
+template <typename real, int T>
+__global__ __launch_bounds__(kThreads) void k_adj_scatter(AdjArgs<real> a)
+{
+    __shared__ real sd_acc[T];
+    const int tid = threadIdx.x;
+    const SeriesDev& sd = a.sd;
+    const int gt = blockIdx.x;
+    const int s = sd.tile_series[gt];
+    const int t = gt - sd.tile0[s];
+    const int N = sd.len[s];
+    const int t0 = t * T;
+    const int nt = min(T, N - t0);
+    const int n0 = a.ps.n[2 * s], n1 = a.ps.n[2 * s + 1];
+    if (!(n0 > 3 && n1 > 3)) return;
+    real* mu = a.mu + sd.off[s];
+    for (int u = tid; u < T; u += kThreads) sd_acc[u] = (real)0;
+    __syncthreads();
+    for (int ty = 0; ty < 2; ++ty) {
+        const int64_t kb = sd.kbase[s] + (ty ? sd.kstride : 0);
+        const int cnt = a.ps.cnt[2 * gt + ty];
+        const int pre = a.ps.pre[2 * gt + ty];
+        for (int i = tid; i < cnt; i += kThreads) {
+            const int x = a.ps.px[kb + pre + i];
+            sd_acc[x - t0] = sd_acc[x - t0] + a.ga[kb + pre + i];
+        }
+        __syncthreads();
+    }
+    for (int u = tid; u < nt; u += kThreads) {
+        const int p = t0 + u;
+        mu[p] = mu[p] - sd_acc[u] / (real)2;
+    }
+}
+
+// dL/dr_k from dL/dIMF of one channel ([M][N]): gr_0 = g_0,
+// gr_k = g_k - g_{k-1} (IMF_0 = x~ - r_1, IMF_k = r_k - r_{k+1}, IMF_{M-1} = r_{M-1})
+template <typename real>
+__global__ void k_adj_seed(const real* gimf, int M, int N, int64_t stride, real* gr)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    gr[p] = gimf[p];
+    for (int k = 1; k < M; ++k)
+        gr[(int64_t)k * stride + p] = gimf[(int64_t)k * N + p] - gimf[(int64_t)(k - 1) * N + p];
+}
+
+// stage level: mu of every realization starts as dL/dr_{k+1} of its channel
+template <typename real>
+__global__ void k_adj_broadcast(SeriesDev sd, const real* gr_next, const int64_t* chan_off, real* mu)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int s = blockIdx.y; s < sd.nseries; s += gridDim.y) {
+        if (p >= sd.len[s]) continue;
+        mu[sd.off[s] + p] = gr_next[chan_off[sd.chan[s]] + p];
+    }
+}
+
+// g_x_i = (lambda - mu_i) / I; gacc = sum_i g_x_i (sequential in i)
+template <typename real>
+__global__ void k_adj_accum(int I, const int* chan_len, const int64_t* chan_off, SeriesDev sd,
+                            const real* gr_next, const real* mu, real* gacc)
+{
+    const int c = blockIdx.y;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= chan_len[c]) return;
+    const real lam = gr_next[chan_off[c] + p];
+    real acc = (real)0;
+    for (int i = 0; i < I; ++i) {
+        const int s = c * I + i;
+        acc = acc + (lam - mu[sd.off[s] + p]) / (real)I;
+    }
+    gacc[chan_off[c] + p] = acc;
+}
+
+// dots[s] = <g_x_s, E_s> (canonical order), one workgroup per series
+template <typename real>
+__global__ __launch_bounds__(1024) void k_adj_dot(int I, SeriesDev sd, const int64_t* chan_off,
+                                                  const real* gr_next, const real* mu,
+                                                  const real* E, real* dots)
+{
+    __shared__ real sred[1024];
+    const int s = blockIdx.x;
+    const int N = sd.len[s];
+    const real* lam = gr_next + chan_off[sd.chan[s]];
+    const real* m = mu + sd.off[s];
+    const real* e = E + sd.eoff[s];
+    real acc = (real)0;
+    for (int p = threadIdx.x; p < N; p += 1024) acc = acc + ((lam[p] - m[p]) / (real)I) * e[p];
+    const real d = canon_sum_block<real>(acc, sred);
+    if (threadIdx.x == 0) dots[s] = d;
+}
+
+// gr_k += gacc (+ eps0 * sum_i dots * (r_k - mean) / ((N-1) std(r_k)), k >= 1)
+template <typename real>
+__global__ __launch_bounds__(1024) void k_adj_stage(int stage, int I, const int* chan_len,
+                                                    const int64_t* chan_off, const real* gacc,
+                                                    const real* dots, const real* r_k, real eps0,
+                                                    real* gr_k)
+{
+    __shared__ real sred[1024];
+    const int c = blockIdx.x;
+    const int N = chan_len[c];
+    const int64_t o = chan_off[c];
+    real coef = (real)0, mu_r = (real)0;
+    if (stage > 0) {
+        real acc = (real)0;
+        for (int p = threadIdx.x; p < N; p += 1024) acc = acc + r_k[o + p];
+        mu_r = canon_sum_block<real>(acc, sred) * (real)(1.0 / (double)N);
+        acc = (real)0;
+        for (int p = threadIdx.x; p < N; p += 1024) {
+            const real dv = r_k[o + p] - mu_r;
+            acc = acc + dv * dv;
+        }
+        const real sq = sqrt(canon_sum_block<real>(acc, sred) * (real)(1.0 / (double)(N - 1)));
+        real dsum = (real)0;
+        for (int i = 0; i < I; ++i) dsum = dsum + dots[c * I + i];
+        coef = eps0 * dsum / ((real)(N - 1) * sq);
+    }
+    for (int p = threadIdx.x; p < N; p += 1024) {
+        real v = gr_k[o + p] + gacc[o + p];
+        if (stage > 0) v = v + coef * (r_k[o + p] - mu_r);
+        gr_k[o + p] = v;
+    }
+}
+
+// dL/dy from dL/dx~ with x~ = y / std(y)
+template <typename real>
+__global__ __launch_bounds__(1024) void k_adj_input(const int* chan_len, const int64_t* chan_off,
+                                                    const real* y, const real* gxt, real* gy)
+{
+    __shared__ real sred[1024];
+    const int c = blockIdx.x;
+    const int N = chan_len[c];
+    const int64_t o = chan_off[c];
+    real acc = (real)0;
+    for (int p = threadIdx.x; p < N; p += 1024) acc = acc + y[o + p];
+    const real mu = canon_sum_block<real>(acc, sred) * (real)(1.0 / (double)N);
+    acc = (real)0;
+    for (int p = threadIdx.x; p < N; p += 1024) {
+        const real dv = y[o + p] - mu;
+        acc = acc + dv * dv;
+    }
+    const real var = canon_sum_block<real>(acc, sred) * (real)(1.0 / (double)(N - 1));
+    acc = (real)0;
+    for (int p = threadIdx.x; p < N; p += 1024) acc = acc + gxt[o + p] * y[o + p];
+    const real dot = canon_sum_block<real>(acc, sred);
+    const real sx = (real)1 / sqrt(var);
+    for (int p = threadIdx.x; p < N; p += 1024)
+        gy[o + p] = sx * gxt[o + p] - sx * sx * sx * dot * (y[o + p] - mu) / (real)(N - 1);
 }
 
 // --------------------------------------------------------------------- noise
@@ -1294,6 +1741,80 @@ hipError_t launch_select_copy(const SeriesDev& sd, int max_len, const real* hA, 
     return hipGetLastError();
 }
 
+template <typename real>
+hipError_t launch_adjoint_pass(const AdjArgs<real>& a, int solve_grid, hipStream_t st)
+{
+    const int T = 2 * (a.sd.cap - 2);
+    if (T == 256)
+        hipLaunchKernelGGL((k_adj_mom<real, 256>), dim3(a.sd.ntiles), dim3(kThreads), 0, st, a);
+    else if (T == 512)
+        hipLaunchKernelGGL((k_adj_mom<real, 512>), dim3(a.sd.ntiles), dim3(kThreads), 0, st, a);
+    else if (T == 1024)
+        hipLaunchKernelGGL((k_adj_mom<real, 1024>), dim3(a.sd.ntiles), dim3(kThreads), 0, st, a);
+    else
+        return hipErrorInvalidValue;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_adj_span<real>), dim3(blocks_for(a.sd.ntiles, 256)), dim3(256), 0, st, a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL((k_adj_knot<real, Geom<real>::B, Geom<real>::H>), dim3(solve_grid),
+                       dim3(kThreads), 0, st, a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (T == 256)
+        hipLaunchKernelGGL((k_adj_scatter<real, 256>), dim3(a.sd.ntiles), dim3(kThreads), 0, st, a);
+    else if (T == 512)
+        hipLaunchKernelGGL((k_adj_scatter<real, 512>), dim3(a.sd.ntiles), dim3(kThreads), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_adj_scatter<real, 1024>), dim3(a.sd.ntiles), dim3(kThreads), 0, st, a);
+    return hipGetLastError();
+}
+
+template <typename real>
+hipError_t launch_adj_seed(const real* gimf, int M, int64_t N, int64_t stride, real* gr,
+                           hipStream_t st)
+{
+    hipLaunchKernelGGL((k_adj_seed<real>), dim3(blocks_for(N, 256)), dim3(256), 0, st, gimf, M,
+                       (int)N, stride, gr);
+    return hipGetLastError();
+}
+
+template <typename real>
+hipError_t launch_adj_broadcast(const SeriesDev& sd, const real* gr_next, const int64_t* chan_off,
+                                real* mu, int max_len, hipStream_t st)
+{
+    const unsigned gy = (unsigned)(sd.nseries < 65535 ? sd.nseries : 65535);
+    hipLaunchKernelGGL((k_adj_broadcast<real>), dim3(blocks_for(max_len, 256), gy), dim3(256), 0,
+                       st, sd, gr_next, chan_off, mu);
+    return hipGetLastError();
+}
+
+template <typename real>
+hipError_t launch_adj_stage(int stage, int nchan, int I, int max_len, const int* chan_len,
+                            const int64_t* chan_off, const SeriesDev& sd, const real* gr_next,
+                            const real* mu, const real* E, const real* r_k, real eps0, real* gr_k,
+                            real* gacc, real* dots, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_adj_accum<real>), dim3(blocks_for(max_len, 256), nchan), dim3(256), 0, st,
+                       I, chan_len, chan_off, sd, gr_next, mu, gacc);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_adj_dot<real>), dim3(sd.nseries), dim3(1024), 0, st, I, sd, chan_off,
+                       gr_next, mu, E, dots);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL((k_adj_stage<real>), dim3(nchan), dim3(1024), 0, st, stage, I, chan_len,
+                       chan_off, gacc, dots, r_k, eps0, gr_k);
+    return hipGetLastError();
+}
+
+template <typename real>
+hipError_t launch_adj_input(int nchan, const int* chan_len, const int64_t* chan_off, const real* y,
+                            const real* gxt, real* gy, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_adj_input<real>), dim3(nchan), dim3(1024), 0, st, chan_len, chan_off, y,
+                       gxt, gy);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st)
 {
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, v, n);
@@ -1333,7 +1854,22 @@ hipError_t launch_check_index(const int* d_x, int64_t N, int* err, hipStream_t s
     template hipError_t launch_select_copy<real>(const SeriesDev&, int, const real*, const real*,  \
                                                  const int*, real*, hipStream_t);
 
+#define ICM_INSTANTIATE_ADJ(real)                                                                  \
+    template hipError_t launch_adjoint_pass<real>(const AdjArgs<real>&, int, hipStream_t);         \
+    template hipError_t launch_adj_seed<real>(const real*, int, int64_t, int64_t, real*,           \
+                                              hipStream_t);                                        \
+    template hipError_t launch_adj_broadcast<real>(const SeriesDev&, const real*, const int64_t*,  \
+                                                   real*, int, hipStream_t);                       \
+    template hipError_t launch_adj_stage<real>(int, int, int, int, const int*, const int64_t*,     \
+                                               const SeriesDev&, const real*, const real*,         \
+                                               const real*, const real*, real, real*, real*,       \
+                                               real*, hipStream_t);                                \
+    template hipError_t launch_adj_input<real>(int, const int*, const int64_t*, const real*,       \
+                                               const real*, real*, hipStream_t);
+
 ICM_INSTANTIATE(float)
 ICM_INSTANTIATE(double)
+ICM_INSTANTIATE_ADJ(float)
+ICM_INSTANTIATE_ADJ(double)
 
 }  // namespace icm

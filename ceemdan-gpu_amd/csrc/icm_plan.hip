@@ -264,6 +264,7 @@ struct PlanBase {
     virtual int forward(const void* const* sig, void* const* imfs, const void* noise,
                         hipStream_t st) = 0;
     virtual int noise_modes_out(void* d_out, int64_t count, hipStream_t st) = 0;
+    virtual int backward(const void* const* gimf, void* const* gsig, hipStream_t st) = 0;
 
     template <typename T>
     int alloc(T** p, int64_t n)
@@ -376,6 +377,7 @@ int64_t plan_bytes_estimate(const icm_config* c)
     struct Tmp : PlanBase {
         int forward(const void* const*, void* const*, const void*, hipStream_t) override { return 0; }
         int noise_modes_out(void*, int64_t, hipStream_t) override { return 0; }
+        int backward(const void* const*, void* const*, hipStream_t) override { return 0; }
     } P;
     P.cfg = *c;
     layout(P);
@@ -390,6 +392,7 @@ int64_t plan_bytes_estimate(const icm_config* c)
     b += 2 * 2 * 2 * P.ntiles_max * 4 + 2 * 2 * P.nser_max * (4 + 2 * s);
     b += (int64_t)kEpochs * 4 + 2 * P.maxwin * 4;
     b += 5 * P.chan_total * s + P.chan_total * M * s;
+    b += (int64_t)(M - 1) * P.chan_total * s;     // residue history
     return b;
 }
 
@@ -410,6 +413,17 @@ struct Plan : PlanBase {
     real *y = nullptr, *xt = nullptr, *r = nullptr, *imf = nullptr;
     real *var_chan = nullptr, *var_noise = nullptr;
     double *inv_noise = nullptr, *sq_chan = nullptr;
+    real* r_hist = nullptr;     // [M-1][chan_total]: rows 1 .. M-2 = r_1 .. r_{M-2} of the last forward
+    bool have_forward = false;
+    // adjoint state (allocated by the first backward)
+    struct Rec {
+        int *px = nullptr, *cnt = nullptr, *pre = nullptr, *n = nullptr, *fixf = nullptr;
+        int* winlist = nullptr;
+        unsigned* wincount = nullptr;
+    };
+    std::vector<Rec> rec;
+    real *mu = nullptr, *mom = nullptr, *edge = nullptr, *ga = nullptr, *gr = nullptr,
+         *gacc = nullptr, *dots = nullptr, *gy = nullptr;
 
     int setup()
     {
@@ -440,6 +454,7 @@ struct Plan : PlanBase {
         if ((rc = alloc(&xt, chan_total))) return rc;
         if ((rc = alloc(&r, chan_total))) return rc;
         if ((rc = alloc(&imf, chan_total * M))) return rc;
+        if ((rc = alloc(&r_hist, chan_total * (M - 1)))) return rc;
         if ((rc = alloc(&var_chan, cfg.num_channels))) return rc;
         if ((rc = alloc(&sq_chan, cfg.num_channels))) return rc;
         if ((rc = alloc(&var_noise, (int64_t)hn.len.size()))) return rc;
@@ -518,7 +533,7 @@ struct Plan : PlanBase {
     // The stage output (h after S subtractions) ends in `out` if given
     // (series offsets), else in buffer S % 2.
     int stage_fixed(int phase, int stage, const DevSeries& d, const InitArgs<real>& ia, int S,
-                    real* out, hipStream_t st)
+                    real* out, hipStream_t st, Rec* record = nullptr)
     {
         const SeriesDev sd = d.view(kstride, sstride, cap);
         int rc;
@@ -532,7 +547,7 @@ struct Plan : PlanBase {
         if ((rc = trace_copy(phase, stage, 0, d, 0, st))) return rc;
         for (int j = 0; j < S; ++j) {
             const int b = j & 1;
-            SolveArgs<real> sa;
+            SolveArgs<real> sa{};
             sa.sd = sd;
             sa.k = knots(b);
             sa.c = kc;
@@ -540,10 +555,22 @@ struct Plan : PlanBase {
             sa.wincount = wincount + e_emit;
             sa.active = nullptr;
             sa.err = err;
+            if (record) {
+                // structure of pass j's input for the adjoint
+                const Rec& R = record[j];
+                sa.px = R.px;
+                sa.fixf = R.fixf;
+                ICM_TRY(launch_copy<float>((float*)R.cnt, (const float*)kcnt[b], 2 * (int64_t)d.ntiles, st));
+                ICM_TRY(launch_copy<float>((float*)R.pre, (const float*)kpre[b], 2 * (int64_t)d.ntiles, st));
+                ICM_TRY(launch_copy<float>((float*)R.n, (const float*)nk[b], 2 * (int64_t)d.nseries, st));
+                ICM_TRY(launch_copy<float>((float*)R.winlist, (const float*)winlist, 2 * d.maxwin, st));
+                ICM_TRY(launch_copy<float>((float*)R.wincount, (const float*)(wincount + e_emit), 1, st));
+            }
             if ((rc = prof_begin(1, 0, st))) return rc;
             ICM_TRY(launch_solve<real>(sa, solve_grid(), st));
             if ((rc = prof_end(st))) return rc;
-            SiftArgs<real> a;
+            if (record && j == S - 1) break;   // the stage output itself is not needed
+            SiftArgs<real> a{};
             a.sd = sd;
             a.segs = d.segs;
             a.h_in = b ? hB : hA;
@@ -589,7 +616,7 @@ struct Plan : PlanBase {
         if ((rc = trace_copy(phase, stage, 0, d, 0, st))) return rc;
         for (int j = 0; j < max_pass; ++j) {
             const int b = j & 1;
-            SolveArgs<real> sa;
+            SolveArgs<real> sa{};
             sa.sd = sd;
             sa.k = knots(b);
             sa.c = kc;
@@ -598,7 +625,7 @@ struct Plan : PlanBase {
             sa.active = active;
             sa.err = err;
             ICM_TRY(launch_solve<real>(sa, solve_grid(), st));
-            SiftArgs<real> a;
+            SiftArgs<real> a{};
             a.sd = sd;
             a.segs = d.segs;
             a.h_in = b ? hB : hA;
@@ -703,6 +730,9 @@ struct Plan : PlanBase {
                                           dsig.eoff, dsig.nidx, hA, hB, parity, fixed_parity, Ek,
                                           inv_noise, sq_chan, eps0, xt, r, imf, d_chan_imf_off,
                                           st));
+            // residues kept for the adjoint (r_0 = x~ is xt)
+            if (k + 1 <= M - 2)
+                ICM_TRY(launch_copy<real>(r_hist + (int64_t)(k + 1) * chan_total, r, chan_total, st));
         }
         // residue row M-1 (cudaICEEMDAN.cu:1298)
         for (int c = 0; c < C; ++c)
@@ -792,6 +822,111 @@ struct Plan : PlanBase {
         timing.signal_ms = t12;
         timing.total_ms = t03;
         prof_collect();
+        if (*h_err) return ICM_ERR_DEVICE;
+        have_forward = true;
+        return ICM_OK;
+    }
+
+    int backward_alloc()
+    {
+        if (!rec.empty()) return ICM_OK;
+        const int S = cfg.max_sifts, M = cfg.num_imfs;
+        int rc;
+        std::vector<Rec> rr(S);
+        int* px_all = nullptr;
+        if ((rc = alloc(&px_all, (int64_t)S * 2 * kstride))) return rc;
+        for (int j = 0; j < S; ++j) {
+            rr[j].px = px_all + (int64_t)j * 2 * kstride;
+            if ((rc = alloc(&rr[j].cnt, 2 * (int64_t)dsig.ntiles))) return rc;
+            if ((rc = alloc(&rr[j].pre, 2 * (int64_t)dsig.ntiles))) return rc;
+            if ((rc = alloc(&rr[j].n, 2 * (int64_t)dsig.nseries))) return rc;
+            if ((rc = alloc(&rr[j].fixf, 4 * (int64_t)dsig.nseries))) return rc;
+            if ((rc = alloc(&rr[j].winlist, 2 * dsig.maxwin))) return rc;
+            if ((rc = alloc(&rr[j].wincount, 1))) return rc;
+        }
+        if ((rc = alloc(&mu, dsig.samples))) return rc;
+        if ((rc = alloc(&mom, 8 * kstride))) return rc;
+        if ((rc = alloc(&edge, 16 * (int64_t)dsig.ntiles))) return rc;
+        if ((rc = alloc(&ga, 2 * kstride))) return rc;
+        if ((rc = alloc(&gr, (int64_t)M * chan_total))) return rc;
+        if ((rc = alloc(&gacc, chan_total))) return rc;
+        if ((rc = alloc(&dots, (int64_t)dsig.nseries))) return rc;
+        if ((rc = alloc(&gy, chan_total))) return rc;
+        rec.swap(rr);
+        return ICM_OK;
+    }
+
+    // Reverse mode over the signal phase of the last forward.
+    int backward(const void* const* gimf, void* const* gsig, hipStream_t st)
+    {
+        if (!have_forward || cfg.stop_mode != ICM_STOP_FIXED) return ICM_ERR_STATE;
+        const int C = cfg.num_channels, M = cfg.num_imfs, I = cfg.num_noise, S = cfg.max_sifts;
+        const real eps0 = (real)cfg.noise_strength;
+        int rc;
+        if ((rc = backward_alloc())) return rc;
+        *h_err = 0;
+        ICM_TRY(hipEventRecord(ev[0], st));
+        ICM_TRY(launch_fill_u32((uint32_t*)err, 0u, 1, st));
+        // dL/dr_k: IMF_0 = x~ - r_1, IMF_k = r_k - r_{k+1}, IMF_{M-1} = r_{M-1}
+        for (int c = 0; c < C; ++c) {
+            if (!gimf[c] || !gsig[c]) return ICM_ERR_INVALID_ARG;
+            ICM_TRY(launch_adj_seed<real>((const real*)gimf[c], M, lens[c], chan_total,
+                                          gr + chan_off[c], st));
+        }
+        if ((rc = reset_control(st))) return rc;
+        const SeriesDev sd = dsig.view(kstride, sstride, cap);
+        for (int k = M - 2; k >= 0; --k) {
+            const real* Ek = modes + (int64_t)k * hn.samples;
+            const real* rk = k == 0 ? xt : r_hist + (int64_t)k * chan_total;
+            InitArgs<real> ia{};
+            ia.E = Ek;
+            ia.eps0 = eps0;
+            ia.chan_off = d_chan_off;
+            if (k == 0) {
+                ia.mode = INIT_STAGE0;
+                ia.chan_sig = xt;
+                ia.scale = inv_noise;
+            } else {
+                ICM_TRY(launch_series_var<real>(rk, d_chan_off, 0, d_chan_len, C, var_chan, st));
+                ICM_TRY(launch_inv_sqrt<real>(var_chan, sq_chan, C, 1, st));
+                ia.mode = INIT_STAGEK;
+                ia.chan_sig = rk;
+                ia.scale = sq_chan;
+            }
+            if ((rc = stage_fixed(ICM_PHASE_SIGNAL, k, dsig, ia, S, nullptr, st, rec.data())))
+                return rc;
+            const real* lam = gr + (int64_t)(k + 1) * chan_total;
+            ICM_TRY(launch_adj_broadcast<real>(sd, lam, d_chan_off, mu, dsig.max_len, st));
+            for (int j = S - 1; j >= 0; --j) {
+                AdjArgs<real> a{};
+                a.sd = sd;
+                a.ps.px = rec[j].px;
+                a.ps.cnt = rec[j].cnt;
+                a.ps.pre = rec[j].pre;
+                a.ps.n = rec[j].n;
+                a.ps.fixf = rec[j].fixf;
+                a.mu = mu;
+                a.mom = mom;
+                a.edge = edge;
+                a.ga = ga;
+                a.winlist = rec[j].winlist;
+                a.wincount = rec[j].wincount;
+                a.err = err;
+                ICM_TRY(launch_adjoint_pass<real>(a, solve_grid(), st));
+            }
+            ICM_TRY(launch_adj_stage<real>(k, C, I, max_chan_len, d_chan_len, d_chan_off, sd, lam,
+                                           mu, Ek, rk, eps0, gr + (int64_t)k * chan_total, gacc,
+                                           dots, st));
+        }
+        ICM_TRY(launch_adj_input<real>(C, d_chan_len, d_chan_off, y, gr, gy, st));
+        for (int c = 0; c < C; ++c)
+            ICM_TRY(launch_copy<real>((real*)gsig[c], gy + chan_off[c], lens[c], st));
+        ICM_TRY(hipEventRecord(ev[3], st));
+        ICM_TRY(hipMemcpyAsync(h_err, err, sizeof(int), hipMemcpyDeviceToHost, st));
+        ICM_TRY(hipStreamSynchronize(st));
+        float t03 = 0;
+        (void)hipEventElapsedTime(&t03, ev[0], ev[3]);
+        timing.backward_ms = t03;
         if (*h_err) return ICM_ERR_DEVICE;
         return ICM_OK;
     }
@@ -1063,12 +1198,11 @@ int icm_noise(int dtype, uint64_t seed, int64_t I, int64_t N, void* d_out, void*
     return ICM_OK;
 }
 
-int icm_backward(icm_plan plan, const void* const* d_signals, const void* const* d_grad_imfs,
-                 void* const* d_grad_signals, const void* d_noise, void* stream)
+int icm_backward(icm_plan plan, const void* const* d_grad_imfs, void* const* d_grad_signals,
+                 void* stream)
 {
-    (void)d_signals; (void)d_grad_imfs; (void)d_grad_signals; (void)d_noise; (void)stream;
-    if (!plan) return ICM_ERR_INVALID_ARG;
-    return ICM_ERR_STATE;
+    if (!plan || !d_grad_imfs || !d_grad_signals) return ICM_ERR_INVALID_ARG;
+    return plan->p->backward(d_grad_imfs, d_grad_signals, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -78,6 +78,7 @@ typedef struct icm_timing {
     double noise_ms;            /* noise generation + noise-mode stages */
     double signal_ms;           /* CEEMDAN stages (reference timer, cudaICEEMDAN.cu:1002-1297) */
     double total_ms;            /* whole forward, input copy to output copy */
+    double backward_ms;         /* last icm_backward, gradient in to gradient out */
 } icm_timing;
 
 const char* icm_status_string(int status);
@@ -138,11 +139,16 @@ int icm_noise(int dtype, uint64_t seed, int64_t I, int64_t N, void* d_out, void*
 int icm_sift_pass(int dtype, const void* d_h, int64_t N, int64_t I, void* d_h_new, int32_t* d_nk,
                   void* stream);
 
-/* Adjoint of the FIXED-mode forward (no reference counterpart): given the
- * forward's inputs and dL/dIMF, writes dL/dx for every channel.  Extrema
- * sets are those of the forward (recomputed, not stored). */
-int icm_backward(icm_plan plan, const void* const* d_signals, const void* const* d_grad_imfs,
-                 void* const* d_grad_signals, const void* d_noise, void* stream);
+/* Adjoint of the last icm_forward of this plan (no reference counterpart):
+ * given dL/dIMF (d_grad_imfs[c]: [M][N_c], the layout of d_imfs), writes
+ * dL/dx (d_grad_signals[c]: [N_c]) for every channel.  FIXED stop rule only
+ * (ICM_ERR_STATE otherwise, or when no forward ran).  The extrema sets and
+ * endpoint-fix branches are those of the forward: each stage's sifting is
+ * re-run from the saved residue to record them, then the passes are
+ * reversed.  The noise is treated as a constant.  Device memory for the
+ * records is allocated on the first call and kept by the plan. */
+int icm_backward(icm_plan plan, const void* const* d_grad_imfs, void* const* d_grad_signals,
+                 void* stream);
 
 #ifdef __cplusplus
 }

@@ -400,9 +400,24 @@ int ocm_sift_mode(const double* x, int N, int stop_mode, int S_or_cap, int B, in
 
 /* ------------------------------------------------------------ full ICEEMDAN */
 
+static int iceemdan_impl(const double* y, int N, int I, int M, int stop_mode, int S_or_cap,
+                         int noise_cap, double eps0, const double* noise, int B, int H,
+                         int nthreads, double* imfs, int* trace, int trace_passes, int* stops,
+                         double* noise_modes_out, double* r_hist);
+
 int ocm_iceemdan(const double* y, int N, int I, int M, int stop_mode, int S_or_cap,
                  int noise_cap, double eps0, const double* noise, int B, int H, int nthreads,
                  double* imfs, int* trace, int trace_passes, int* stops, double* noise_modes_out)
+{
+    return iceemdan_impl(y, N, I, M, stop_mode, S_or_cap, noise_cap, eps0, noise, B, H, nthreads,
+                         imfs, trace, trace_passes, stops, noise_modes_out, NULL);
+}
+
+/* r_hist (optional): M x N, row 0 = x~, row k = r_k (stage k input) */
+static int iceemdan_impl(const double* y, int N, int I, int M, int stop_mode, int S_or_cap,
+                         int noise_cap, double eps0, const double* noise, int B, int H,
+                         int nthreads, double* imfs, int* trace, int trace_passes, int* stops,
+                         double* noise_modes_out, double* r_hist)
 {
     if (N < 4 || I < 1 || M < 2 || S_or_cap < 1 || noise_cap < 1 || B < 3 ||
         H < 1 || 2L * H >= (long)NLANES * B)
@@ -453,6 +468,7 @@ int ocm_iceemdan(const double* y, int N, int I, int M, int stop_mode, int S_or_c
     double var_x = ocm_var_canon(y, N);
     double inv_x = 1.0 / sqrt(var_x);
     for (int p = 0; p < N; ++p) xt[p] = y[p] * inv_x;
+    if (r_hist) memcpy(r_hist, xt, sizeof(double) * N);
     for (int i = 0; i < I; ++i) inv_i[i] = 1.0 / sqrt(ocm_var_canon(modes + (long)i * N, N));
 
     for (int k = 0; k < M - 1; ++k) {
@@ -503,6 +519,7 @@ int ocm_iceemdan(const double* y, int N, int I, int M, int stop_mode, int S_or_c
                 r[p] = tem / (double)I;
             }
         }
+        if (r_hist) memcpy(r_hist + (long)(k + 1) * N, r, sizeof(double) * N);
     }
     memcpy(imfs + (long)(M - 1) * N, r, sizeof(double) * N);
     if (noise_modes_out) memcpy(noise_modes_out, modes, sizeof(double) * (size_t)(M - 1) * IN);
@@ -562,4 +579,229 @@ void ocm_noise(uint64_t seed, long I, long N, double* out)
             for (int j = 0; j < 4 && 4 * q + j < N; ++j) out[i * N + 4 * q + j] = z[j];
         }
     }
+}
+
+/* ================================================================ adjoint
+ * With the extrema sets of every pass frozen, one sift pass is linear:
+ * h_{j+1} = h_j - (U(h_j) + L(h_j)) / 2, and an envelope is the chain
+ *   knot gather -> endpoint fix -> rhs f(a) -> c = A^{-1} f
+ *   -> interval coefficients (a, b, c, d) -> evaluation at every sample.
+ * The adjoint runs the chain backwards (scatter-add for the gather, A^T
+ * solve, transposed evaluation as per-interval moments of the gradient).
+ * The A^T solve here is the global Thomas algorithm (exact). */
+
+/* Thomas solve of A^T x = r for the natural-spline matrix A of the knots. */
+static void solve_transposed(int n, const int* pos, const double* r, double* x)
+{
+    double* cp = (double*)malloc(sizeof(double) * n * 2);
+    double* dp = cp + n;
+    for (int g = 0; g < n; ++g) {
+        /* row g of A^T: sub = A[g-1][g], diag = A[g][g], sup = A[g+1][g] */
+        double sub = 0.0, sup = 0.0, diag = 1.0;
+        if (g >= 1 && g - 1 >= 1 && g - 1 <= n - 2) sub = (double)(pos[g] - pos[g - 1]);
+        if (g >= 1 && g <= n - 2) diag = (double)(2 * ((pos[g] - pos[g - 1]) + (pos[g + 1] - pos[g])));
+        if (g + 1 <= n - 2 && g + 1 >= 1) sup = (double)(pos[g + 1] - pos[g]);
+        if (g == 0) {
+            cp[g] = sup / diag;
+            dp[g] = r[g] / diag;
+        } else {
+            double den = diag - sub * cp[g - 1];
+            cp[g] = sup / den;
+            dp[g] = (r[g] - sub * dp[g - 1]) / den;
+        }
+    }
+    x[n - 1] = dp[n - 1];
+    for (int g = n - 2; g >= 0; --g) x[g] = dp[g] - cp[g] * x[g + 1];
+    free(cp);
+}
+
+void ocm_envelope_adjoint(int N, int n, const int* pos, int fixed_lo, int fixed_hi,
+                          const double* g, double* out)
+{
+    double* mo = (double*)calloc((size_t)4 * n + 3 * (size_t)n, sizeof(double));
+    double* M0 = mo; double* M1 = mo + n; double* M2 = mo + 2 * n; double* M3 = mo + 3 * n;
+    double* gc = mo + 4 * n; double* gf = gc + n; double* ga = gf + n;
+    /* per-interval moments of g (transposed evaluation) */
+    int k = 0;
+    for (int p = 0; p < N; ++p) {
+        while (k + 1 < n - 1 && p > pos[k + 1]) ++k;
+        double t = (double)(p - pos[k]);
+        M0[k] += g[p];
+        M1[k] += g[p] * t;
+        M2[k] += g[p] * t * t;
+        M3[k] += g[p] * t * t * t;
+    }
+    /* coefficient formulas (cudaICEEMDAN.cu:394-398) transposed */
+    for (int q = 0; q <= n - 2; ++q) {
+        double hq = (double)(pos[q + 1] - pos[q]);
+        ga[q] += M0[q];
+        gc[q] += M2[q];
+        /* b_q = (a_{q+1}-a_q)/h - h (2 c_q + c_{q+1}) / 3 */
+        ga[q + 1] += M1[q] / hq;
+        ga[q] -= M1[q] / hq;
+        gc[q] -= 2.0 * hq / 3.0 * M1[q];
+        gc[q + 1] -= hq / 3.0 * M1[q];
+        /* d_q = (c_{q+1} - c_q) / (3 h) */
+        gc[q + 1] += M3[q] / (3.0 * hq);
+        gc[q] -= M3[q] / (3.0 * hq);
+    }
+    /* c = A^{-1} f  ->  gf = A^{-T} gc */
+    solve_transposed(n, pos, gc, gf);
+    /* f_g = 3 ((a_{g+1}-a_g)/h_g - (a_g-a_{g-1})/h_{g-1}), interior g */
+    for (int q = 1; q <= n - 2; ++q) {
+        double hl = (double)(pos[q] - pos[q - 1]);
+        double hr = (double)(pos[q + 1] - pos[q]);
+        ga[q + 1] += 3.0 * gf[q] / hr;
+        ga[q] -= 3.0 * gf[q] / hr + 3.0 * gf[q] / hl;
+        ga[q - 1] += 3.0 * gf[q] / hl;
+    }
+    /* endpoint fix (cudaICEEMDAN.cu:209-291) with its forward branch */
+    if (n >= 6 && fixed_lo) {
+        double rr = (double)(pos[1] - pos[0]) / (double)(pos[2] - pos[1]);
+        ga[1] += ga[0] * (1.0 + rr);
+        ga[2] -= ga[0] * rr;
+        ga[0] = 0.0;
+    }
+    if (n >= 6 && fixed_hi) {
+        double rr = (double)(pos[n - 1] - pos[n - 2]) / (double)(pos[n - 2] - pos[n - 3]);
+        ga[n - 2] += ga[n - 1] * (1.0 + rr);
+        ga[n - 3] -= ga[n - 1] * rr;
+        ga[n - 1] = 0.0;
+    }
+    for (int q = 0; q < n; ++q) out[pos[q]] += ga[q];
+    free(mo);
+}
+
+/* forward of one mode extraction recording the frozen structure of each pass */
+typedef struct {
+    int nmax, nmin, fix[4];
+    int* pmax;
+    int* pmin;
+} pass_rec;
+
+static void sift_record(const double* x, int N, int S, int B, int H, double* mode, pass_rec* rec)
+{
+    double* h = (double*)malloc(sizeof(double) * N * 2);
+    double* hn = h + N;
+    double* val = (double*)malloc(sizeof(double) * N * 4);
+    double* c = val + N; double* U = c + N; double* L = U + N;
+    int* pos = (int*)malloc(sizeof(int) * N);
+    memcpy(h, x, sizeof(double) * N);
+    for (int j = 0; j < S; ++j) {
+        for (int ty = 0; ty < 2; ++ty) {
+            int n = ocm_extrema(h, N, ty == 0, pos, val);
+            double v0 = val[0], v1 = val[n - 1];
+            ocm_boundary_fix(n, pos, val, ty == 0);
+            rec[j].fix[2 * ty] = (n >= 6) && (val[0] != v0);
+            rec[j].fix[2 * ty + 1] = (n >= 6) && (val[n - 1] != v1);
+            int* keep = (int*)malloc(sizeof(int) * n);
+            memcpy(keep, pos, sizeof(int) * n);
+            if (ty == 0) { rec[j].nmax = n; rec[j].pmax = keep; }
+            else { rec[j].nmin = n; rec[j].pmin = keep; }
+            ocm_spline_solve(n, pos, val, B, H, c);
+            ocm_envelope(N, n, pos, val, c, ty == 0 ? U : L);
+        }
+        int use_mean = rec[j].nmax > 3 && rec[j].nmin > 3;
+        for (int p = 0; p < N; ++p) hn[p] = h[p] - (use_mean ? (U[p] + L[p]) / 2.0 : 0.0);
+        memcpy(h, hn, sizeof(double) * N);
+    }
+    memcpy(mode, h, sizeof(double) * N);
+    free(h); free(val); free(pos);
+}
+
+/* mu <- Q^T mu for the recorded passes (Q = P_{S-1} ... P_0) */
+static void sift_adjoint(int N, int S, const pass_rec* rec, double* mu, double* tmp)
+{
+    for (int j = S - 1; j >= 0; --j) {
+        if (!(rec[j].nmax > 3 && rec[j].nmin > 3)) continue;
+        for (int p = 0; p < N; ++p) tmp[p] = 0.0;
+        ocm_envelope_adjoint(N, rec[j].nmax, rec[j].pmax, rec[j].fix[0], rec[j].fix[1], mu, tmp);
+        ocm_envelope_adjoint(N, rec[j].nmin, rec[j].pmin, rec[j].fix[2], rec[j].fix[3], mu, tmp);
+        for (int p = 0; p < N; ++p) mu[p] = mu[p] - 0.5 * tmp[p];
+    }
+}
+
+int ocm_iceemdan_grad(const double* y, int N, int I, int M, int S, double eps0,
+                      const double* noise, int B, int H, int nthreads, const double* grad_imfs,
+                      double* grad_y, double* imfs)
+{
+    if (N < 4 || I < 1 || M < 2 || S < 1) return -1;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+    long IN = (long)I * N;
+    double* nm = (double*)malloc(sizeof(double) * (size_t)(M - 1) * IN);
+    double* rs = (double*)malloc(sizeof(double) * (size_t)M * N);    /* r_0 = x~, r_1 .. r_{M-1} */
+    double* gr = (double*)calloc((size_t)M * N, sizeof(double));     /* dL/dr_k, gr[0] = dL/dx~ */
+    int rc = iceemdan_impl(y, N, I, M, OCM_STOP_FIXED, S, S, eps0, noise, B, H, nthreads, imfs,
+                           NULL, 0, NULL, nm, rs);
+    if (rc) return rc;
+    double var_x = ocm_var_canon(y, N);
+    double s_x = 1.0 / sqrt(var_x);
+    /* IMF_0 = x~ - r_1, IMF_k = r_k - r_{k+1}, IMF_{M-1} = r_{M-1} */
+    for (int p = 0; p < N; ++p) {
+        gr[p] += grad_imfs[p];
+        gr[N + p] -= grad_imfs[p];
+    }
+    for (int k = 1; k <= M - 2; ++k)
+        for (int p = 0; p < N; ++p) {
+            gr[(long)k * N + p] += grad_imfs[(long)k * N + p];
+            gr[(long)(k + 1) * N + p] -= grad_imfs[(long)k * N + p];
+        }
+    for (int p = 0; p < N; ++p) gr[(long)(M - 1) * N + p] += grad_imfs[(long)(M - 1) * N + p];
+
+    double* inv_i = (double*)malloc(sizeof(double) * I);
+    for (int i = 0; i < I; ++i) inv_i[i] = 1.0 / sqrt(ocm_var_canon(nm + (long)i * N, N));
+    double* gsum = (double*)malloc(sizeof(double) * (size_t)I * N);
+    double* dots = (double*)malloc(sizeof(double) * I);
+    for (int k = M - 2; k >= 0; --k) {
+        const double* rk = rs + (long)k * N;
+        const double* lam = gr + (long)(k + 1) * N;
+        double sq = k > 0 ? sqrt(ocm_var_canon(rk, N)) : 0.0;
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int i = 0; i < I; ++i) {
+            const double* E = nm + (long)k * IN + (long)i * N;
+            double* xi = (double*)malloc(sizeof(double) * N * 4);
+            double* mode = xi + N; double* mu = mode + N; double* tmp = mu + N;
+            for (int p = 0; p < N; ++p)
+                xi[p] = k == 0 ? rk[p] + eps0 * E[p] * inv_i[i] : rk[p] + eps0 * E[p] * sq;
+            pass_rec* rec = (pass_rec*)calloc(S, sizeof(pass_rec));
+            sift_record(xi, N, S, B, H, mode, rec);
+            memcpy(mu, lam, sizeof(double) * N);
+            sift_adjoint(N, S, rec, mu, tmp);
+            /* d r_{k+1} / d x_i = (I - Q_i) / I */
+            double* gx = gsum + (long)i * N;
+            double dot = 0.0;
+            for (int p = 0; p < N; ++p) {
+                gx[p] = (lam[p] - mu[p]) / (double)I;
+                dot += gx[p] * E[p];
+            }
+            dots[i] = dot;
+            for (int j = 0; j < S; ++j) { free(rec[j].pmax); free(rec[j].pmin); }
+            free(rec);
+            free(xi);
+        }
+        double* g = gr + (long)k * N;
+        double dsum = 0.0;
+        for (int i = 0; i < I; ++i) dsum += dots[i];
+        for (int p = 0; p < N; ++p) {
+            double acc = 0.0;
+            for (int i = 0; i < I; ++i) acc += gsum[(long)i * N + p];
+            g[p] += acc;
+        }
+        if (k > 0) {
+            /* x_i = r_k + eps0 sigma(r_k) E_i: chain through sigma(r_k) */
+            double mu_r = ocm_sum_canon(rk, N) * (1.0 / (double)N);
+            double coef = eps0 * dsum / ((double)(N - 1) * sq);
+            for (int p = 0; p < N; ++p) g[p] += coef * (rk[p] - mu_r);
+        }
+    }
+    /* x~ = y / std(y) */
+    double mu_y = ocm_sum_canon(y, N) * (1.0 / (double)N);
+    double dot = 0.0;
+    for (int p = 0; p < N; ++p) dot += gr[p] * y[p];
+    for (int p = 0; p < N; ++p)
+        grad_y[p] = s_x * gr[p] - s_x * s_x * s_x * dot * (y[p] - mu_y) / (double)(N - 1);
+    free(nm); free(rs); free(gr); free(inv_i); free(gsum); free(dots);
+    return 0;
 }

@@ -69,6 +69,20 @@ int ocm_iceemdan(const double* y, int N, int I, int M, int stop_mode, int S_or_c
                  int noise_cap, double eps0, const double* noise, int B, int H, int nthreads,
                  double* imfs, int* trace, int trace_passes, int* stops, double* noise_modes_out);
 
+/* Adjoint of one envelope operator with frozen knots: given dL/d(envelope)
+ * g[N], accumulates into out[N] (+=) the gradient with respect to the
+ * sifted samples h.  pos/n: knot positions; fixed_lo/fixed_hi: whether the
+ * endpoint fix replaced the value of knot 0 / knot n-1 in the forward. */
+void ocm_envelope_adjoint(int N, int n, const int* pos, int fixed_lo, int fixed_hi,
+                          const double* g, double* out);
+
+/* Gradient of L(IMFs) with respect to the input channel y, FIXED stop rule,
+ * extrema sets and endpoint-fix branches frozen at the forward's values.
+ * grad_imfs: dL/dIMF, M x N.  grad_y: N (output).  Also returns the IMFs. */
+int ocm_iceemdan_grad(const double* y, int N, int I, int M, int S, double eps0,
+                      const double* noise, int B, int H, int nthreads, const double* grad_imfs,
+                      double* grad_y, double* imfs);
+
 /* Philox4x64-10 block (Salmon et al. SC'11) and the Gaussian stream built on it. */
 void ocm_philox4x64(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]);
 void ocm_noise(uint64_t seed, long I, long N, double* out);

@@ -55,6 +55,10 @@ def _setup(L):
     L.ocm_iceemdan.argtypes = [dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                C.c_double, dp, C.c_int, C.c_int, C.c_int, dp, ip, C.c_int, ip, dp]
     L.ocm_iceemdan.restype = C.c_int
+    L.ocm_envelope_adjoint.argtypes = [C.c_int, C.c_int, ip, C.c_int, C.c_int, dp, dp]
+    L.ocm_iceemdan_grad.argtypes = [dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, dp,
+                                    C.c_int, C.c_int, C.c_int, dp, dp, dp]
+    L.ocm_iceemdan_grad.restype = C.c_int
     u64p = C.POINTER(C.c_uint64)
     L.ocm_philox4x64.argtypes = [u64p, u64p, u64p]
     L.ocm_noise.argtypes = [C.c_uint64, C.c_long, C.c_long, dp]
@@ -199,3 +203,27 @@ def noise(seed, I, N):
     out = np.empty((I, N), np.float64)
     lib().ocm_noise(C.c_uint64(seed), I, N, _d(out))
     return out
+
+
+def envelope_adjoint(N, pos, g, fixed_lo=0, fixed_hi=0):
+    pos = np.ascontiguousarray(pos, np.int32)
+    out = np.zeros(N, np.float64)
+    lib().ocm_envelope_adjoint(N, pos.size, _i(pos), int(fixed_lo), int(fixed_hi), _d(_f64(g)),
+                               _d(out))
+    return out
+
+
+def iceemdan_grad(y, noise, M, grad_imfs, S=10, eps0=0.2, geom=GEOM_F64, nthreads=0):
+    """dL/dy for L with dL/dIMF = grad_imfs (M x N), FIXED rule, frozen extrema.
+    Returns (grad_y, imfs)."""
+    y = _f64(y)
+    noise = _f64(noise)
+    I, N = noise.shape
+    g = _f64(grad_imfs).reshape(M, N)
+    gy = np.empty(N, np.float64)
+    imfs = np.empty((M, N), np.float64)
+    rc = lib().ocm_iceemdan_grad(_d(y), N, I, M, S, eps0, _d(noise), geom[0], geom[1], nthreads,
+                                 _d(g), _d(gy), _d(imfs))
+    if rc != 0:
+        raise ValueError(f"oracle iceemdan_grad failed: {rc}")
+    return gy, imfs

@@ -313,6 +313,91 @@ def test_iceemdan_rilling_stops(orc):
     assert np.max(np.abs(imfs.sum(0) - xt)) < 1e-12
 
 
+# ---------------------------------------------------------------- adjoint
+
+def _frozen_envelope(orc, N, pos, flo, fhi):
+    """The envelope as a linear map of the samples at the knot positions,
+    with the endpoint-fix branch held fixed; built from the oracle's
+    forward pieces (scipy pins those), independent of the adjoint code."""
+    def env(h):
+        v = h[pos].copy()
+        if flo:
+            r = (pos[1] - pos[0]) / (pos[2] - pos[1])
+            v[0] = v[1] * (1 + r) - v[2] * r
+        if fhi:
+            r = (pos[-1] - pos[-2]) / (pos[-2] - pos[-3])
+            v[-1] = v[-2] * (1 + r) - v[-3] * r
+        return orc.envelope(N, pos, v, orc.spline_solve_thomas(pos, v))
+    return env
+
+
+@pytest.mark.parametrize("seed,is_max", [(0, True), (1, False), (5, True), (8, False)])
+def test_envelope_adjoint_is_jacobian_transpose(orc, seed, is_max):
+    """ocm_envelope_adjoint(g) == J^T g with J the dense Jacobian of the
+    frozen-knot envelope (assembled column by column from the forward)."""
+    rng = np.random.default_rng(seed)
+    N = 240
+    h = rng.standard_normal(N)
+    pos, val = orc.extrema(h, is_max)
+    fixed = orc.boundary_fix(pos, val, is_max)
+    flo, fhi = int(fixed[0] != val[0]), int(fixed[-1] != val[-1])
+    env = _frozen_envelope(orc, N, pos, flo, fhi)
+    J = np.stack([env(np.eye(N)[q]) for q in range(N)], axis=1)
+    g = rng.standard_normal(N)
+    adj = orc.envelope_adjoint(N, pos, g, flo, fhi)
+    ref = J.T @ g
+    assert np.max(np.abs(adj - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_envelope_adjoint_both_fix_branches(orc):
+    """A signal whose ends both get extrapolated (rising ramps into each
+    end): both endpoint-fix adjoint branches are exercised."""
+    N = 200
+    n = np.arange(N)
+    h = np.sin(2 * np.pi * n / 23.0) + 0.02 * (n - N / 2) ** 2 / N
+    found = False
+    for is_max in (True, False):
+        pos, val = orc.extrema(h, is_max)
+        fixed = orc.boundary_fix(pos, val, is_max)
+        flo, fhi = int(fixed[0] != val[0]), int(fixed[-1] != val[-1])
+        if not (flo and fhi):
+            continue
+        found = True
+        env = _frozen_envelope(orc, N, pos, flo, fhi)
+        J = np.stack([env(np.eye(N)[q]) for q in range(N)], axis=1)
+        g = np.cos(n / 7.0)
+        adj = orc.envelope_adjoint(N, pos, g, flo, fhi)
+        assert np.max(np.abs(adj - J.T @ g)) <= 1e-12 * np.max(np.abs(J.T @ g))
+    assert found
+
+
+@pytest.mark.parametrize("eps0", [0.2, 0.0])
+def test_iceemdan_grad_matches_finite_differences(orc, eps0):
+    """dL/dy of L = sum_k w_k |IMF_k|^2 against central differences
+    (h = 1e-5 keeps every extrema set unchanged; the error of the
+    difference quotient is then O(h^2))."""
+    rng = np.random.default_rng(11)
+    N, I, M, S = 1024, 6, 5, 10
+    t = np.arange(N) / 1000.0
+    x = np.sin(2 * np.pi * 5 * t) + 0.5 * np.sin(2 * np.pi * 40 * t) + 0.1 * rng.standard_normal(N)
+    w = orc.noise(1, I, N)
+    wk = rng.uniform(0, 1, M)
+
+    def loss(xx):
+        im = orc.iceemdan(xx, w, M, S_or_cap=S, eps0=eps0)["imfs"]
+        return float(np.sum(wk * np.sum(im ** 2, 1))), im
+
+    _, im0 = loss(x)
+    gy, im = orc.iceemdan_grad(x, w, M, 2 * wk[:, None] * im0, S=S, eps0=eps0)
+    assert np.array_equal(im, im0)
+    for _ in range(3):
+        v = rng.standard_normal(N)
+        v /= np.linalg.norm(v)
+        hh = 1e-5
+        fd = (loss(x + hh * v)[0] - loss(x - hh * v)[0]) / (2 * hh)
+        assert abs(fd - gy @ v) <= 1e-6 * abs(fd), (fd, gy @ v)
+
+
 def test_golden_fixtures(orc):
     """The oracle reproduces its committed outputs bit for bit."""
     path = os.path.join(GOLD, "oracle_fixtures.npz")
