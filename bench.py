@@ -37,6 +37,9 @@ WORKLOADS = {
                desc="three-tone N=4096 fs=1 kHz + N(0,0.1^2) (seed 0), I=50, M=10, 10 sifts, float64"),
     "c3": dict(I=200, M=14, S=10, dtype="f32",
                desc="64 EEG-like channels, N ~ U[8192, 262144] (seed 3), I=200, M=14, 10 sifts, float32"),
+    "c4": dict(I=100, M=12, S=10, dtype="f64",
+               desc="32 EEG-like channels x N=16384 (seed 4), I=100, M=12, 10 sifts, float64; "
+                    "forward + adjoint of L = sum_k w_k |IMF_k|^2, w_k ~ U(0,1) (seed 5)"),
     "c5": dict(I=500, M=16, S=10, dtype="f32",
                desc="128 EEG-like channels x N=244032 (+50 Hz 5 uV, seed 6), I=500, M=16, 10 sifts, float32"),
 }
@@ -50,6 +53,8 @@ def signals_for(name, sig):
     if name == "c3":
         lens = sig.batch_lengths(64, 8192, 262144, seed=3)
         return [sig.eeg_like(n, seed=1000 + c) for c, n in enumerate(lens)]
+    if name == "c4":
+        return [sig.eeg_like(16384, seed=4000 + c) for c in range(32)]
     if name == "c5":
         return [sig.eeg_like(244032, seed=6000 + c, line_uv=5.0) for c in range(128)]
     raise ValueError(name)
@@ -79,6 +84,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-sample", type=int, default=24, help="realizations timed on the CPU")
     ap.add_argument("--graph", action="store_true", help="time the hipGraph replay (no kernel events)")
+    ap.add_argument("--no-backward", action="store_true", help="skip the adjoint timing")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -158,6 +164,20 @@ def main():
                             "kernel": "k_sift", "avg_launch_ms": avg_ms,
                             "alg_bytes_per_launch": bytes_per_launch,
                             "launches": int(sift_n)}
+    if not args.no_backward:
+        # adjoint of L = sum_k w_k |IMF_k|^2 of the last forward, w_k ~ U(0,1) (seed 5)
+        wk = torch.tensor(np.random.default_rng(5).uniform(0, 1, M), dtype=dtype, device=dev)
+        grads = [2 * wk[:, None] * o for o in out]
+        gx = plan.backward(grads)
+        torch.cuda.synchronize()
+        tb = time.perf_counter()
+        for _ in range(args.steps):
+            gx = plan.backward(grads)
+        torch.cuda.synchronize()
+        bwd_ms = 1e3 * (time.perf_counter() - tb) / args.steps
+        line["backward"] = {"ms_per_call": bwd_ms, "backward_forward_ratio": bwd_ms / ms_per_step,
+                            "loss": "sum_k w_k |IMF_k|^2, w_k ~ U(0,1) seed 5",
+                            "grad_finite": all(bool(torch.isfinite(g).all()) for g in gx)}
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         sample_I = max(1, min(args.cpu_sample, I))

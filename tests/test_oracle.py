@@ -403,7 +403,12 @@ def test_golden_fixtures(orc):
     path = os.path.join(GOLD, "oracle_fixtures.npz")
     if not os.path.exists(path):
         pytest.skip("fixtures not generated")
-    from tests.golden.make_fixtures import CASES, run_case
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_fixtures",
+                                                  os.path.join(GOLD, "make_fixtures.py"))
+    mf = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mf)
+    CASES, run_case = mf.CASES, mf.run_case
     data = np.load(path, allow_pickle=False)
     for name in CASES:
         got = run_case(orc, name)
