@@ -1051,15 +1051,29 @@ __global__ __launch_bounds__(1024) void k_series_var(const real* base, const int
 
 // 1. moments sum_p mu_p t^m, m = 0..3, of the runs of samples of each knot
 // interval inside a tile (interval k = (x_k, x_{k+1}], sample 0 in interval 0).
-// Runs bounded by two knots of the tile are complete intervals; the first
-// and last run of a tile go to the edge array for k_adj_span.
+// A sample's run is the number of the tile's knots strictly before it (from
+// a bitmap of the knots); runs are summed by a segmented scan (wave shuffles,
+// then a carry across waves and 256-sample chunks).  Runs bounded by two
+// knots of the tile are complete intervals; the first and last run of a tile
+// go to the edge array for k_adj_span.
 template <typename real, int T>
 __global__ __launch_bounds__(kThreads) void k_adj_mom(AdjArgs<real> a)
 {
     constexpr int KC = T / 2 + 2;
-    __shared__ real sterm[T][4];
-    __shared__ int skp[KC];
+    constexpr int NWV = kThreads / kWave;
+    constexpr int NWORD = T / 64;
+    constexpr int NOKEY = 1 << 28;
+    __shared__ int skp[2][KC];
+    __shared__ unsigned long long sbits[2][NWORD];
+    __shared__ int swpre[2][NWORD];
+    __shared__ int s_tail_key[2][NWV], s_head_key[2][NWV];
+    __shared__ real s_tail[2][NWV][4];
+    __shared__ int s_in_key[2][NWV];
+    __shared__ real s_in[2][NWV][4];
+    __shared__ int s_cc_key[2];
+    __shared__ real s_cc[2][4];
     const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
     const SeriesDev& sd = a.sd;
     const int gt = blockIdx.x;
     const int s = sd.tile_series[gt];
@@ -1070,53 +1084,130 @@ __global__ __launch_bounds__(kThreads) void k_adj_mom(AdjArgs<real> a)
     const int n0 = a.ps.n[2 * s], n1 = a.ps.n[2 * s + 1];
     if (!(n0 > 3 && n1 > 3)) return;   // the pass did not subtract: identity
     const real* mu = a.mu + sd.off[s];
+    int cnt[2], pre[2], left[2];
+    int64_t kb[2];
+#pragma unroll
     for (int ty = 0; ty < 2; ++ty) {
-        const int64_t kb = sd.kbase[s] + (ty ? sd.kstride : 0);
-        const int cnt = a.ps.cnt[2 * gt + ty];
-        const int pre = a.ps.pre[2 * gt + ty];
-        for (int i = tid; i < cnt; i += kThreads) skp[i] = a.ps.px[kb + pre + i];
-        const int left = pre > 0 ? a.ps.px[kb + pre - 1] : 0;
-        __syncthreads();
-        // run of each sample = knots of the tile strictly before it
-        for (int u = tid; u < nt; u += kThreads) {
-            int lo = 0, hi = cnt;           // first knot >= sample
-            const int p = t0 + u;
-            while (lo < hi) {
-                const int m = (lo + hi) >> 1;
-                if (skp[m] < p) lo = m + 1;
-                else hi = m;
+        kb[ty] = sd.kbase[s] + (ty ? sd.kstride : 0);
+        cnt[ty] = a.ps.cnt[2 * gt + ty];
+        pre[ty] = a.ps.pre[2 * gt + ty];
+        left[ty] = pre[ty] > 0 ? a.ps.px[kb[ty] + pre[ty] - 1] : 0;
+    }
+    for (int i = tid; i < 2 * NWORD; i += kThreads) sbits[i / NWORD][i % NWORD] = 0ull;
+    if (tid < 2) {
+        s_cc_key[tid] = NOKEY;
+        for (int m = 0; m < 4; ++m) s_cc[tid][m] = (real)0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ty = 0; ty < 2; ++ty)
+        for (int i = tid; i < cnt[ty]; i += kThreads) {
+            const int x = a.ps.px[kb[ty] + pre[ty] + i];
+            skp[ty][i] = x;
+            const int u = x - t0;
+            atomicOr(&sbits[ty][u >> 6], 1ull << (u & 63));
+        }
+    __syncthreads();
+    if (tid < 2) {
+        int run = 0;
+        for (int w = 0; w < NWORD; ++w) {
+            swpre[tid][w] = run;
+            run += __popcll(sbits[tid][w]);
+        }
+    }
+    // empty edge runs (a knot on the tile's first / last sample)
+    if (tid < 2) {
+        const int ty = tid;
+        real* e = a.edge + 4 * (4 * (int64_t)gt + 2 * ty);
+        if (cnt[ty] > 0 && (t0 == 0 || skp[ty][0] == t0))
+            for (int m = 0; m < 4; ++m) e[m] = (real)0;
+        if (cnt[ty] > 0 && skp[ty][cnt[ty] - 1] == t0 + nt - 1)
+            for (int m = 0; m < 4; ++m) e[4 + m] = (real)0;
+    }
+    __syncthreads();
+
+    auto key_of = [&](int ty, int u) -> int {
+        // knots of the tile strictly before sample u; sample 0 joins run 1
+        if (t0 == 0 && u == 0) return 1;
+        const unsigned long long below = (u & 63) ? (sbits[ty][u >> 6] << (64 - (u & 63))) : 0ull;
+        return swpre[ty][u >> 6] + __popcll(below);
+    };
+
+    for (int c0 = 0; c0 < T; c0 += kThreads) {
+        const int u = c0 + tid;
+        const bool valid = u < nt;
+        int key[2];
+        real v[2][4];
+#pragma unroll
+        for (int ty = 0; ty < 2; ++ty) {
+            key[ty] = valid ? key_of(ty, u) : NOKEY + 1 + ty;
+            real g = (real)0, tt = (real)0;
+            if (valid) {
+                const int r = key[ty];
+                const int lp = (t0 == 0 && u == 0) ? 0 : (r >= 1 ? skp[ty][r - 1] : left[ty]);
+                tt = (real)(t0 + u - lp);
+                g = mu[t0 + u];
             }
-            const int r = lo;
-            int lp = r >= 1 ? skp[r - 1] : left;
-            if (t0 == 0 && r == 0) lp = 0;   // sample 0 belongs to interval 0
-            const real tt = (real)(p - lp);
-            const real v = mu[p];
-            sterm[u][0] = v;
-            sterm[u][1] = v * tt;
-            sterm[u][2] = v * tt * tt;
-            sterm[u][3] = v * tt * tt * tt;
+            v[ty][0] = g;
+            v[ty][1] = g * tt;
+            v[ty][2] = v[ty][1] * tt;
+            v[ty][3] = v[ty][2] * tt;
+        }
+        // segmented inclusive scan inside the wave
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+#pragma unroll
+            for (int ty = 0; ty < 2; ++ty) {
+                const int ku = shfl_up(key[ty], o);
+                const bool take = lane >= o && ku == key[ty];
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const real w = shfl_up(v[ty][m], o);
+                    if (take) v[ty][m] = v[ty][m] + w;
+                }
+            }
+        }
+        if (lane == kWave - 1)
+            for (int ty = 0; ty < 2; ++ty) {
+                s_tail_key[ty][wv] = key[ty];
+                for (int m = 0; m < 4; ++m) s_tail[ty][wv][m] = v[ty][m];
+            }
+        if (lane == 0)
+            for (int ty = 0; ty < 2; ++ty) s_head_key[ty][wv] = key[ty];
+        __syncthreads();
+        // carries into each wave (and into the next chunk), one thread per type
+        if (tid < 2) {
+            const int ty = tid;
+            int ck = s_cc_key[ty];
+            real cv[4];
+            for (int m = 0; m < 4; ++m) cv[m] = s_cc[ty][m];
+            for (int w = 0; w < NWV; ++w) {
+                s_in_key[ty][w] = ck;
+                for (int m = 0; m < 4; ++m) s_in[ty][w][m] = cv[m];
+                const int tk = s_tail_key[ty][w];
+                const bool chain = (s_head_key[ty][w] == tk) && (ck == tk);
+                for (int m = 0; m < 4; ++m) cv[m] = s_tail[ty][w][m] + (chain ? cv[m] : (real)0);
+                ck = tk;
+            }
+            s_cc_key[ty] = ck;
+            for (int m = 0; m < 4; ++m) s_cc[ty][m] = cv[m];
         }
         __syncthreads();
-        for (int r = tid; r <= cnt; r += kThreads) {
-            int start = r == 0 ? 0 : skp[r - 1] - t0 + 1;
-            const int end = r == cnt ? nt - 1 : skp[r] - t0;
-            if (t0 == 0 && r == 1) start = 0;     // sample 0 joins interval 0
-            if (t0 == 0 && r == 0) continue;
-            real m0 = (real)0, m1 = (real)0, m2 = (real)0, m3 = (real)0;
-            for (int u = start; u <= end; ++u) {
-                m0 += sterm[u][0];
-                m1 += sterm[u][1];
-                m2 += sterm[u][2];
-                m3 += sterm[u][3];
-            }
+#pragma unroll
+        for (int ty = 0; ty < 2; ++ty) {
+            if (!valid) continue;
+            if (s_in_key[ty][wv] == key[ty])
+                for (int m = 0; m < 4; ++m) v[ty][m] = v[ty][m] + s_in[ty][wv][m];
+            // run ends here: write its moments
+            const bool end = (u == nt - 1) || (key_of(ty, u + 1) != key[ty]);
+            if (!end) continue;
+            const int r = key[ty];
             real* dst;
-            if (r >= 1 && r < cnt) dst = a.mom + 4 * (kb + pre + r - 1);
+            if (r >= 1 && r < cnt[ty]) dst = a.mom + 4 * (kb[ty] + pre[ty] + r - 1);
             else dst = a.edge + 4 * (4 * (int64_t)gt + 2 * ty + (r == 0 ? 0 : 1));
-            dst[0] = m0;
-            dst[1] = m1;
-            dst[2] = m2;
-            dst[3] = m3;
+            for (int m = 0; m < 4; ++m) dst[m] = v[ty][m];
         }
+        // the carry words are rewritten by the next chunk
         __syncthreads();
     }
 }
