@@ -60,6 +60,18 @@ def signals_for(name, sig):
     raise ValueError(name)
 
 
+def measured_traffic(workload, dtype):
+    """HBM bytes per k_sift launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (None when it was not profiled)."""
+    path = os.path.join(ROOT, "profiles", f"k_sift_{workload}_fp{64 if dtype == 'f64' else 32}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(x, I_full, M, S, sample_I, threads):
     """FP64 oracle (OpenMP over realizations) on sample_I of the realizations."""
     import numpy as np
@@ -160,7 +172,8 @@ def main():
         bytes_per_launch = sift_bytes / sift_n
         achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
         line["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
-                            "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                            "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                            "traffic": measured_traffic(args.workload, W["dtype"]),
                             "kernel": "k_sift", "avg_launch_ms": avg_ms,
                             "alg_bytes_per_launch": bytes_per_launch,
                             "launches": int(sift_n)}
