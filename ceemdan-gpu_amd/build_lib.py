@@ -16,6 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libiceemdan_mi355x.so")
+EXAMPLES = os.path.join(HERE, "examples")
+BIN = os.path.join(HERE, "bin")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 ARCH = "gfx950"
@@ -50,7 +52,25 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd))
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_examples(verbose)
     return LIB
+
+
+def build_examples(verbose: bool = False):
+    """Host programs calling the library (linked with an rpath to it)."""
+    os.makedirs(BIN, exist_ok=True)
+    for f in sorted(os.listdir(EXAMPLES)):
+        if not f.endswith(".cpp"):
+            continue
+        exe = os.path.join(BIN, f[:-4])
+        cmd = [HIPCC, "-O2", "-std=c++17", "-Wall", os.path.join(EXAMPLES, f), "-o", exe + ".tmp",
+               f"-L{HERE}", "-liceemdan_mi355x", "-Wl,-rpath,$ORIGIN/.."]
+        if verbose:
+            print(" ".join(cmd))
+        p = subprocess.run(cmd, capture_output=True, text=True)
+        if p.returncode != 0:
+            raise RuntimeError(f"example build failed on {f}:\n{p.stderr}")
+        os.replace(exe + ".tmp", exe)
 
 
 if __name__ == "__main__":
