@@ -28,6 +28,8 @@ template <typename T>
 __device__ __forceinline__ T shfl_up(T v, int d) { return __shfl_up(v, (unsigned)d, kWave); }
 template <typename T>
 __device__ __forceinline__ T shfl_down(T v, int d) { return __shfl_down(v, (unsigned)d, kWave); }
+template <typename T>
+__device__ __forceinline__ T shfl(T v, int src) { return __shfl(v, src, kWave); }
 
 // ------------------------------------------------------------------ Philox
 struct u64x4 {

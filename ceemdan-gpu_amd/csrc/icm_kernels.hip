@@ -1039,6 +1039,301 @@ __global__ __launch_bounds__(1024) void k_series_var(const real* base, const int
     if (threadIdx.x == 0) var_out[s] = ss * (real)(1.0 / (double)(N - 1));
 }
 
+// ------------------------------------------------- sift pass, wave per tile
+// One wavefront sifts a run of consecutive 256-sample tiles of one series;
+// lane l owns samples t0 + 64 r + l, r = 0..3.  Everything a tile needs is
+// wave-local: the knot flags of h are ballots (their ranks give each
+// sample's interval), the knot table [pre-2, pre+cnt] of both types lives in
+// the wave's own LDS slice, and no workgroup barrier is used, so the four
+// waves of a workgroup run independent segments.
+constexpr int kWT = 256;               // samples per tile
+constexpr int kWR = kWT / kWave;       // 64-sample chunks per tile
+constexpr int kWKC = kWT / 2 + 8;      // table entries per type (>= cnt + 3)
+constexpr int kWNC = (kWKC + kWave - 1) / kWave;
+
+template <typename real>
+struct WTile {
+    real h[kWR];
+    real hl, hr;          // h[t0 - 1], h[t0 + kWT] (0 outside the series)
+    real c[2][kWNC];      // c of knots pre-2+j, j = lane + 64 k
+    int nbx;              // lanes 0..5: positions of knots pre-1, pre-2, pre+cnt per type
+    real nby;
+};
+
+template <typename real>
+__device__ __forceinline__ real wave_prev(real v, real carry, int lane)
+{
+    const real x = shfl_up(v, 1);
+    return lane == 0 ? carry : x;
+}
+template <typename real>
+__device__ __forceinline__ real wave_next(real v, real carry, int lane)
+{
+    const real x = shfl_down(v, 1);
+    return lane == kWave - 1 ? carry : x;
+}
+
+template <typename real>
+__global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs)
+{
+    constexpr int NW = kThreads / kWave;
+    __shared__ int s_tp[NW][2][kWKC];
+    __shared__ Coef<real> s_tc[NW][2][kWKC];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int sid = blockIdx.x * NW + wv;
+    if (sid >= nsegs) return;
+    const SeriesDev& sd = a.sd;
+    const Segment seg = a.segs[sid];
+    const int s = seg.s;
+    if (a.active && !a.active[s]) return;
+    int(*tp)[kWKC] = s_tp[wv];
+    Coef<real>(*tc)[kWKC] = s_tc[wv];
+    const int N = sd.len[s];
+    const int tb = sd.tile0[s];
+    const int64_t kb0 = sd.kbase[s], kb1 = sd.kstride + sd.kbase[s];
+    const real* h = a.h_in + sd.off[s];
+    real* hout = a.h_out + sd.off[s];
+    const int nn[2] = {a.kin.n[2 * s], a.kin.n[2 * s + 1]};
+    const real fx[2][2] = {{a.kin.fix[4 * s], a.kin.fix[4 * s + 1]},
+                           {a.kin.fix[4 * s + 2], a.kin.fix[4 * s + 3]}};
+    const bool use_mean = (nn[0] > 3) && (nn[1] > 3);
+
+    auto load = [&](int t, const int* pre, const int* cnt, WTile<real>& w) {
+        const int t0 = t * kWT;
+#pragma unroll
+        for (int r = 0; r < kWR; ++r) {
+            const int p = t0 + r * kWave + lane;
+            w.h[r] = p < N ? h[p] : (real)0;
+        }
+        w.hl = t0 >= 1 ? h[t0 - 1] : (real)0;
+        w.hr = t0 + kWT < N ? h[t0 + kWT] : (real)0;
+#pragma unroll
+        for (int ty = 0; ty < 2; ++ty) {
+            const real* cp = a.c + (ty ? kb1 : kb0);
+#pragma unroll
+            for (int k = 0; k < kWNC; ++k) {
+                const int j = lane + k * kWave;
+                const int g = pre[ty] - 2 + j;
+                w.c[ty][k] = (j < cnt[ty] + 3 && g >= 0 && g < nn[ty]) ? cp[g] : (real)0;
+            }
+        }
+        w.nbx = -1;
+        w.nby = (real)0;
+        if (lane < 6) {
+            const int64_t o = 6 * (int64_t)(tb + t) + lane;
+            w.nbx = a.kin.nbx[o];
+            w.nby = a.kin.nby[o];
+        }
+    };
+
+    int mpre[2], mcnt[2], npre[2] = {0, 0}, ncnt[2] = {0, 0};
+    {
+        const int gt = tb + seg.t_first;
+        mpre[0] = a.kin.pre[2 * gt];
+        mpre[1] = a.kin.pre[2 * gt + 1];
+        mcnt[0] = a.kin.cnt[2 * gt];
+        mcnt[1] = a.kin.cnt[2 * gt + 1];
+    }
+    if (seg.count > 1) {
+        const int gt = tb + seg.t_first + 1;
+        npre[0] = a.kin.pre[2 * gt];
+        npre[1] = a.kin.pre[2 * gt + 1];
+        ncnt[0] = a.kin.cnt[2 * gt];
+        ncnt[1] = a.kin.cnt[2 * gt + 1];
+    }
+    WTile<real> cur, nxt;
+    load(seg.t_first, mpre, mcnt, cur);
+
+    for (int i = 0; i < seg.count; ++i) {
+        const int t = seg.t_first + i;
+        const int gt = tb + t;
+        const int t0 = t * kWT;
+        const int nt = min(kWT, N - t0);
+        const int pre[2] = {mpre[0], mpre[1]}, cnt[2] = {mcnt[0], mcnt[1]};
+        int n2pre[2] = {0, 0}, n2cnt[2] = {0, 0};
+        if (i + 1 < seg.count) {
+            load(t + 1, npre, ncnt, nxt);
+            if (i + 2 < seg.count) {
+                const int g2 = gt + 2;
+                n2pre[0] = a.kin.pre[2 * g2];
+                n2pre[1] = a.kin.pre[2 * g2 + 1];
+                n2cnt[0] = a.kin.cnt[2 * g2];
+                n2cnt[1] = a.kin.cnt[2 * g2 + 1];
+            }
+        }
+
+        // ---- knot flags of h (find_extrema_shfl_*, cudaICEEMDAN.cu:92-148)
+        uint64_t bm[2][kWR];
+        int cb[2][kWR];
+        int tot[2] = {0, 0};
+#pragma unroll
+        for (int r = 0; r < kWR; ++r) {
+            const int u = r * kWave + lane;
+            const int p = t0 + u;
+            const real lft = wave_prev(cur.h[r], r == 0 ? cur.hl : shfl(cur.h[r > 0 ? r - 1 : 0], kWave - 1), lane);
+            const real rgt = wave_next(cur.h[r], r == kWR - 1 ? cur.hr : shfl(cur.h[r < kWR - 1 ? r + 1 : r], 0), lane);
+            const bool valid = u < nt;
+            const bool end = p == 0 || p == N - 1;
+            const real v = cur.h[r];
+            bm[0][r] = wave_ballot(valid && (end || ((v > lft) && (v > rgt))));
+            bm[1][r] = wave_ballot(valid && (end || ((v < lft) && (v < rgt))));
+#pragma unroll
+            for (int ty = 0; ty < 2; ++ty) {
+                cb[ty][r] = tot[ty];
+                tot[ty] += __popcll(bm[ty][r]);
+            }
+        }
+        if (tot[0] != cnt[0] || tot[1] != cnt[1] || cnt[0] + 3 > kWKC || cnt[1] + 3 > kWKC) {
+            if (lane == 0) atomicOr(a.pc.err, ICM_DEVERR_KNOTCAP);
+            return;
+        }
+
+        // ---- knot table: index j <-> knot pre - 2 + j
+#pragma unroll
+        for (int r = 0; r < kWR; ++r) {
+            const int p = t0 + r * kWave + lane;
+#pragma unroll
+            for (int ty = 0; ty < 2; ++ty) {
+                if ((bm[ty][r] >> lane) & 1ull) {
+                    const int j = cb[ty][r] + lanes_below(bm[ty][r]) + 2;
+                    const int g = pre[ty] - 2 + j;
+                    real y = cur.h[r];
+                    if (g == 0) y = fx[ty][0];
+                    if (g == nn[ty] - 1) y = fx[ty][1];
+                    tp[ty][j] = p;
+                    tc[ty][j].a = y;
+                }
+            }
+        }
+        if (lane < 6) {
+            const int ty = lane / 3, w = lane % 3;
+            const int j = w == 0 ? 1 : (w == 1 ? 0 : cnt[ty] + 2);
+            const int g = pre[ty] - 2 + j;
+            if (cur.nbx >= 0 && g >= 0 && g < nn[ty]) {
+                real y = cur.nby;
+                if (g == 0) y = fx[ty][0];
+                if (g == nn[ty] - 1) y = fx[ty][1];
+                tp[ty][j] = cur.nbx;
+                tc[ty][j].a = y;
+            }
+        }
+#pragma unroll
+        for (int ty = 0; ty < 2; ++ty)
+#pragma unroll
+            for (int k = 0; k < kWNC; ++k) {
+                const int j = lane + k * kWave;
+                if (j < cnt[ty] + 3) tc[ty][j].c = cur.c[ty][k];
+            }
+        wave_sync();
+        // ---- interval coefficients (spline_coefficients, cudaICEEMDAN.cu:394-398)
+#pragma unroll
+        for (int ty = 0; ty < 2; ++ty)
+            for (int j = lane; j < cnt[ty] + 2; j += kWave) {
+                const int g = pre[ty] - 2 + j;
+                if (g < 0 || g + 1 > nn[ty] - 1) continue;
+                const real hk = (real)(tp[ty][j + 1] - tp[ty][j]);
+                const real a0 = tc[ty][j].a, a1 = tc[ty][j + 1].a;
+                const real c0 = tc[ty][j].c, c1 = tc[ty][j + 1].c;
+                tc[ty][j].b = (a1 - a0) / hk - hk * ((real)2 * c0 + c1) / (real)3;
+                tc[ty][j].d = (c1 - c0) / ((real)3 * hk);
+            }
+        wave_sync();
+
+        // ---- envelopes, mean, h_new = h - m (interpolate :403-439,
+        // averageUppperLower :441-454, updateRealizations :527-544)
+        real hn[kWR];
+        int sxc = 0;
+        bool sxm = false;
+#pragma unroll
+        for (int r = 0; r < kWR; ++r) {
+            const int u = r * kWave + lane;
+            const int p = t0 + u;
+            hn[r] = cur.h[r];
+            if (u < nt) {
+                const int ju = p == 0 ? 2 : cb[0][r] + lanes_below(bm[0][r]) + 1;
+                const int jl = p == 0 ? 2 : cb[1][r] + lanes_below(bm[1][r]) + 1;
+                const real U = horner(tc[0][ju], (real)(p - tp[0][ju]));
+                const real L = horner(tc[1][jl], (real)(p - tp[1][jl]));
+                const real mm = use_mean ? (U + L) / (real)2 : (real)0;
+                hn[r] = cur.h[r] - mm;
+                hout[p] = hn[r];
+                if (a.sx_cnt) {
+                    const real sx = fabs((U + L) / (U - L));
+                    sxc += (sx > (real)0.05) ? 1 : 0;
+                    sxm = sxm || (sx > (real)0.5);
+                }
+            }
+        }
+        if (a.sx_cnt) {
+            for (int o = 32; o >= 1; o >>= 1) sxc += __shfl_xor(sxc, o, kWave);
+            const uint64_t mk = wave_ballot(sxm);
+            if (lane == 0) {
+                a.sx_cnt[gt] = sxc;
+                a.sx_mark[gt] = mk != 0ull ? 1 : 0;
+            }
+        }
+
+        if (a.emit) {
+            // h_new just outside the tile, evaluated by every lane (same LDS words)
+            real hnl = (real)0, hnr = (real)0;
+            if (t0 >= 1) {
+                const int p = t0 - 1;
+                const int ju = tp[0][1] == p ? 0 : 1;
+                const int jl = tp[1][1] == p ? 0 : 1;
+                const real U = horner(tc[0][ju], (real)(p - tp[0][ju]));
+                const real L = horner(tc[1][jl], (real)(p - tp[1][jl]));
+                hnl = cur.hl - (use_mean ? (U + L) / (real)2 : (real)0);
+            }
+            if (t0 + kWT < N) {
+                const int p = t0 + kWT;
+                const int ju = cnt[0] + 1, jl = cnt[1] + 1;
+                const real U = horner(tc[0][ju], (real)(p - tp[0][ju]));
+                const real L = horner(tc[1][jl], (real)(p - tp[1][jl]));
+                hnr = cur.hr - (use_mean ? (U + L) / (real)2 : (real)0);
+            }
+            // ---- knots of h_new into the tile's slots (select_extrema_*, :150-207)
+            const int64_t slot0 = (int64_t)gt * sd.cap;
+            int run[2] = {0, 0};
+#pragma unroll
+            for (int r = 0; r < kWR; ++r) {
+                const int u = r * kWave + lane;
+                const int p = t0 + u;
+                const real lft = wave_prev(hn[r], r == 0 ? hnl : shfl(hn[r > 0 ? r - 1 : 0], kWave - 1), lane);
+                const real rgt = wave_next(hn[r], r == kWR - 1 ? hnr : shfl(hn[r < kWR - 1 ? r + 1 : r], 0), lane);
+                const bool valid = u < nt;
+                const bool end = p == 0 || p == N - 1;
+                const real v = hn[r];
+                const uint64_t fm[2] = {wave_ballot(valid && (end || ((v > lft) && (v > rgt)))),
+                                        wave_ballot(valid && (end || ((v < lft) && (v < rgt))))};
+#pragma unroll
+                for (int ty = 0; ty < 2; ++ty) {
+                    if ((fm[ty] >> lane) & 1ull) {
+                        const int64_t o = (ty ? sd.sstride : 0) + slot0 + run[ty] + lanes_below(fm[ty]);
+                        a.kout.x[o] = p;
+                        a.kout.y[o] = v;
+                    }
+                    run[ty] += __popcll(fm[ty]);
+                }
+            }
+            if (lane == 0) {
+                a.kout.cnt[2 * gt] = run[0];
+                a.kout.cnt[2 * gt + 1] = run[1];
+            }
+        }
+        // the table is rewritten by the next tile
+        wave_sync();
+        cur = nxt;
+        mpre[0] = npre[0];
+        mpre[1] = npre[1];
+        mcnt[0] = ncnt[0];
+        mcnt[1] = ncnt[1];
+        npre[0] = n2pre[0];
+        npre[1] = n2pre[1];
+        ncnt[0] = n2cnt[0];
+        ncnt[1] = n2cnt[1];
+    }
+}
+
 // ================================================================ adjoint
 // With the extrema sets of every pass frozen (recorded by the forward), one
 // sift pass is linear: h' = h - (U + L) / 2.  Its transpose maps the
@@ -1731,7 +2026,8 @@ hipError_t launch_sift(const SiftArgs<real>& a, int nsegs, hipStream_t st)
 {
     const int T = 2 * (a.sd.cap - 2);
     if (T == 256)
-        hipLaunchKernelGGL((k_sift<real, 256>), dim3(nsegs), dim3(kThreads), 0, st, a);
+        hipLaunchKernelGGL((k_sift_w<real>), dim3(blocks_for(nsegs, kThreads / kWave)),
+                           dim3(kThreads), 0, st, a, nsegs);
     else if (T == 512)
         hipLaunchKernelGGL((k_sift<real, 512>), dim3(nsegs), dim3(kThreads), 0, st, a);
     else if (T == 1024)

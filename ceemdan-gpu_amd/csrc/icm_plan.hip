@@ -45,12 +45,13 @@ const uint32_t kEpochs = 65536;
 
 // --------------------------------------------------------------- geometry
 
-// samples per tile of the tile kernels (256, 512 or 1024)
+// samples per tile of the tile kernels: 256 (one wavefront per tile in the
+// sift pass), or 512 / 1024 (one workgroup per tile)
 int tile_of(int)
 {
     const char* env = std::getenv("ICM_TILE");
-    const int t = env ? std::atoi(env) : 512;
-    return (t == 256 || t == 1024) ? t : 512;
+    const int t = env ? std::atoi(env) : 256;
+    return (t == 512 || t == 1024) ? t : 256;
 }
 int solve_B(int dtype) { return dtype == ICM_DTYPE_F64 ? 12 : 8; }
 int solve_H(int dtype) { return dtype == ICM_DTYPE_F64 ? 64 : 32; }
@@ -127,12 +128,13 @@ static void build_series(HostSeries& hs, const std::vector<int64_t>& lens, int I
     hs.samples = off;
     hs.kcap = kb;
     hs.ntiles = tiles;
-    // segments of consecutive tiles, one workgroup each: long enough to
-    // amortize the prefetch pipeline, short enough to fill every CU
+    // segments of consecutive tiles (one wavefront each for 256-sample
+    // tiles, one workgroup each otherwise): long enough to amortize the
+    // prefetch pipeline, short enough to fill every CU
     int seg = 1;
     const char* env = std::getenv("ICM_SEGMENT_TILES");
     if (env && std::atoi(env) > 0) seg = std::atoi(env);
-    else seg = std::max(1, std::min(32, tiles / 6144));
+    else seg = std::max(1, std::min(32, tiles / (T == 256 ? 16384 : 6144)));
     for (size_t s = 0; s < hs.ntl.size(); ++s)
         for (int t = 0; t < hs.ntl[s]; t += seg) {
             Segment g;
