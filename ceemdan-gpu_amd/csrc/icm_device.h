@@ -31,6 +31,53 @@ __device__ __forceinline__ T shfl_down(T v, int d) { return __shfl_down(v, (unsi
 template <typename T>
 __device__ __forceinline__ T shfl(T v, int src) { return __shfl(v, src, kWave); }
 
+// Whole-wave shifts by one lane (DPP wave_shr:1 / wave_shl:1): lane i gets
+// lane i-1 (shift_prev) or lane i+1 (shift_next); the lane with no source
+// gets `fill`.  Bit moves only, so they are exact for any type.
+__device__ __forceinline__ int dpp_prev_i(int v, int fill)
+{
+    return __builtin_amdgcn_update_dpp(fill, v, 0x138, 0xF, 0xF, false);
+}
+__device__ __forceinline__ int dpp_next_i(int v, int fill)
+{
+    return __builtin_amdgcn_update_dpp(fill, v, 0x130, 0xF, 0xF, false);
+}
+__device__ __forceinline__ float shift_prev(float v, float fill)
+{
+    return __int_as_float(dpp_prev_i(__float_as_int(v), __float_as_int(fill)));
+}
+__device__ __forceinline__ float shift_next(float v, float fill)
+{
+    return __int_as_float(dpp_next_i(__float_as_int(v), __float_as_int(fill)));
+}
+__device__ __forceinline__ double shift_prev(double v, double fill)
+{
+    const long long b = __double_as_longlong(v), f = __double_as_longlong(fill);
+    const int lo = dpp_prev_i((int)b, (int)f);
+    const int hi = dpp_prev_i((int)(b >> 32), (int)(f >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double shift_next(double v, double fill)
+{
+    const long long b = __double_as_longlong(v), f = __double_as_longlong(fill);
+    const int lo = dpp_next_i((int)b, (int)f);
+    const int hi = dpp_next_i((int)(b >> 32), (int)(f >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// value of lane `l` (wave-uniform result)
+__device__ __forceinline__ float lane_value(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double lane_value(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // ------------------------------------------------------------------ Philox
 struct u64x4 {
     uint64_t v[4];

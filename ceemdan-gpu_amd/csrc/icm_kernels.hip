@@ -17,6 +17,8 @@
 // (all cudaICEEMDAN.cu).
 #include "icm_device.h"
 #include "icm_kernels.h"
+
+#include <type_traits>
 #include "icm_errors.h"
 
 namespace icm {
@@ -1060,18 +1062,6 @@ struct WTile {
     real nby;
 };
 
-template <typename real>
-__device__ __forceinline__ real wave_prev(real v, real carry, int lane)
-{
-    const real x = shfl_up(v, 1);
-    return lane == 0 ? carry : x;
-}
-template <typename real>
-__device__ __forceinline__ real wave_next(real v, real carry, int lane)
-{
-    const real x = shfl_down(v, 1);
-    return lane == kWave - 1 ? carry : x;
-}
 
 template <typename real>
 __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs)
@@ -1162,73 +1152,78 @@ __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs
             }
         }
 
-        // ---- knot flags of h (find_extrema_shfl_*, cudaICEEMDAN.cu:92-148)
-        uint64_t bm[2][kWR];
-        int cb[2][kWR];
-        int tot[2] = {0, 0};
+        // interior tiles (neither series end inside, all 256 samples valid)
+        // take a copy of the body without the per-lane end tests
+        auto tile = [&](auto full_c) -> bool {
+            constexpr bool FULL = decltype(full_c)::value;
+            // ---- knot flags of h (find_extrema_shfl_*, cudaICEEMDAN.cu:92-148)
+            uint64_t bm[2][kWR];
+            int cb[2][kWR];
+            int tot[2] = {0, 0};
 #pragma unroll
-        for (int r = 0; r < kWR; ++r) {
-            const int u = r * kWave + lane;
-            const int p = t0 + u;
-            const real lft = wave_prev(cur.h[r], r == 0 ? cur.hl : shfl(cur.h[r > 0 ? r - 1 : 0], kWave - 1), lane);
-            const real rgt = wave_next(cur.h[r], r == kWR - 1 ? cur.hr : shfl(cur.h[r < kWR - 1 ? r + 1 : r], 0), lane);
-            const bool valid = u < nt;
-            const bool end = p == 0 || p == N - 1;
-            const real v = cur.h[r];
-            bm[0][r] = wave_ballot(valid && (end || ((v > lft) && (v > rgt))));
-            bm[1][r] = wave_ballot(valid && (end || ((v < lft) && (v < rgt))));
+            for (int r = 0; r < kWR; ++r) {
+                const int u = r * kWave + lane;
+                const int p = t0 + u;
+                const real lft = shift_prev(cur.h[r], r == 0 ? cur.hl : lane_value(cur.h[r > 0 ? r - 1 : 0], kWave - 1));
+                const real rgt = shift_next(cur.h[r], r == kWR - 1 ? cur.hr : lane_value(cur.h[r < kWR - 1 ? r + 1 : r], 0));
+                const bool valid = FULL || u < nt;
+                const bool end = !FULL && (p == 0 || p == N - 1);
+                const real v = cur.h[r];
+                bm[0][r] = wave_ballot(valid && (end || ((v > lft) && (v > rgt))));
+                bm[1][r] = wave_ballot(valid && (end || ((v < lft) && (v < rgt))));
 #pragma unroll
-            for (int ty = 0; ty < 2; ++ty) {
-                cb[ty][r] = tot[ty];
-                tot[ty] += __popcll(bm[ty][r]);
+                for (int ty = 0; ty < 2; ++ty) {
+                    cb[ty][r] = tot[ty];
+                    tot[ty] += __popcll(bm[ty][r]);
+                }
             }
-        }
-        if (tot[0] != cnt[0] || tot[1] != cnt[1] || cnt[0] + 3 > kWKC || cnt[1] + 3 > kWKC) {
-            if (lane == 0) atomicOr(a.pc.err, ICM_DEVERR_KNOTCAP);
-            return;
-        }
+            if (tot[0] != cnt[0] || tot[1] != cnt[1] || cnt[0] + 3 > kWKC || cnt[1] + 3 > kWKC) {
+                if (lane == 0) atomicOr(a.pc.err, ICM_DEVERR_KNOTCAP);
+                return false;
+            }
 
-        // ---- knot table: index j <-> knot pre - 2 + j
+            // ---- knot table: index j <-> knot pre - 2 + j
 #pragma unroll
-        for (int r = 0; r < kWR; ++r) {
-            const int p = t0 + r * kWave + lane;
+            for (int r = 0; r < kWR; ++r) {
+                const int p = t0 + r * kWave + lane;
 #pragma unroll
-            for (int ty = 0; ty < 2; ++ty) {
-                if ((bm[ty][r] >> lane) & 1ull) {
-                    const int j = cb[ty][r] + lanes_below(bm[ty][r]) + 2;
-                    const int g = pre[ty] - 2 + j;
-                    real y = cur.h[r];
+                for (int ty = 0; ty < 2; ++ty) {
+                    if ((bm[ty][r] >> lane) & 1ull) {
+                        const int j = cb[ty][r] + lanes_below(bm[ty][r]) + 2;
+                        const int g = pre[ty] - 2 + j;
+                        real y = cur.h[r];
+                        if (g == 0) y = fx[ty][0];
+                        if (g == nn[ty] - 1) y = fx[ty][1];
+                        tp[ty][j] = p;
+                        tc[ty][j].a = y;
+                    }
+                }
+            }
+            if (lane < 6) {
+                const int ty = lane / 3, w = lane % 3;
+                const int j = w == 0 ? 1 : (w == 1 ? 0 : cnt[ty] + 2);
+                const int g = pre[ty] - 2 + j;
+                if (cur.nbx >= 0 && g >= 0 && g < nn[ty]) {
+                    real y = cur.nby;
                     if (g == 0) y = fx[ty][0];
                     if (g == nn[ty] - 1) y = fx[ty][1];
-                    tp[ty][j] = p;
+                    tp[ty][j] = cur.nbx;
                     tc[ty][j].a = y;
                 }
             }
-        }
-        if (lane < 6) {
-            const int ty = lane / 3, w = lane % 3;
-            const int j = w == 0 ? 1 : (w == 1 ? 0 : cnt[ty] + 2);
-            const int g = pre[ty] - 2 + j;
-            if (cur.nbx >= 0 && g >= 0 && g < nn[ty]) {
-                real y = cur.nby;
-                if (g == 0) y = fx[ty][0];
-                if (g == nn[ty] - 1) y = fx[ty][1];
-                tp[ty][j] = cur.nbx;
-                tc[ty][j].a = y;
-            }
-        }
 #pragma unroll
-        for (int ty = 0; ty < 2; ++ty)
+            for (int ty = 0; ty < 2; ++ty)
 #pragma unroll
-            for (int k = 0; k < kWNC; ++k) {
-                const int j = lane + k * kWave;
-                if (j < cnt[ty] + 3) tc[ty][j].c = cur.c[ty][k];
-            }
-        wave_sync();
-        // ---- interval coefficients (spline_coefficients, cudaICEEMDAN.cu:394-398)
-#pragma unroll
-        for (int ty = 0; ty < 2; ++ty)
-            for (int j = lane; j < cnt[ty] + 2; j += kWave) {
+                for (int k = 0; k < kWNC; ++k) {
+                    const int j = lane + k * kWave;
+                    if (j < cnt[ty] + 3) tc[ty][j].c = cur.c[ty][k];
+                }
+            wave_sync();
+            // ---- interval coefficients (spline_coefficients, cudaICEEMDAN.cu:394-398)
+            // both types in one strided loop (intervals j of type 0, then of type 1)
+            for (int q = lane; q < cnt[0] + cnt[1] + 4; q += kWave) {
+                const int ty = q >= cnt[0] + 2;
+                const int j = ty ? q - (cnt[0] + 2) : q;
                 const int g = pre[ty] - 2 + j;
                 if (g < 0 || g + 1 > nn[ty] - 1) continue;
                 const real hk = (real)(tp[ty][j + 1] - tp[ty][j]);
@@ -1237,89 +1232,95 @@ __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs
                 tc[ty][j].b = (a1 - a0) / hk - hk * ((real)2 * c0 + c1) / (real)3;
                 tc[ty][j].d = (c1 - c0) / ((real)3 * hk);
             }
-        wave_sync();
+            wave_sync();
 
-        // ---- envelopes, mean, h_new = h - m (interpolate :403-439,
-        // averageUppperLower :441-454, updateRealizations :527-544)
-        real hn[kWR];
-        int sxc = 0;
-        bool sxm = false;
-#pragma unroll
-        for (int r = 0; r < kWR; ++r) {
-            const int u = r * kWave + lane;
-            const int p = t0 + u;
-            hn[r] = cur.h[r];
-            if (u < nt) {
-                const int ju = p == 0 ? 2 : cb[0][r] + lanes_below(bm[0][r]) + 1;
-                const int jl = p == 0 ? 2 : cb[1][r] + lanes_below(bm[1][r]) + 1;
-                const real U = horner(tc[0][ju], (real)(p - tp[0][ju]));
-                const real L = horner(tc[1][jl], (real)(p - tp[1][jl]));
-                const real mm = use_mean ? (U + L) / (real)2 : (real)0;
-                hn[r] = cur.h[r] - mm;
-                hout[p] = hn[r];
-                if (a.sx_cnt) {
-                    const real sx = fabs((U + L) / (U - L));
-                    sxc += (sx > (real)0.05) ? 1 : 0;
-                    sxm = sxm || (sx > (real)0.5);
-                }
-            }
-        }
-        if (a.sx_cnt) {
-            for (int o = 32; o >= 1; o >>= 1) sxc += __shfl_xor(sxc, o, kWave);
-            const uint64_t mk = wave_ballot(sxm);
-            if (lane == 0) {
-                a.sx_cnt[gt] = sxc;
-                a.sx_mark[gt] = mk != 0ull ? 1 : 0;
-            }
-        }
-
-        if (a.emit) {
-            // h_new just outside the tile, evaluated by every lane (same LDS words)
-            real hnl = (real)0, hnr = (real)0;
-            if (t0 >= 1) {
-                const int p = t0 - 1;
-                const int ju = tp[0][1] == p ? 0 : 1;
-                const int jl = tp[1][1] == p ? 0 : 1;
-                const real U = horner(tc[0][ju], (real)(p - tp[0][ju]));
-                const real L = horner(tc[1][jl], (real)(p - tp[1][jl]));
-                hnl = cur.hl - (use_mean ? (U + L) / (real)2 : (real)0);
-            }
-            if (t0 + kWT < N) {
-                const int p = t0 + kWT;
-                const int ju = cnt[0] + 1, jl = cnt[1] + 1;
-                const real U = horner(tc[0][ju], (real)(p - tp[0][ju]));
-                const real L = horner(tc[1][jl], (real)(p - tp[1][jl]));
-                hnr = cur.hr - (use_mean ? (U + L) / (real)2 : (real)0);
-            }
-            // ---- knots of h_new into the tile's slots (select_extrema_*, :150-207)
-            const int64_t slot0 = (int64_t)gt * sd.cap;
-            int run[2] = {0, 0};
+            // ---- envelopes, mean, h_new = h - m (interpolate :403-439,
+            // averageUppperLower :441-454, updateRealizations :527-544)
+            real hn[kWR];
+            int sxc = 0;
+            bool sxm = false;
 #pragma unroll
             for (int r = 0; r < kWR; ++r) {
                 const int u = r * kWave + lane;
                 const int p = t0 + u;
-                const real lft = wave_prev(hn[r], r == 0 ? hnl : shfl(hn[r > 0 ? r - 1 : 0], kWave - 1), lane);
-                const real rgt = wave_next(hn[r], r == kWR - 1 ? hnr : shfl(hn[r < kWR - 1 ? r + 1 : r], 0), lane);
-                const bool valid = u < nt;
-                const bool end = p == 0 || p == N - 1;
-                const real v = hn[r];
-                const uint64_t fm[2] = {wave_ballot(valid && (end || ((v > lft) && (v > rgt)))),
-                                        wave_ballot(valid && (end || ((v < lft) && (v < rgt))))};
-#pragma unroll
-                for (int ty = 0; ty < 2; ++ty) {
-                    if ((fm[ty] >> lane) & 1ull) {
-                        const int64_t o = (ty ? sd.sstride : 0) + slot0 + run[ty] + lanes_below(fm[ty]);
-                        a.kout.x[o] = p;
-                        a.kout.y[o] = v;
+                hn[r] = cur.h[r];
+                if (FULL || u < nt) {
+                    const bool p0 = !FULL && p == 0;
+                    const int ju = p0 ? 2 : cb[0][r] + lanes_below(bm[0][r]) + 1;
+                    const int jl = p0 ? 2 : cb[1][r] + lanes_below(bm[1][r]) + 1;
+                    const real U = horner(tc[0][ju], (real)(p - tp[0][ju]));
+                    const real L = horner(tc[1][jl], (real)(p - tp[1][jl]));
+                    const real mm = use_mean ? (U + L) / (real)2 : (real)0;
+                    hn[r] = cur.h[r] - mm;
+                    hout[p] = hn[r];
+                    if (a.sx_cnt) {
+                        const real sx = fabs((U + L) / (U - L));
+                        sxc += (sx > (real)0.05) ? 1 : 0;
+                        sxm = sxm || (sx > (real)0.5);
                     }
-                    run[ty] += __popcll(fm[ty]);
                 }
             }
-            if (lane == 0) {
-                a.kout.cnt[2 * gt] = run[0];
-                a.kout.cnt[2 * gt + 1] = run[1];
+            if (a.sx_cnt) {
+                for (int o = 32; o >= 1; o >>= 1) sxc += __shfl_xor(sxc, o, kWave);
+                const uint64_t mk = wave_ballot(sxm);
+                if (lane == 0) {
+                    a.sx_cnt[gt] = sxc;
+                    a.sx_mark[gt] = mk != 0ull ? 1 : 0;
+                }
             }
-        }
+
+            if (a.emit) {
+                // h_new just outside the tile, evaluated by every lane (same LDS words)
+                real hnl = (real)0, hnr = (real)0;
+                if (FULL || t0 >= 1) {
+                    const int p = t0 - 1;
+                    const int ju = tp[0][1] == p ? 0 : 1;
+                    const int jl = tp[1][1] == p ? 0 : 1;
+                    const real U = horner(tc[0][ju], (real)(p - tp[0][ju]));
+                    const real L = horner(tc[1][jl], (real)(p - tp[1][jl]));
+                    hnl = cur.hl - (use_mean ? (U + L) / (real)2 : (real)0);
+                }
+                if (FULL || t0 + kWT < N) {
+                    const int p = t0 + kWT;
+                    const int ju = cnt[0] + 1, jl = cnt[1] + 1;
+                    const real U = horner(tc[0][ju], (real)(p - tp[0][ju]));
+                    const real L = horner(tc[1][jl], (real)(p - tp[1][jl]));
+                    hnr = cur.hr - (use_mean ? (U + L) / (real)2 : (real)0);
+                }
+                // ---- knots of h_new into the tile's slots (select_extrema_*, :150-207)
+                const int64_t slot0 = (int64_t)gt * sd.cap;
+                int run[2] = {0, 0};
+#pragma unroll
+                for (int r = 0; r < kWR; ++r) {
+                    const int u = r * kWave + lane;
+                    const int p = t0 + u;
+                    const real lft = shift_prev(hn[r], r == 0 ? hnl : lane_value(hn[r > 0 ? r - 1 : 0], kWave - 1));
+                    const real rgt = shift_next(hn[r], r == kWR - 1 ? hnr : lane_value(hn[r < kWR - 1 ? r + 1 : r], 0));
+                    const bool valid = FULL || u < nt;
+                    const bool end = !FULL && (p == 0 || p == N - 1);
+                    const real v = hn[r];
+                    const uint64_t fm[2] = {wave_ballot(valid && (end || ((v > lft) && (v > rgt)))),
+                                            wave_ballot(valid && (end || ((v < lft) && (v < rgt))))};
+#pragma unroll
+                    for (int ty = 0; ty < 2; ++ty) {
+                        if ((fm[ty] >> lane) & 1ull) {
+                            const int64_t o = (ty ? sd.sstride : 0) + slot0 + run[ty] + lanes_below(fm[ty]);
+                            a.kout.x[o] = p;
+                            a.kout.y[o] = v;
+                        }
+                        run[ty] += __popcll(fm[ty]);
+                    }
+                }
+                if (lane == 0) {
+                    a.kout.cnt[2 * gt] = run[0];
+                    a.kout.cnt[2 * gt + 1] = run[1];
+                }
+            }
+            return true;
+        };
+        const bool interior = t0 > 0 && t0 + kWT < N;
+        const bool ok = interior ? tile(std::true_type{}) : tile(std::false_type{});
+        if (!ok) return;
         // the table is rewritten by the next tile
         wave_sync();
         cur = nxt;
