@@ -72,6 +72,33 @@ def measured_traffic(workload, dtype):
     return None
 
 
+def batched_vs_per_signal(icm, sig, torch, dev, channels=8, N=16384, I=100, M=12, S=10):
+    """Sample-sifts/s of one plan over `channels` channels against one plan
+    per channel run back to back (same FIXED settings, float64)."""
+    xs = [torch.tensor(sig.eeg_like(N, seed=4000 + c), device=dev) for c in range(channels)]
+    sifts = (M - 1) * I * N * S * channels
+    plan = icm.Plan([N] * channels, I, M, dtype=torch.float64, max_sifts=S, use_graph=True,
+                    device=dev)
+    plan.forward(xs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan.forward(xs)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    plan.close()
+    single = icm.Plan([N], I, M, dtype=torch.float64, max_sifts=S, use_graph=True, device=dev)
+    single.forward([xs[0]])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for x in xs:
+        single.forward([x])
+    torch.cuda.synchronize()
+    ts = time.perf_counter() - t0
+    single.close()
+    return {"workload": f"{channels} EEG-like channels x N={N}, I={I}, M={M}, {S} sifts, float64",
+            "batched": sifts / tb, "per_signal": sifts / ts, "ratio": ts / tb}
+
+
 def cpu_baseline(x, I_full, M, S, sample_I, threads):
     """FP64 oracle (OpenMP over realizations) on sample_I of the realizations."""
     import numpy as np
@@ -97,6 +124,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=24, help="realizations timed on the CPU")
     ap.add_argument("--graph", action="store_true", help="time the hipGraph replay (no kernel events)")
     ap.add_argument("--no-backward", action="store_true", help="skip the adjoint timing")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched vs per-signal comparison")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -191,6 +220,8 @@ def main():
         line["backward"] = {"ms_per_call": bwd_ms, "backward_forward_ratio": bwd_ms / ms_per_step,
                             "loss": "sum_k w_k |IMF_k|^2, w_k ~ U(0,1) seed 5",
                             "grad_finite": all(bool(torch.isfinite(g).all()) for g in gx)}
+    if not args.no_batched:
+        line["batched_vs_per_signal"] = batched_vs_per_signal(icm, sig, torch, dev)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         sample_I = max(1, min(args.cpu_sample, I))
