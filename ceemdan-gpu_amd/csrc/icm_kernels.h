@@ -135,8 +135,29 @@ struct AdjArgs {
     int* err;
 };
 
+// One launch of the fused per-tile adjoint step (256-sample tiles, one
+// wavefront each): optional broadcast of dL/dr_{k+1} into mu, optional
+// scatter of the knot gradients of pass `sc` (mu <- mu - G^T/2), optional
+// per-interval moments of the resulting mu over the knots of pass `mo`.
+template <typename real>
+struct AdjTileArgs {
+    SeriesDev sd;
+    real* mu;
+    const real* bcast;          // non-null: mu = bcast[chan_off[chan[s]] + p] first
+    const int64_t* chan_off;
+    PassRec sc;                 // scatter pass (sc.px == nullptr: none)
+    const real* ga;             // knot gradients of pass sc
+    PassRec mo;                 // moments pass (mo.px == nullptr: none)
+    real* mom;
+    real* edge;
+};
+
 template <typename real>
 hipError_t launch_adjoint_pass(const AdjArgs<real>& a, int solve_grid, hipStream_t st);
+template <typename real>
+hipError_t launch_adj_tile(const AdjTileArgs<real>& a, hipStream_t st);
+template <typename real>
+hipError_t launch_adj_span_knot(const AdjArgs<real>& a, int solve_grid, hipStream_t st);
 template <typename real>
 hipError_t launch_adj_seed(const real* gimf, int M, int64_t N, int64_t stride, real* gr,
                            hipStream_t st);

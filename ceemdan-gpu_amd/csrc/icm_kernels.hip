@@ -1508,6 +1508,196 @@ __global__ __launch_bounds__(kThreads) void k_adj_mom(AdjArgs<real> a)
     }
 }
 
+// 1'. Fused per-tile step for 256-sample tiles: one wavefront per tile,
+// lane l owns the four consecutive samples 4l .. 4l+3.  Optionally
+// mu <- broadcast, then mu <- mu - (G_max^T + G_min^T)/2 for the scatter
+// pass, then the moments of mu over the knot intervals of the moment pass:
+// each lane sums its own samples run by run; the run a lane ends with is
+// continued across lanes by a segmented scan keyed by the run number.
+template <typename real>
+__global__ __launch_bounds__(kThreads) void k_adj_tile(AdjTileArgs<real> a)
+{
+    constexpr int T = kWT;
+    constexpr int NW = kThreads / kWave;
+    constexpr int KC = T / 2 + 2;
+    constexpr int NOKEY = 1 << 28;
+    __shared__ real s_delta[NW][T];
+    __shared__ int s_tp[NW][KC];
+    __shared__ unsigned long long s_bits[NW][T / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int gt = blockIdx.x * NW + wv;
+    const SeriesDev& sd = a.sd;
+    if (gt >= sd.ntiles) return;
+    const int s = sd.tile_series[gt];
+    const int t = gt - sd.tile0[s];
+    const int N = sd.len[s];
+    const int t0 = t * T;
+    const int nt = min(T, N - t0);
+    real* mu = a.mu + sd.off[s];
+    real* delta = s_delta[wv];
+    int* tp = s_tp[wv];
+    unsigned long long* bits = s_bits[wv];
+    const int ub = 4 * lane;
+
+    real v[4];
+    bool dirty = false;
+    if (a.bcast) {
+        const real* src = a.bcast + a.chan_off[sd.chan[s]];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = ub + q < nt ? src[t0 + ub + q] : (real)0;
+        dirty = true;
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = ub + q < nt ? mu[t0 + ub + q] : (real)0;
+    }
+    // ---- scatter of the knot gradients of pass sc
+    if (a.sc.px && a.sc.n[2 * s] > 3 && a.sc.n[2 * s + 1] > 3) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) delta[ub + q] = (real)0;
+        wave_sync();
+        for (int ty = 0; ty < 2; ++ty) {
+            const int64_t kb = sd.kbase[s] + (ty ? sd.kstride : 0);
+            const int cnt = a.sc.cnt[2 * gt + ty], pre = a.sc.pre[2 * gt + ty];
+            for (int i = lane; i < cnt; i += kWave) {
+                const int x = a.sc.px[kb + pre + i] - t0;
+                delta[x] = delta[x] + a.ga[kb + pre + i];
+            }
+            wave_sync();
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = v[q] - delta[ub + q] / (real)2;
+        dirty = true;
+    }
+    if (dirty) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (ub + q < nt) mu[t0 + ub + q] = v[q];
+    }
+    // ---- moments of mu over the intervals of pass mo
+    if (!(a.mo.px && a.mo.n[2 * s] > 3 && a.mo.n[2 * s + 1] > 3)) return;
+    for (int ty = 0; ty < 2; ++ty) {
+        const int64_t kb = sd.kbase[s] + (ty ? sd.kstride : 0);
+        const int cnt = a.mo.cnt[2 * gt + ty], pre = a.mo.pre[2 * gt + ty];
+        const int left = pre > 0 ? a.mo.px[kb + pre - 1] : 0;
+        wave_sync();
+        if (lane < T / 64) bits[lane] = 0ull;
+        wave_sync();
+        for (int i = lane; i < cnt; i += kWave) {
+            const int x = a.mo.px[kb + pre + i];
+            tp[i] = x;
+            atomicOr(&bits[(x - t0) >> 6], 1ull << ((x - t0) & 63));
+        }
+        wave_sync();
+        real* e = a.edge + 4 * (4 * (int64_t)gt + 2 * ty);
+        if (lane == 0) {
+            // empty edge runs (a knot on the tile's first / last sample)
+            if (cnt > 0 && (t0 == 0 || tp[0] == t0))
+                for (int m = 0; m < 4; ++m) e[m] = (real)0;
+            if (cnt > 0 && tp[cnt - 1] == t0 + nt - 1)
+                for (int m = 0; m < 4; ++m) e[4 + m] = (real)0;
+        }
+        // run number of sample u: knots of the tile strictly before it
+        // (sample 0 of the series joins run 1, the interval (x_0, x_1])
+        auto key_of = [&](int u) -> int {
+            if (u >= nt) return NOKEY;
+            if (t0 == 0 && u == 0) return 1;
+            int c = 0;
+#pragma unroll
+            for (int w = 0; w < T / 64; ++w) {
+                const unsigned long long b = bits[w];
+                const int lo = w * 64;
+                if (u >= lo + 64) c += __popcll(b);
+                else if (u > lo) c += __popcll(b << (64 - (u - lo)));
+            }
+            return c;
+        };
+        auto dst_of = [&](int r) -> real* {
+            if (r >= 1 && r < cnt) return a.mom + 4 * (kb + pre + r - 1);
+            return e + (r == 0 ? 0 : 4);
+        };
+        int key[4];
+        real m[4][4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int u = ub + q;
+            key[q] = key_of(u);
+            real tt = (real)0;
+            if (key[q] != NOKEY) {
+                const int lp = (t0 == 0 && u == 0) ? 0 : (key[q] >= 1 ? tp[key[q] - 1] : left);
+                tt = (real)(t0 + u - lp);
+            }
+            const real g = key[q] != NOKEY ? v[q] : (real)0;
+            m[q][0] = g;
+            m[q][1] = g * tt;
+            m[q][2] = m[q][1] * tt;
+            m[q][3] = m[q][2] * tt;
+        }
+        // runs inside the lane: the first (maybe continued from the left),
+        // complete middle runs, the last (maybe continued to the right)
+        const int kfirst = key[0];
+        real first[4] = {m[0][0], m[0][1], m[0][2], m[0][3]};
+        bool closed = false;
+        int kcur = kfirst;
+        real cur[4] = {m[0][0], m[0][1], m[0][2], m[0][3]};
+#pragma unroll
+        for (int q = 1; q < 4; ++q) {
+            if (key[q] == kcur) {
+#pragma unroll
+                for (int z = 0; z < 4; ++z) cur[z] = cur[z] + m[q][z];
+            } else {
+                if (!closed) {
+#pragma unroll
+                    for (int z = 0; z < 4; ++z) first[z] = cur[z];
+                    closed = true;
+                } else if (kcur != NOKEY) {
+                    real* d = dst_of(kcur);
+#pragma unroll
+                    for (int z = 0; z < 4; ++z) d[z] = cur[z];
+                }
+                kcur = key[q];
+#pragma unroll
+                for (int z = 0; z < 4; ++z) cur[z] = m[q][z];
+            }
+        }
+        // segmented inclusive scan of the last runs over the lanes
+        real sc[4] = {cur[0], cur[1], cur[2], cur[3]};
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int ku = shfl_up(kcur, o);
+            const bool take = lane >= o && ku == kcur;
+#pragma unroll
+            for (int z = 0; z < 4; ++z) {
+                const real w = shfl_up(sc[z], o);
+                if (take) sc[z] = sc[z] + w;
+            }
+        }
+        // shuffles run on every lane (a disabled source lane gives no defined value)
+        const int kup = shfl_up(kcur, 1);
+        const int kprev = lane > 0 ? kup : NOKEY + 1;
+        real prevs[4];
+#pragma unroll
+        for (int z = 0; z < 4; ++z) prevs[z] = shfl_up(sc[z], 1);
+        const int knext_first = shfl_down(kfirst, 1);
+        if (closed && kfirst != NOKEY) {
+            // the first run ends inside this lane
+            real tot[4];
+#pragma unroll
+            for (int z = 0; z < 4; ++z)
+                tot[z] = (kprev == kfirst) ? prevs[z] + first[z] : first[z];
+            real* d = dst_of(kfirst);
+#pragma unroll
+            for (int z = 0; z < 4; ++z) d[z] = tot[z];
+        }
+        // the last run ends at this lane's end unless the next lane continues it
+        const bool ends = (lane == kWave - 1) || (knext_first != kcur);
+        if (ends && kcur != NOKEY) {
+            real* d = dst_of(kcur);
+#pragma unroll
+            for (int z = 0; z < 4; ++z) d[z] = sc[z];
+        }
+    }
+}
+
 // 2. intervals that leave their tile: the last run of the tile holding the
 // left knot plus the first runs of the following tiles up to the right knot
 template <typename real>
@@ -2158,6 +2348,26 @@ hipError_t launch_adjoint_pass(const AdjArgs<real>& a, int solve_grid, hipStream
 }
 
 template <typename real>
+hipError_t launch_adj_tile(const AdjTileArgs<real>& a, hipStream_t st)
+{
+    if (2 * (a.sd.cap - 2) != kWT) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_adj_tile<real>), dim3(blocks_for(a.sd.ntiles, kThreads / kWave)),
+                       dim3(kThreads), 0, st, a);
+    return hipGetLastError();
+}
+
+template <typename real>
+hipError_t launch_adj_span_knot(const AdjArgs<real>& a, int solve_grid, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_adj_span<real>), dim3(blocks_for(a.sd.ntiles, 256)), dim3(256), 0, st, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_adj_knot<real, Geom<real>::B, Geom<real>::H>), dim3(solve_grid),
+                       dim3(kThreads), 0, st, a);
+    return hipGetLastError();
+}
+
+template <typename real>
 hipError_t launch_adj_seed(const real* gimf, int M, int64_t N, int64_t stride, real* gr,
                            hipStream_t st)
 {
@@ -2244,6 +2454,8 @@ hipError_t launch_check_index(const int* d_x, int64_t N, int* err, hipStream_t s
 
 #define ICM_INSTANTIATE_ADJ(real)                                                                  \
     template hipError_t launch_adjoint_pass<real>(const AdjArgs<real>&, int, hipStream_t);         \
+    template hipError_t launch_adj_tile<real>(const AdjTileArgs<real>&, hipStream_t);              \
+    template hipError_t launch_adj_span_knot<real>(const AdjArgs<real>&, int, hipStream_t);        \
     template hipError_t launch_adj_seed<real>(const real*, int, int64_t, int64_t, real*,           \
                                               hipStream_t);                                        \
     template hipError_t launch_adj_broadcast<real>(const SeriesDev&, const real*, const int64_t*,  \

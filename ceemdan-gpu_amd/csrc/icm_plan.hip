@@ -898,15 +898,21 @@ struct Plan : PlanBase {
             if ((rc = stage_fixed(ICM_PHASE_SIGNAL, k, dsig, ia, S, nullptr, st, rec.data())))
                 return rc;
             const real* lam = gr + (int64_t)(k + 1) * chan_total;
-            ICM_TRY(launch_adj_broadcast<real>(sd, lam, d_chan_off, mu, dsig.max_len, st));
-            for (int j = S - 1; j >= 0; --j) {
+            auto rec_of = [&](int j) {
+                PassRec r{};
+                if (j >= 0 && j < S) {
+                    r.px = rec[j].px;
+                    r.cnt = rec[j].cnt;
+                    r.pre = rec[j].pre;
+                    r.n = rec[j].n;
+                    r.fixf = rec[j].fixf;
+                }
+                return r;
+            };
+            auto adj_args = [&](int j) {
                 AdjArgs<real> a{};
                 a.sd = sd;
-                a.ps.px = rec[j].px;
-                a.ps.cnt = rec[j].cnt;
-                a.ps.pre = rec[j].pre;
-                a.ps.n = rec[j].n;
-                a.ps.fixf = rec[j].fixf;
+                a.ps = rec_of(j);
                 a.mu = mu;
                 a.mom = mom;
                 a.edge = edge;
@@ -914,7 +920,28 @@ struct Plan : PlanBase {
                 a.winlist = rec[j].winlist;
                 a.wincount = rec[j].wincount;
                 a.err = err;
-                ICM_TRY(launch_adjoint_pass<real>(a, solve_grid(), st));
+                return a;
+            };
+            if (2 * (cap - 2) == 256) {
+                // fused per-tile steps: (broadcast | scatter of pass j+1) + moments of pass j
+                for (int j = S - 1; j >= -1; --j) {
+                    AdjTileArgs<real> ta{};
+                    ta.sd = sd;
+                    ta.mu = mu;
+                    ta.bcast = j == S - 1 ? lam : nullptr;
+                    ta.chan_off = d_chan_off;
+                    ta.sc = rec_of(j + 1);
+                    ta.ga = ga;
+                    ta.mo = rec_of(j);
+                    ta.mom = mom;
+                    ta.edge = edge;
+                    ICM_TRY(launch_adj_tile<real>(ta, st));
+                    if (j >= 0) ICM_TRY(launch_adj_span_knot<real>(adj_args(j), solve_grid(), st));
+                }
+            } else {
+                ICM_TRY(launch_adj_broadcast<real>(sd, lam, d_chan_off, mu, dsig.max_len, st));
+                for (int j = S - 1; j >= 0; --j)
+                    ICM_TRY(launch_adjoint_pass<real>(adj_args(j), solve_grid(), st));
             }
             ICM_TRY(launch_adj_stage<real>(k, C, I, max_chan_len, d_chan_len, d_chan_off, sd, lam,
                                            mu, Ek, rk, eps0, gr + (int64_t)k * chan_total, gacc,
