@@ -1092,7 +1092,7 @@ int icm_plan_get_info(icm_plan plan, icm_plan_info* info)
     info->signal_samples = P.hsig.samples;
     info->tiles_noise = P.hn.ntiles;
     info->tiles_signal = P.hsig.ntiles;
-    info->tile = tile_of(P.cfg.dtype);
+    info->tile = 2 * (P.cap - 2);
     info->solve_rows_per_lane = solve_B(P.cfg.dtype);
     info->solve_halo = solve_H(P.cfg.dtype);
     return ICM_OK;
