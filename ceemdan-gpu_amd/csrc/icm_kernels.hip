@@ -157,15 +157,12 @@ struct LastTwo {
 
 __device__ __forceinline__ LastTwo lt_combine(const LastTwo& A, const LastTwo& B)
 {
-    if (B.c >= 2) return B;
-    if (B.c == 1) {
-        LastTwo r;
-        r.c = min(A.c + 1, 2);
-        r.l1 = B.l1;
-        r.l2 = A.l1;
-        return r;
-    }
-    return A;
+    // written as selects (no early returns) so the result stays in registers
+    LastTwo r;
+    r.c = B.c >= 2 ? 2 : (B.c == 1 ? min(A.c + 1, 2) : A.c);
+    r.l1 = B.c >= 1 ? B.l1 : A.l1;
+    r.l2 = B.c >= 2 ? B.l2 : (B.c == 1 ? A.l1 : A.l2);
+    return r;
 }
 
 __device__ __forceinline__ LastTwo lt_shfl_up(const LastTwo& v, int o)
@@ -272,6 +269,152 @@ __global__ void k_knot_prefix(SeriesDev sd, KnotsDev<real> k, PassCtl pc, const 
     }
     wbase = __shfl(wbase, 0, kWave);
     for (int w = lane; w < wmx + wmn; w += kWave) {
+        const int type = w >= wmx;
+        const int wi = type ? w - wmx : w;
+        pc.winlist[2 * (wbase + w)] = s;
+        pc.winlist[2 * (wbase + w) + 1] = (wi << 1) | type;
+    }
+}
+
+// The same outputs with one 1024-thread workgroup per series and one tile per
+// thread (chunks of 1024 tiles): workgroup scans forward (counts, last two
+// knots) and backward (next knot), carried from chunk to chunk.
+constexpr int kPT = 1024;
+
+__device__ __forceinline__ LastTwo lt_none()
+{
+    LastTwo r;
+    r.c = 0;
+    r.l1 = -1;
+    r.l2 = -1;
+    return r;
+}
+
+template <typename real>
+__global__ __launch_bounds__(kPT) void k_knot_prefix_b(SeriesDev sd, KnotsDev<real> k, PassCtl pc,
+                                                       const int* active)
+{
+    constexpr int NWV = kPT / kWave;
+    constexpr int INF = 0x7FFFFFFF;
+    __shared__ int s_tot[NWV], s_ex_tot[NWV], s_all;
+    __shared__ LastTwo s_lt[NWV], s_ex_lt[NWV], s_all_lt;
+    __shared__ int s_min[NWV];
+    __shared__ unsigned s_wbase;
+    const int s = blockIdx.x;
+    if (active && !active[s]) return;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int tb = sd.tile0[s], nt = sd.ntl[s];
+    const int cap = sd.cap;
+    int ntot[2] = {0, 0};
+    for (int ty = 0; ty < 2; ++ty) {
+        const int64_t so = ty ? sd.sstride : 0;
+        // forward: exclusive count and the last two knots before each tile
+        int carry = 0;
+        LastTwo carry_lt = lt_none();
+        for (int base = 0; base < nt; base += kPT) {
+            const int t = base + tid;
+            const int c = t < nt ? k.cnt[2 * (tb + t) + ty] : 0;
+            LastTwo own;
+            own.c = min(c, 2);
+            own.l1 = c >= 1 ? (int64_t)(tb + t) * cap + (c - 1) : -1;
+            own.l2 = c >= 2 ? (int64_t)(tb + t) * cap + (c - 2) : -1;
+            int inc = c;
+            LastTwo sc = own;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const int v = shfl_up(inc, o);
+                const LastTwo w = lt_shfl_up(sc, o);
+                if (lane >= o) {
+                    inc += v;
+                    sc = lt_combine(w, sc);
+                }
+            }
+            const int xi = shfl_up(inc, 1);
+            const LastTwo xl = lt_shfl_up(sc, 1);
+            __syncthreads();   // earlier readers of s_* are done
+            if (lane == kWave - 1) {
+                s_tot[wv] = inc;
+                s_lt[wv] = sc;
+            }
+            __syncthreads();
+            if (wv == 0) {
+                // exclusive scan of the wave totals by the first NWV lanes
+                int v = lane < NWV ? s_tot[lane] : 0;
+                LastTwo l = lane < NWV ? s_lt[lane] : lt_none();
+                int vi = v;
+                LastTwo li = l;
+                for (int o = 1; o < NWV; o <<= 1) {
+                    const int a1 = shfl_up(vi, o);
+                    const LastTwo b1 = lt_shfl_up(li, o);
+                    if (lane >= o) {
+                        vi += a1;
+                        li = lt_combine(b1, li);
+                    }
+                }
+                const LastTwo le = lt_shfl_up(li, 1);
+                if (lane < NWV) {
+                    s_ex_tot[lane] = vi - v;
+                    s_ex_lt[lane] = lane > 0 ? le : lt_none();
+                }
+                if (lane == NWV - 1) {
+                    s_all = vi;
+                    s_all_lt = li;
+                }
+            }
+            __syncthreads();
+            const int ex = carry + s_ex_tot[wv] + (lane > 0 ? xi : 0);
+            const LastTwo before =
+                lt_combine(carry_lt, lt_combine(s_ex_lt[wv], lane > 0 ? xl : lt_none()));
+            if (t < nt) {
+                k.pre[2 * (tb + t) + ty] = ex;
+                const int64_t o = 6 * (int64_t)(tb + t) + 3 * ty;
+                k.nbx[o] = before.l1 >= 0 ? k.x[so + before.l1] : -1;
+                k.nby[o] = before.l1 >= 0 ? k.y[so + before.l1] : (real)0;
+                k.nbx[o + 1] = before.l2 >= 0 ? k.x[so + before.l2] : -1;
+                k.nby[o + 1] = before.l2 >= 0 ? k.y[so + before.l2] : (real)0;
+            }
+            carry += s_all;
+            carry_lt = lt_combine(carry_lt, s_all_lt);
+        }
+        ntot[ty] = carry;
+        // backward: first non-empty tile after each tile
+        int carry_min = INF;
+        for (int base = ((nt - 1) / kPT) * kPT; base >= 0; base -= kPT) {
+            const int t = base + tid;
+            const int c = t < nt ? k.cnt[2 * (tb + t) + ty] : 0;
+            int mn = c > 0 ? t : INF;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const int m = shfl_down(mn, o);
+                if (lane + o < kWave) mn = min(mn, m);
+            }
+            const int xm = shfl_down(mn, 1);
+            __syncthreads();
+            if (lane == 0) s_min[wv] = mn;
+            __syncthreads();
+            int after = min(carry_min, lane < kWave - 1 ? xm : INF);
+            int all = carry_min;
+            for (int w = 0; w < NWV; ++w) {
+                if (w > wv) after = min(after, s_min[w]);
+                all = min(all, s_min[w]);
+            }
+            if (t < nt) {
+                const int64_t o = 6 * (int64_t)(tb + t) + 3 * ty + 2;
+                const int64_t sl = so + (int64_t)(tb + after) * cap;
+                k.nbx[o] = after == INF ? -1 : k.x[sl];
+                k.nby[o] = after == INF ? (real)0 : k.y[sl];
+            }
+            carry_min = all;
+        }
+    }
+    const int wmx = (ntot[0] + pc.K - 1) / pc.K;
+    const int wmn = (ntot[1] + pc.K - 1) / pc.K;
+    if (tid == 0) {
+        k.n[2 * s] = ntot[0];
+        k.n[2 * s + 1] = ntot[1];
+        s_wbase = atomicAdd(&pc.wincount[pc.epoch], (unsigned)(wmx + wmn));
+    }
+    __syncthreads();
+    const unsigned wbase = s_wbase;
+    for (int w = tid; w < wmx + wmn; w += kPT) {
         const int type = w >= wmx;
         const int wi = type ? w - wmx : w;
         pc.winlist[2 * (wbase + w)] = s;
@@ -1335,6 +1478,82 @@ __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs
     }
 }
 
+// Stage start for 256-sample tiles, one wavefront per tile: forms x_i
+// (the same arithmetic as init_value) and writes its knots to the tile slots.
+template <typename real>
+__global__ __launch_bounds__(kThreads) void k_init_w(SeriesDev sd, InitArgs<real> ia, real* h_out,
+                                                     KnotsDev<real> kout)
+{
+    using P = typename Promote<real>::type;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int gt = blockIdx.x * (kThreads / kWave) + wv;
+    if (gt >= sd.ntiles) return;
+    const int s = sd.tile_series[gt];
+    const int t0 = (gt - sd.tile0[s]) * kWT;
+    const int N = sd.len[s];
+    const int nt = min(kWT, N - t0);
+    real* out = h_out + sd.off[s];
+    const real* src = nullptr;
+    const real* base = nullptr;
+    const real* E = nullptr;
+    P scl0 = (P)0;
+    real sclk = (real)0;
+    if (ia.mode == INIT_COPY) {
+        src = ia.src + sd.off[s];
+    } else {
+        const int c = sd.chan[s];
+        base = ia.chan_sig + ia.chan_off[c];
+        E = ia.E + sd.eoff[s];
+        if (ia.mode == INIT_STAGE0) scl0 = (P)ia.scale[sd.nidx[s]];
+        else sclk = (real)ia.scale[c];
+    }
+    auto val = [&](int p) -> real {
+        if (ia.mode == INIT_COPY) return src[p];
+        const real e = E[p];
+        if (ia.mode == INIT_STAGE0) return (real)((P)base[p] + (P)(ia.eps0 * e) * scl0);
+        return base[p] + (ia.eps0 * e) * sclk;
+    };
+    real x[kWR];
+#pragma unroll
+    for (int r = 0; r < kWR; ++r) {
+        const int u = r * kWave + lane;
+        x[r] = (real)0;
+        if (u < nt) {
+            x[r] = val(t0 + u);
+            out[t0 + u] = x[r];
+        }
+    }
+    const real xl = t0 >= 1 ? val(t0 - 1) : (real)0;
+    const real xr = t0 + kWT < N ? val(t0 + kWT) : (real)0;
+    const int64_t slot0 = (int64_t)gt * sd.cap;
+    int run[2] = {0, 0};
+#pragma unroll
+    for (int r = 0; r < kWR; ++r) {
+        const int u = r * kWave + lane;
+        const int p = t0 + u;
+        const real lft = shift_prev(x[r], r == 0 ? xl : lane_value(x[r > 0 ? r - 1 : 0], kWave - 1));
+        const real rgt = shift_next(x[r], r == kWR - 1 ? xr : lane_value(x[r < kWR - 1 ? r + 1 : r], 0));
+        const bool valid = u < nt;
+        const bool end = p == 0 || p == N - 1;
+        const real v = x[r];
+        const uint64_t fm[2] = {wave_ballot(valid && (end || ((v > lft) && (v > rgt)))),
+                                wave_ballot(valid && (end || ((v < lft) && (v < rgt))))};
+#pragma unroll
+        for (int ty = 0; ty < 2; ++ty) {
+            if ((fm[ty] >> lane) & 1ull) {
+                const int64_t o = (ty ? sd.sstride : 0) + slot0 + run[ty] + lanes_below(fm[ty]);
+                kout.x[o] = p;
+                kout.y[o] = v;
+            }
+            run[ty] += __popcll(fm[ty]);
+        }
+    }
+    if (lane == 0) {
+        kout.cnt[2 * gt] = run[0];
+        kout.cnt[2 * gt + 1] = run[1];
+    }
+}
+
 // ================================================================ adjoint
 // With the extrema sets of every pass frozen (recorded by the forward), one
 // sift pass is linear: h' = h - (U + L) / 2.  Its transpose maps the
@@ -2190,8 +2409,8 @@ hipError_t launch_init(const SeriesDev& sd, const InitArgs<real>& ia, real* h_ou
 {
     const int T = 2 * (sd.cap - 2);
     if (T == 256)
-        hipLaunchKernelGGL((k_init<real, 256>), dim3(sd.ntiles), dim3(kThreads), 0, st, sd, ia,
-                           h_out, kout);
+        hipLaunchKernelGGL((k_init_w<real>), dim3(blocks_for(sd.ntiles, kThreads / kWave)),
+                           dim3(kThreads), 0, st, sd, ia, h_out, kout);
     else if (T == 512)
         hipLaunchKernelGGL((k_init<real, 512>), dim3(sd.ntiles), dim3(kThreads), 0, st, sd, ia,
                            h_out, kout);
@@ -2207,7 +2426,7 @@ template <typename real>
 hipError_t launch_knot_prefix(const SeriesDev& sd, const KnotsDev<real>& k, const PassCtl& pc,
                               const int* active, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_knot_prefix<real>), dim3(sd.nseries), dim3(kWave), 0, st, sd, k, pc,
+    hipLaunchKernelGGL((k_knot_prefix_b<real>), dim3(sd.nseries), dim3(kPT), 0, st, sd, k, pc,
                        active);
     return hipGetLastError();
 }
