@@ -2399,7 +2399,7 @@ __global__ void k_check_index(const int* x, int64_t N, int* err)
 static inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 template <typename real> struct Geom;
-template <> struct Geom<double> { static constexpr int B = 12, H = 64; };
+template <> struct Geom<double> { static constexpr int B = 8, H = 32; };
 template <> struct Geom<float> { static constexpr int B = 8, H = 32; };
 
 // tile size (samples per tile) is a plan parameter: sd.cap = T/2 + 2

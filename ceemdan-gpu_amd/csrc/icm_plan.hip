@@ -53,8 +53,11 @@ int tile_of(int)
     const int t = env ? std::atoi(env) : 256;
     return (t == 512 || t == 1024) ? t : 256;
 }
-int solve_B(int dtype) { return dtype == ICM_DTYPE_F64 ? 12 : 8; }
-int solve_H(int dtype) { return dtype == ICM_DTYPE_F64 ? 64 : 32; }
+// windowed solve geometry (both precisions): 64 lanes x 8 rows, 32 halo rows
+// on each side; the dropped coupling decays by >= 3.7x per row, so 32 rows
+// are below FP64 round-off
+int solve_B(int) { return 8; }
+int solve_H(int) { return 32; }
 int solve_K(int dtype) { return 64 * solve_B(dtype) - 2 * solve_H(dtype); }
 
 int validate(const icm_config* c)

@@ -15,7 +15,7 @@ STOP_FIXED = 0
 STOP_RILLING = 1
 
 # default windowed-solve geometry of the FP64 path: 64 lanes x B rows, H halo
-GEOM_F64 = (12, 64)
+GEOM_F64 = (8, 32)
 GEOM_F32 = (8, 32)
 
 _lib = None

@@ -39,18 +39,18 @@ def run_case(orc, name):
         x = _eeg(2000)
         w = orc.noise(7, 6, x.size)
         r = orc.iceemdan(x, w, 5, S_or_cap=10)
-        return {"imfs": r["imfs"]}
+        return {"noise": w, "imfs": r["imfs"]}
     if name == "rilling_two_tone":
         x = _two_tone()
         w = orc.noise(1234, 8, x.size)
         r = orc.iceemdan(x, w, 3, stop_mode=orc.STOP_RILLING, S_or_cap=100, noise_cap=2000)
-        return {"imfs": r["imfs"], "stops": r["stops"]}
+        return {"x": x, "noise": w, "imfs": r["imfs"], "stops": r["stops"]}
     if name == "grad_eeg":
         x = _eeg(800)
         w = orc.noise(3, 4, x.size)
         G = np.cos(np.arange(4 * x.size).reshape(4, x.size) / 17.0)
         gy, _ = orc.iceemdan_grad(x, w, 4, G, S=6)
-        return {"grad": gy}
+        return {"noise": w, "grad": gy}
     raise KeyError(name)
 
 
