@@ -885,13 +885,17 @@ __device__ __forceinline__ void wave_sync()
 }
 
 // Copy knots [glo, ghi) of (series s, type) from the tile slots into LDS
-// (sp/sy index = g - lo).  One wavefront; tpre is 64 ints of LDS scratch.
+// (sp/sy index = g - lo).  One wavefront; tpre and tcnt are 64 ints of LDS
+// scratch each.  Tiles are taken 64 at a time; where they hold many knots
+// each (dense early modes) the wave copies one tile's contiguous slots at a
+// time, otherwise every lane finds the tile of its own knot by binary search.
 template <typename real>
 __device__ void gather_knots(const SeriesDev& sd, const KnotsDev<real>& k, int s, int type,
-                             long lo, int glo, int ghi, int* sp, real* sy, int* tpre)
+                             long lo, int glo, int ghi, int* sp, real* sy, int* tpre, int* tcnt)
 {
     const int lane = threadIdx.x & 63;
     const int tb = sd.tile0[s], ntl = sd.ntl[s];
+    const int64_t so = type ? sd.sstride : 0;
     // 64-ary search for the tile holding knot glo
     int a = 0, b = ntl;
     while (b - a > 1) {
@@ -911,23 +915,40 @@ __device__ void gather_knots(const SeriesDev& sd, const KnotsDev<real>& k, int s
         const int p = valid ? k.pre[2 * (tb + tl) + type] : 0x7FFFFFFF;
         const int c = valid ? k.cnt[2 * (tb + tl) + type] : 0;
         tpre[lane] = p;
+        tcnt[lane] = c;
         const uint64_t vm = wave_ballot(valid);
         if (vm == 0ull) break;   // counts inconsistent with n: leave the rest
         const int last = 63 - __builtin_clzll(vm);
         const int pend = __shfl(p + c, last, kWave);
-        wave_sync();
+        // tiles of this group that hold knots of [g, gend)
         const int gend = min(ghi, pend);
-        for (int gi = g + lane; gi < gend; gi += kWave) {
-            int x = 0, y = last;
-            while (x < y) {
-                const int mid = (x + y + 1) >> 1;
-                if (tpre[mid] <= gi) x = mid;
-                else y = mid - 1;
+        const uint64_t used = wave_ballot(valid && p < gend && p + c > g);
+        wave_sync();
+        if ((gend - g) >= 16 * __popcll(used)) {
+            uint64_t m = used;
+            while (m) {
+                const int j = __builtin_ctzll(m);
+                m &= m - 1;
+                const int pj = tpre[j], cj = tcnt[j];
+                const int k0 = max(pj, g), k1 = min(pj + cj, gend);
+                const int64_t base = so + (int64_t)(tb + tile + j) * sd.cap - pj;
+                for (int gi = k0 + lane; gi < k1; gi += kWave) {
+                    sp[gi - lo] = k.x[base + gi];
+                    sy[gi - lo] = k.y[base + gi];
+                }
             }
-            const int64_t slot = (type ? sd.sstride : 0) + (int64_t)(tb + tile + x) * sd.cap +
-                                 (gi - tpre[x]);
-            sp[gi - lo] = k.x[slot];
-            sy[gi - lo] = k.y[slot];
+        } else {
+            for (int gi = g + lane; gi < gend; gi += kWave) {
+                int x = 0, y = last;
+                while (x < y) {
+                    const int mid = (x + y + 1) >> 1;
+                    if (tpre[mid] <= gi) x = mid;
+                    else y = mid - 1;
+                }
+                const int64_t slot = so + (int64_t)(tb + tile + x) * sd.cap + (gi - tpre[x]);
+                sp[gi - lo] = k.x[slot];
+                sy[gi - lo] = k.y[slot];
+            }
         }
         g = gend;
         tile += kWave;
@@ -1024,7 +1045,7 @@ __global__ __launch_bounds__(kThreads) void k_solve(SolveArgs<real> a)
     constexpr int NW = kThreads / kWave;
     __shared__ int s_pos[NW][W + 2];
     __shared__ real s_y[NW][W + 2];
-    __shared__ int s_tpre[NW][kWave];
+    __shared__ int s_tpre[NW][kWave], s_tcnt[NW][kWave];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     int* sp = s_pos[wv];
     real* sy = s_y[wv];
@@ -1041,7 +1062,7 @@ __global__ __launch_bounds__(kThreads) void k_solve(SolveArgs<real> a)
         const long lo = g0 - 1;
         const int glo = (int)max(lo, 0L);
         const int ghi = (int)min(lo + W + 2, (long)n);
-        gather_knots<real>(a.sd, a.k, s, type, lo, glo, ghi, sp, sy, s_tpre[wv]);
+        gather_knots<real>(a.sd, a.k, s, type, lo, glo, ghi, sp, sy, s_tpre[wv], s_tcnt[wv]);
         wave_sync();
         // endpoint fix-up when n >= 6 (setBoundaryMax/Min, cudaICEEMDAN.cu:209-291);
         // idempotent, so every window that holds an end applies it
