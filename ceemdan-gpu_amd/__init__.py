@@ -81,6 +81,12 @@ _SIGS = {
                                 C.c_void_p, C.c_void_p]),
     "icm_backward": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                C.c_void_p]),
+    "icm_forward_begin": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p]),
+    "icm_forward_stages": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.c_void_p]),
+    "icm_forward_finish": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_void_p]),
+    "icm_state_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "icm_state_save": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+    "icm_state_load": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
     "icm_stats_mean": (C.c_int, [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "icm_stats_variance": (C.c_int, [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
                                      C.c_void_p, C.c_void_p]),
@@ -272,6 +278,70 @@ class Plan:
                "icm_forward")
         self.forward_count = getattr(self, "forward_count", 0) + 1
         return out
+
+    # ---- staged forward with stage-boundary checkpoints
+    def _signals(self, signals):
+        torch = _torch()
+        if isinstance(signals, torch.Tensor):
+            signals = [signals] if signals.dim() == 1 else list(signals)
+        if len(signals) != len(self.lengths):
+            raise ValueError("number of channels differs from the plan")
+        for s, n in zip(signals, self.lengths):
+            if s.numel() != n or s.dtype != self.dtype:
+                raise ValueError("channel length/dtype differs from the plan")
+            _require_cuda(s)
+        return signals
+
+    def forward_begin(self, signals, noise=None):
+        """Inputs, noise modes and stage-0 statistics; returns stage 0."""
+        signals = self._signals(signals)
+        if noise is not None:
+            _require_cuda(noise)
+        sp = (C.c_void_p * len(signals))(*[s.data_ptr() for s in signals])
+        _check(lib().icm_forward_begin(self._h, sp,
+                                       C.c_void_p(noise.data_ptr()) if noise is not None else None,
+                                       _stream_ptr(self.device)), "icm_forward_begin")
+        return 0
+
+    def forward_stages(self, count):
+        """Runs up to `count` CEEMDAN stages; returns the next stage."""
+        st = C.c_int()
+        _check(lib().icm_forward_stages(self._h, int(count), C.byref(st), _stream_ptr(self.device)),
+               "icm_forward_stages")
+        return st.value
+
+    def forward_finish(self, out=None):
+        torch = _torch()
+        if out is None:
+            out = [torch.empty((self.num_imfs, n), dtype=self.dtype, device=self.device)
+                   for n in self.lengths]
+        for o in out:
+            _require_cuda(o)
+        op = (C.c_void_p * len(out))(*[o.data_ptr() for o in out])
+        _check(lib().icm_forward_finish(self._h, op, _stream_ptr(self.device)), "icm_forward_finish")
+        self.forward_count = getattr(self, "forward_count", 0) + 1
+        return out
+
+    def save_state(self):
+        """Host copy (numpy uint8) of the state between two stages."""
+        import numpy as np
+        n = C.c_int64()
+        _check(lib().icm_state_bytes(self._h, C.byref(n)), "icm_state_bytes")
+        buf = np.empty(n.value, np.uint8)
+        _check(lib().icm_state_save(self._h, C.c_void_p(buf.ctypes.data), n.value,
+                                    _stream_ptr(self.device)), "icm_state_save")
+        return buf
+
+    def load_state(self, buf, noise=None):
+        """Resumes a saved state; returns its stage."""
+        import numpy as np
+        buf = np.ascontiguousarray(buf, dtype=np.uint8)
+        if noise is not None:
+            _require_cuda(noise)
+        _check(lib().icm_state_load(self._h, C.c_void_p(buf.ctypes.data), buf.size,
+                                    C.c_void_p(noise.data_ptr()) if noise is not None else None,
+                                    _stream_ptr(self.device)), "icm_state_load")
+        return self.forward_stages(0)
 
     def backward(self, grad_imfs, out=None):
         """dL/dx of the last forward from dL/dIMF (list of [num_imfs, N_c]

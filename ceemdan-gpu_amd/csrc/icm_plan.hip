@@ -270,6 +270,12 @@ struct PlanBase {
                         hipStream_t st) = 0;
     virtual int noise_modes_out(void* d_out, int64_t count, hipStream_t st) = 0;
     virtual int backward(const void* const* gimf, void* const* gsig, hipStream_t st) = 0;
+    virtual int forward_begin(const void* const* sig, const void* noise, hipStream_t st) = 0;
+    virtual int forward_stages(int count, int* stage_out, hipStream_t st) = 0;
+    virtual int forward_finish(void* const* out, hipStream_t st) = 0;
+    virtual int64_t state_bytes() const = 0;
+    virtual int state_save(void* host, int64_t bytes, hipStream_t st) = 0;
+    virtual int state_load(const void* host, int64_t bytes, const void* noise, hipStream_t st) = 0;
 
     template <typename T>
     int alloc(T** p, int64_t n)
@@ -383,6 +389,12 @@ int64_t plan_bytes_estimate(const icm_config* c)
         int forward(const void* const*, void* const*, const void*, hipStream_t) override { return 0; }
         int noise_modes_out(void*, int64_t, hipStream_t) override { return 0; }
         int backward(const void* const*, void* const*, hipStream_t) override { return 0; }
+        int forward_begin(const void* const*, const void*, hipStream_t) override { return 0; }
+        int forward_stages(int, int*, hipStream_t) override { return 0; }
+        int forward_finish(void* const*, hipStream_t) override { return 0; }
+        int64_t state_bytes() const override { return 0; }
+        int state_save(void*, int64_t, hipStream_t) override { return 0; }
+        int state_load(const void*, int64_t, const void*, hipStream_t) override { return 0; }
     } P;
     P.cfg = *c;
     layout(P);
@@ -418,7 +430,8 @@ struct Plan : PlanBase {
     real *y = nullptr, *xt = nullptr, *r = nullptr, *imf = nullptr;
     real *var_chan = nullptr, *var_noise = nullptr;
     double *inv_noise = nullptr, *sq_chan = nullptr;
-    real* r_hist = nullptr;     // [M-1][chan_total]: rows 1 .. M-2 = r_1 .. r_{M-2} of the last forward
+    real* r_hist = nullptr;
+    int stage_ = -1;            // next CEEMDAN stage of a staged forward (-1: none in progress)     // [M-1][chan_total]: rows 1 .. M-2 = r_1 .. r_{M-2} of the last forward
     bool have_forward = false;
     // adjoint state (allocated by the first backward)
     struct Rec {
@@ -692,58 +705,77 @@ struct Plan : PlanBase {
     }
 
     // CEEMDAN stages (cudaICEEMDAN.cu:1003-1298)
-    int signal_phase(hipStream_t st)
+    // stage-0 statistics: var(x) per channel, var(E_1(w^i)) per noise series
+    int signal_begin(hipStream_t st)
     {
-        const int M = cfg.num_imfs, I = cfg.num_noise, C = cfg.num_channels;
-        const real eps0 = (real)cfg.noise_strength;
-        int rc;
-        // stage-0 statistics: var(x) per channel, var(E_1(w^i)) per noise series
+        const int C = cfg.num_channels;
         ICM_TRY(launch_series_var<real>(y, d_chan_off, 0, d_chan_len, C, var_chan, st));
         for (int c = 0; c < C; ++c)
             ICM_TRY(launch_standardize<real>(y + chan_off[c], var_chan + c, xt + chan_off[c],
                                              lens[c], st));
         ICM_TRY(launch_series_var<real>(modes, dn.off, 0, dn.len, dn.nseries, var_noise, st));
         ICM_TRY(launch_inv_sqrt<real>(var_noise, inv_noise, dn.nseries, 0, st));
-        for (int k = 0; k < M - 1; ++k) {
-            const real* Ek = modes + (int64_t)k * hn.samples;
-            InitArgs<real> ia{};
-            ia.E = Ek;
-            ia.eps0 = eps0;
-            ia.chan_off = d_chan_off;
-            if (k == 0) {
-                ia.mode = INIT_STAGE0;
-                ia.chan_sig = xt;
-                ia.scale = inv_noise;
-            } else {
-                ICM_TRY(launch_series_var<real>(r, d_chan_off, 0, d_chan_len, C, var_chan, st));
-                ICM_TRY(launch_inv_sqrt<real>(var_chan, sq_chan, C, 1, st));
-                ia.mode = INIT_STAGEK;
-                ia.chan_sig = r;
-                ia.scale = sq_chan;
-            }
-            const int* parity = nullptr;
-            int fixed_parity = 0;
-            if (cfg.stop_mode == ICM_STOP_FIXED) {
-                if ((rc = stage_fixed(ICM_PHASE_SIGNAL, k, dsig, ia, cfg.max_sifts, nullptr, st)))
-                    return rc;
-                fixed_parity = cfg.max_sifts & 1;
-            } else {
-                if ((rc = stage_rilling(ICM_PHASE_SIGNAL, k, dsig, ia, cfg.max_sifts, st))) return rc;
-                parity = stop_pass;
-            }
-            ICM_TRY(launch_ensemble<real>(k, C, I, max_chan_len, d_chan_len, d_chan_off, dsig.off,
-                                          dsig.eoff, dsig.nidx, hA, hB, parity, fixed_parity, Ek,
-                                          inv_noise, sq_chan, eps0, xt, r, imf, d_chan_imf_off,
-                                          st));
-            // residues kept for the adjoint (r_0 = x~ is xt)
-            if (k + 1 <= M - 2)
-                ICM_TRY(launch_copy<real>(r_hist + (int64_t)(k + 1) * chan_total, r, chan_total, st));
+        return ICM_OK;
+    }
+
+    // CEEMDAN stage k: r_k -> r_{k+1}, IMF_k (cudaICEEMDAN.cu:1003-1298)
+    int signal_stage(int k, hipStream_t st)
+    {
+        const int M = cfg.num_imfs, I = cfg.num_noise, C = cfg.num_channels;
+        const real eps0 = (real)cfg.noise_strength;
+        int rc;
+        const real* Ek = modes + (int64_t)k * hn.samples;
+        InitArgs<real> ia{};
+        ia.E = Ek;
+        ia.eps0 = eps0;
+        ia.chan_off = d_chan_off;
+        if (k == 0) {
+            ia.mode = INIT_STAGE0;
+            ia.chan_sig = xt;
+            ia.scale = inv_noise;
+        } else {
+            ICM_TRY(launch_series_var<real>(r, d_chan_off, 0, d_chan_len, C, var_chan, st));
+            ICM_TRY(launch_inv_sqrt<real>(var_chan, sq_chan, C, 1, st));
+            ia.mode = INIT_STAGEK;
+            ia.chan_sig = r;
+            ia.scale = sq_chan;
         }
-        // residue row M-1 (cudaICEEMDAN.cu:1298)
+        const int* parity = nullptr;
+        int fixed_parity = 0;
+        if (cfg.stop_mode == ICM_STOP_FIXED) {
+            if ((rc = stage_fixed(ICM_PHASE_SIGNAL, k, dsig, ia, cfg.max_sifts, nullptr, st)))
+                return rc;
+            fixed_parity = cfg.max_sifts & 1;
+        } else {
+            if ((rc = stage_rilling(ICM_PHASE_SIGNAL, k, dsig, ia, cfg.max_sifts, st))) return rc;
+            parity = stop_pass;
+        }
+        ICM_TRY(launch_ensemble<real>(k, C, I, max_chan_len, d_chan_len, d_chan_off, dsig.off,
+                                      dsig.eoff, dsig.nidx, hA, hB, parity, fixed_parity, Ek,
+                                      inv_noise, sq_chan, eps0, xt, r, imf, d_chan_imf_off, st));
+        // residues kept for the adjoint (r_0 = x~ is xt)
+        if (k + 1 <= M - 2)
+            ICM_TRY(launch_copy<real>(r_hist + (int64_t)(k + 1) * chan_total, r, chan_total, st));
+        return ICM_OK;
+    }
+
+    // residue row M-1 (cudaICEEMDAN.cu:1298)
+    int signal_end(hipStream_t st)
+    {
+        const int M = cfg.num_imfs, C = cfg.num_channels;
         for (int c = 0; c < C; ++c)
             ICM_TRY(launch_copy<real>(imf + chan_imf_off[c] + (int64_t)(M - 1) * lens[c],
                                       r + chan_off[c], lens[c], st));
         return ICM_OK;
+    }
+
+    int signal_phase(hipStream_t st)
+    {
+        int rc;
+        if ((rc = signal_begin(st))) return rc;
+        for (int k = 0; k < cfg.num_imfs - 1; ++k)
+            if ((rc = signal_stage(k, st))) return rc;
+        return signal_end(st);
     }
 
     int capture_or_run(int which, hipStream_t st)
@@ -792,23 +824,13 @@ struct Plan : PlanBase {
     {
         const int C = cfg.num_channels, M = cfg.num_imfs;
         *h_err = 0;
+        stage_ = -1;
         ICM_TRY(hipEventRecord(ev[0], st));
         ICM_TRY(launch_fill_u32((uint32_t*)err, 0u, 1, st));
-        for (int c = 0; c < C; ++c) {
-            if (!sig[c] || !out[c]) return ICM_ERR_INVALID_ARG;
-            ICM_TRY(launch_copy<real>(y + chan_off[c], (const real*)sig[c], lens[c], st));
-        }
-        noise_external = (noise != nullptr);
-        if (noise_external) {
-            if (glen.size() != 1) return ICM_ERR_INVALID_ARG;
-            ICM_TRY(launch_copy<real>(wres, (const real*)noise, hn.samples, st));
-        } else {
-            // white noise w^i (cudaICEEMDAN.cu:739-741), one set per distinct length
-            for (size_t g = 0; g < glen.size(); ++g)
-                ICM_TRY(launch_noise<real>(cfg.seed, cfg.num_noise, glen[g], wres + group_off[g],
-                                           glen[g], st));
-        }
+        for (int c = 0; c < C; ++c)
+            if (!out[c]) return ICM_ERR_INVALID_ARG;
         int rc;
+        if ((rc = inputs_and_noise(sig, noise, st))) return rc;
         if ((rc = reset_control(st))) return rc;
         if ((rc = capture_or_run(0, st))) return rc;
         ICM_TRY(hipEventRecord(ev[1], st));
@@ -829,6 +851,179 @@ struct Plan : PlanBase {
         prof_collect();
         if (*h_err) return ICM_ERR_DEVICE;
         have_forward = true;
+        return ICM_OK;
+    }
+
+    // inputs into the plan (d_signals may be null: y is then left as it is)
+    // and the white noise w^i (cudaICEEMDAN.cu:739-741), one set per
+    // distinct length, or the caller's noise
+    int inputs_and_noise(const void* const* sig, const void* noise, hipStream_t st)
+    {
+        const int C = cfg.num_channels;
+        if (sig) {
+            for (int c = 0; c < C; ++c) {
+                if (!sig[c]) return ICM_ERR_INVALID_ARG;
+                ICM_TRY(launch_copy<real>(y + chan_off[c], (const real*)sig[c], lens[c], st));
+            }
+        }
+        noise_external = (noise != nullptr);
+        if (noise_external) {
+            if (glen.size() != 1) return ICM_ERR_INVALID_ARG;
+            ICM_TRY(launch_copy<real>(wres, (const real*)noise, hn.samples, st));
+        } else {
+            for (size_t g = 0; g < glen.size(); ++g)
+                ICM_TRY(launch_noise<real>(cfg.seed, cfg.num_noise, glen[g], wres + group_off[g],
+                                           glen[g], st));
+        }
+        return ICM_OK;
+    }
+
+    int sync_err(hipStream_t st)
+    {
+        ICM_TRY(hipMemcpyAsync(h_err, err, sizeof(int), hipMemcpyDeviceToHost, st));
+        ICM_TRY(hipStreamSynchronize(st));
+        return *h_err ? ICM_ERR_DEVICE : ICM_OK;
+    }
+
+    // ---- staged forward: noise modes and stage-0 statistics, then CEEMDAN
+    // stages a few at a time, with the state between stages saveable
+    int forward_begin(const void* const* sig, const void* noise, hipStream_t st) override
+    {
+        *h_err = 0;
+        have_forward = false;
+        ICM_TRY(launch_fill_u32((uint32_t*)err, 0u, 1, st));
+        int rc;
+        if ((rc = inputs_and_noise(sig, noise, st))) return rc;
+        if ((rc = reset_control(st))) return rc;
+        if ((rc = noise_phase(st))) return rc;
+        if ((rc = signal_begin(st))) return rc;
+        if ((rc = sync_err(st))) return rc;
+        stage_ = 0;
+        return ICM_OK;
+    }
+
+    int forward_stages(int count, int* stage_out, hipStream_t st) override
+    {
+        if (stage_ < 0) return ICM_ERR_STATE;
+        int rc;
+        for (int n = 0; n < count && stage_ < cfg.num_imfs - 1; ++n) {
+            if ((rc = signal_stage(stage_, st))) return rc;
+            ++stage_;
+        }
+        if ((rc = sync_err(st))) return rc;
+        if (stage_out) *stage_out = stage_;
+        return ICM_OK;
+    }
+
+    int forward_finish(void* const* out, hipStream_t st) override
+    {
+        const int C = cfg.num_channels, M = cfg.num_imfs;
+        if (stage_ != M - 1) return ICM_ERR_STATE;
+        for (int c = 0; c < C; ++c)
+            if (!out[c]) return ICM_ERR_INVALID_ARG;
+        int rc;
+        if ((rc = signal_end(st))) return rc;
+        for (int c = 0; c < C; ++c)
+            ICM_TRY(launch_copy<real>((real*)out[c], imf + chan_imf_off[c], (int64_t)M * lens[c], st));
+        if ((rc = sync_err(st))) return rc;
+        stage_ = -1;
+        have_forward = true;
+        return ICM_OK;
+    }
+
+    // state between stages: header, then y, r_k, IMF rows, residue history
+    struct StateHeader {
+        uint64_t magic, version;
+        int32_t dtype, num_channels, num_noise, num_imfs, stop_mode, max_sifts, noise_max_sifts,
+            noise_external, stage, pad;
+        uint64_t seed;
+        double noise_strength;
+        int64_t chan_total, len_hash;
+    };
+    static constexpr uint64_t kStateMagic = 0x5445415453434d49ull;   // "IMCSTATE"
+
+    int64_t len_hash() const
+    {
+        uint64_t h = 1469598103934665603ull;
+        for (int64_t L : lens) h = (h ^ (uint64_t)L) * 1099511628211ull;
+        return (int64_t)(h >> 1);
+    }
+    int64_t state_bytes() const override
+    {
+        const int M = cfg.num_imfs;
+        return (int64_t)sizeof(StateHeader) +
+               (int64_t)sizeof(real) * chan_total * (1 + 1 + M + (M - 1));
+    }
+    StateHeader header() const
+    {
+        StateHeader h{};
+        h.magic = kStateMagic;
+        h.version = 1;
+        h.dtype = cfg.dtype;
+        h.num_channels = cfg.num_channels;
+        h.num_noise = cfg.num_noise;
+        h.num_imfs = cfg.num_imfs;
+        h.stop_mode = cfg.stop_mode;
+        h.max_sifts = cfg.max_sifts;
+        h.noise_max_sifts = cfg.noise_max_sifts;
+        h.noise_external = noise_external ? 1 : 0;
+        h.stage = stage_;
+        h.seed = cfg.seed;
+        h.noise_strength = cfg.noise_strength;
+        h.chan_total = chan_total;
+        h.len_hash = len_hash();
+        return h;
+    }
+    int state_save(void* host, int64_t bytes, hipStream_t st) override
+    {
+        if (stage_ < 0) return ICM_ERR_STATE;
+        if (!host || bytes < state_bytes()) return ICM_ERR_INVALID_ARG;
+        const int M = cfg.num_imfs;
+        ICM_TRY(hipStreamSynchronize(st));
+        const StateHeader h = header();
+        std::memcpy(host, &h, sizeof h);
+        char* p = (char*)host + sizeof h;
+        const size_t row = sizeof(real) * (size_t)chan_total;
+        ICM_TRY(hipMemcpy(p, y, row, hipMemcpyDeviceToHost));
+        ICM_TRY(hipMemcpy(p + row, r, row, hipMemcpyDeviceToHost));
+        ICM_TRY(hipMemcpy(p + 2 * row, imf, row * M, hipMemcpyDeviceToHost));
+        ICM_TRY(hipMemcpy(p + (2 + M) * row, r_hist, row * (M - 1), hipMemcpyDeviceToHost));
+        return ICM_OK;
+    }
+    int state_load(const void* host, int64_t bytes, const void* noise, hipStream_t st) override
+    {
+        if (!host || bytes < state_bytes()) return ICM_ERR_INVALID_ARG;
+        StateHeader h;
+        std::memcpy(&h, host, sizeof h);
+        StateHeader mine = header();
+        if (h.magic != kStateMagic || h.version != 1 || h.dtype != mine.dtype ||
+            h.num_channels != mine.num_channels || h.num_noise != mine.num_noise ||
+            h.num_imfs != mine.num_imfs || h.stop_mode != mine.stop_mode ||
+            h.max_sifts != mine.max_sifts || h.noise_max_sifts != mine.noise_max_sifts ||
+            h.seed != mine.seed || h.noise_strength != mine.noise_strength ||
+            h.chan_total != mine.chan_total || h.len_hash != mine.len_hash ||
+            h.stage < 0 || h.stage > cfg.num_imfs - 1)
+            return ICM_ERR_INVALID_ARG;
+        // the noise modes are recomputed (deterministic), from the same noise
+        if ((h.noise_external != 0) != (noise != nullptr)) return ICM_ERR_INVALID_ARG;
+        const int M = cfg.num_imfs;
+        const char* p = (const char*)host + sizeof h;
+        const size_t row = sizeof(real) * (size_t)chan_total;
+        ICM_TRY(hipStreamSynchronize(st));
+        ICM_TRY(hipMemcpy(y, p, row, hipMemcpyHostToDevice));
+        int rc;
+        *h_err = 0;
+        have_forward = false;
+        ICM_TRY(launch_fill_u32((uint32_t*)err, 0u, 1, st));
+        if ((rc = inputs_and_noise(nullptr, noise, st))) return rc;
+        if ((rc = reset_control(st))) return rc;
+        if ((rc = noise_phase(st))) return rc;
+        if ((rc = signal_begin(st))) return rc;
+        if ((rc = sync_err(st))) return rc;
+        ICM_TRY(hipMemcpy(r, p + row, row, hipMemcpyHostToDevice));
+        ICM_TRY(hipMemcpy(imf, p + 2 * row, row * M, hipMemcpyHostToDevice));
+        ICM_TRY(hipMemcpy(r_hist, p + (2 + M) * row, row * (M - 1), hipMemcpyHostToDevice));
+        stage_ = h.stage;
         return ICM_OK;
     }
 
@@ -1228,6 +1423,45 @@ int icm_noise(int dtype, uint64_t seed, int64_t I, int64_t N, void* d_out, void*
     if (e != hipSuccess) return ICM_ERR_HIP;
     if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return ICM_ERR_HIP;
     return ICM_OK;
+}
+
+int icm_forward_begin(icm_plan plan, const void* const* d_signals, const void* d_noise,
+                      void* stream)
+{
+    if (!plan || !d_signals) return ICM_ERR_INVALID_ARG;
+    return plan->p->forward_begin(d_signals, d_noise, (hipStream_t)stream);
+}
+
+int icm_forward_stages(icm_plan plan, int count, int* stage, void* stream)
+{
+    if (!plan || count < 0) return ICM_ERR_INVALID_ARG;
+    return plan->p->forward_stages(count, stage, (hipStream_t)stream);
+}
+
+int icm_forward_finish(icm_plan plan, void* const* d_imfs, void* stream)
+{
+    if (!plan || !d_imfs) return ICM_ERR_INVALID_ARG;
+    return plan->p->forward_finish(d_imfs, (hipStream_t)stream);
+}
+
+int icm_state_bytes(icm_plan plan, int64_t* bytes)
+{
+    if (!plan || !bytes) return ICM_ERR_INVALID_ARG;
+    *bytes = plan->p->state_bytes();
+    return ICM_OK;
+}
+
+int icm_state_save(icm_plan plan, void* host, int64_t bytes, void* stream)
+{
+    if (!plan) return ICM_ERR_INVALID_ARG;
+    return plan->p->state_save(host, bytes, (hipStream_t)stream);
+}
+
+int icm_state_load(icm_plan plan, const void* host, int64_t bytes, const void* d_noise,
+                   void* stream)
+{
+    if (!plan) return ICM_ERR_INVALID_ARG;
+    return plan->p->state_load(host, bytes, d_noise, (hipStream_t)stream);
 }
 
 int icm_backward(icm_plan plan, const void* const* d_grad_imfs, void* const* d_grad_signals,

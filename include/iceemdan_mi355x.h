@@ -139,6 +139,28 @@ int icm_noise(int dtype, uint64_t seed, int64_t I, int64_t N, void* d_out, void*
 int icm_sift_pass(int dtype, const void* d_h, int64_t N, int64_t I, void* d_h_new, int32_t* d_nk,
                   void* stream);
 
+/* Staged forward with stage-boundary checkpoints (no reference counterpart;
+ * the reference runs a decomposition in one call and cannot resume one).
+ *   icm_forward_begin   inputs, noise modes and stage-0 statistics; the plan
+ *                       is then before CEEMDAN stage 0
+ *   icm_forward_stages  runs up to `count` CEEMDAN stages; *stage (may be
+ *                       NULL) receives the next stage, M-1 when all are done
+ *   icm_forward_finish  after the last stage: residue row, copy to d_imfs
+ * The result equals icm_forward's bit for bit.  Between stages the state
+ * (stage k, the inputs, r_k, rows 0..k-1, the residue history) can be
+ * copied to host memory and loaded into a plan of the same configuration,
+ * in this or another process; loading recomputes the noise modes from the
+ * seed, or from d_noise when the saved run was given its noise (pass the
+ * same noise again).  icm_state_bytes gives the size of the host buffer. */
+int icm_forward_begin(icm_plan plan, const void* const* d_signals, const void* d_noise,
+                      void* stream);
+int icm_forward_stages(icm_plan plan, int count, int* stage, void* stream);
+int icm_forward_finish(icm_plan plan, void* const* d_imfs, void* stream);
+int icm_state_bytes(icm_plan plan, int64_t* bytes);
+int icm_state_save(icm_plan plan, void* host, int64_t bytes, void* stream);
+int icm_state_load(icm_plan plan, const void* host, int64_t bytes, const void* d_noise,
+                   void* stream);
+
 /* Adjoint of the last icm_forward of this plan (no reference counterpart):
  * given dL/dIMF (d_grad_imfs[c]: [M][N_c], the layout of d_imfs), writes
  * dL/dx (d_grad_signals[c]: [N_c]) for every channel.  FIXED stop rule only
