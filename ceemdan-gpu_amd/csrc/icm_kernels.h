@@ -110,6 +110,7 @@ struct SolveArgs {
     int* err;
     int* px;                // optional [2][kstride]: knot positions, compact (for the adjoint)
     int* fixf;              // optional [nseries][2][2]: endpoint fix applied (for the adjoint)
+    int nak;                // not-a-knot spline ends
 };
 
 // ---------------------------------------------------------------- adjoint

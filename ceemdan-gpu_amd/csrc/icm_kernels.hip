@@ -859,12 +859,14 @@ __global__ __launch_bounds__(kThreads) void k_sift(SiftArgs<real> a)
 
 // ------------------------------------------------------- windowed spline solve
 
-// Row g of the natural-spline system (cudaICEEMDAN.cu:310-343); rows outside
+// Row g of the spline system (cudaICEEMDAN.cu:310-343); rows outside
 // [0, n) and the two end rows are identity rows with a zero right-hand side.
-// LDS index i holds knot g0 - 1 + i.
+// LDS index i holds knot g0 - 1 + i.  Not-a-knot ends (nak, n >= 4): c_0 is
+// eliminated from row 1 and c_{n-1} from row n-2 (both stay diagonally
+// dominant); the end values follow from nak_end after the solve.
 template <typename real>
 __device__ __forceinline__ void spline_row(int n, long g, int i, const int* sp, const real* sy,
-                                           real& dl, real& d, real& du, real& f)
+                                           bool nak, real& dl, real& d, real& du, real& f)
 {
     if (g <= 0 || g >= n - 1) {
         dl = (real)0; d = (real)1; du = (real)0; f = (real)0;
@@ -876,6 +878,26 @@ __device__ __forceinline__ void spline_row(int n, long g, int i, const int* sp, 
     du = (real)hr;
     d = (real)(2 * (hl + hr));
     f = (real)3 * ((sy[i + 1] - sy[i]) / (real)hr - (sy[i] - sy[i - 1]) / (real)hl);
+    if (nak && n >= 4) {
+        if (g == 1) {
+            dl = (real)0;
+            d = (real)(3 * hl + 2 * hr) + (real)hl * (real)hl / (real)hr;
+            du = (real)hr - (real)hl * (real)hl / (real)hr;
+        }
+        if (g == n - 2) {
+            dl = (real)hl - (real)hr * (real)hr / (real)hl;
+            d = (real)(2 * hl + 3 * hr) + (real)hr * (real)hr / (real)hl;
+            du = (real)0;
+        }
+    }
+}
+
+// not-a-knot end value: c_e = c_1 (1 + r) - c_2 r, r = h_e / h_next
+template <typename real>
+__device__ __forceinline__ real nak_end(real c1, real c2, int h_e, int h_next)
+{
+    const real r = (real)h_e / (real)h_next;
+    return c1 * ((real)1 + r) - c2 * r;
 }
 
 __device__ __forceinline__ void wave_sync()
@@ -1110,18 +1132,39 @@ __global__ __launch_bounds__(kThreads) void k_solve(SolveArgs<real> a)
         partition_pcr<real, B>(
             lane,
             [&](int r, real& dl, real& d, real& du, real& f) {
-                spline_row<real>(n, g0 + r, r + 1, sp, sy, dl, d, du, f);
+                spline_row<real>(n, g0 + r, r + 1, sp, sy, a.nak, dl, d, du, f);
             },
             xs);
         real* cout = a.c + kb;
         const int rb = lane * B;
+        const bool nak = a.nak && n >= 4;
 #pragma unroll
         for (int q = 0; q < B; ++q) {
             const int r = rb + q;
             const long gg = g0 + r;
             if (r >= H && r < H + K && gg < n) {
-                cout[gg] = xs[q];
+                if (!(nak && (gg == 0 || gg == n - 1))) cout[gg] = xs[q];
                 if (a.px) a.px[kb + gg] = sp[r + 1];
+            }
+        }
+        if (nak) {
+            // the window that keeps an end knot recovers it from its own rows
+            wave_sync();
+#pragma unroll
+            for (int q = 0; q < B; ++q) sy[rb + q + 1] = xs[q];
+            wave_sync();
+            if (lane == 0) {
+                const long r0 = -g0, r1 = (long)(n - 1) - g0;
+                if (r0 >= H && r0 < H + K) {
+                    const int i0 = (int)r0 + 1;
+                    cout[0] = nak_end<real>(sy[i0 + 1], sy[i0 + 2], sp[i0 + 1] - sp[i0],
+                                            sp[i0 + 2] - sp[i0 + 1]);
+                }
+                if (r1 >= H && r1 < H + K) {
+                    const int i1 = (int)r1 + 1;
+                    cout[n - 1] = nak_end<real>(sy[i1 - 1], sy[i1 - 2], sp[i1] - sp[i1 - 1],
+                                                sp[i1 - 1] - sp[i1 - 2]);
+                }
             }
         }
         wave_sync();

@@ -73,6 +73,8 @@ int validate(const icm_config* c)
     if (c->max_sifts < 1 || c->max_sifts > 100000) return ICM_ERR_INVALID_ARG;
     if (c->noise_max_sifts < 1 || c->noise_max_sifts > 100000) return ICM_ERR_INVALID_ARG;
     if (!std::isfinite(c->noise_strength)) return ICM_ERR_INVALID_ARG;
+    if (c->boundary != ICM_BOUNDARY_NATURAL && c->boundary != ICM_BOUNDARY_NOT_A_KNOT)
+        return ICM_ERR_INVALID_ARG;
     int64_t total = 0;
     for (int i = 0; i < c->num_channels; ++i) total += c->lengths[i];
     if (total * (int64_t)c->num_noise > (int64_t)1 << 40) return ICM_ERR_INVALID_ARG;
@@ -573,6 +575,7 @@ struct Plan : PlanBase {
             sa.wincount = wincount + e_emit;
             sa.active = nullptr;
             sa.err = err;
+            sa.nak = cfg.boundary == ICM_BOUNDARY_NOT_A_KNOT;
             if (record) {
                 // structure of pass j's input for the adjoint
                 const Rec& R = record[j];
@@ -642,6 +645,7 @@ struct Plan : PlanBase {
             sa.wincount = wincount + e_emit;
             sa.active = active;
             sa.err = err;
+            sa.nak = cfg.boundary == ICM_BOUNDARY_NOT_A_KNOT;
             ICM_TRY(launch_solve<real>(sa, solve_grid(), st));
             SiftArgs<real> a{};
             a.sd = sd;
@@ -968,6 +972,7 @@ struct Plan : PlanBase {
         h.noise_max_sifts = cfg.noise_max_sifts;
         h.noise_external = noise_external ? 1 : 0;
         h.stage = stage_;
+        h.pad = cfg.boundary;
         h.seed = cfg.seed;
         h.noise_strength = cfg.noise_strength;
         h.chan_total = chan_total;
@@ -1000,7 +1005,7 @@ struct Plan : PlanBase {
             h.num_channels != mine.num_channels || h.num_noise != mine.num_noise ||
             h.num_imfs != mine.num_imfs || h.stop_mode != mine.stop_mode ||
             h.max_sifts != mine.max_sifts || h.noise_max_sifts != mine.noise_max_sifts ||
-            h.seed != mine.seed || h.noise_strength != mine.noise_strength ||
+            h.seed != mine.seed || h.noise_strength != mine.noise_strength || h.pad != mine.pad ||
             h.chan_total != mine.chan_total || h.len_hash != mine.len_hash ||
             h.stage < 0 || h.stage > cfg.num_imfs - 1)
             return ICM_ERR_INVALID_ARG;
@@ -1059,7 +1064,9 @@ struct Plan : PlanBase {
     // Reverse mode over the signal phase of the last forward.
     int backward(const void* const* gimf, void* const* gsig, hipStream_t st)
     {
-        if (!have_forward || cfg.stop_mode != ICM_STOP_FIXED) return ICM_ERR_STATE;
+        if (!have_forward || cfg.stop_mode != ICM_STOP_FIXED ||
+            cfg.boundary != ICM_BOUNDARY_NATURAL)
+            return ICM_ERR_STATE;
         const int C = cfg.num_channels, M = cfg.num_imfs, I = cfg.num_noise, S = cfg.max_sifts;
         const real eps0 = (real)cfg.noise_strength;
         int rc;

@@ -42,6 +42,9 @@ extern "C" {
 #define ICM_STOP_FIXED 0         /* exactly max_sifts subtractions per stage */
 #define ICM_STOP_RILLING 1       /* reference stop rule, capped at max_sifts */
 
+#define ICM_BOUNDARY_NATURAL 0      /* natural spline ends (the reference's) */
+#define ICM_BOUNDARY_NOT_A_KNOT 1   /* not-a-knot ends (cudaICEEMDAN.cu:345-380 was never wired) */
+
 #define ICM_PHASE_NOISE 0
 #define ICM_PHASE_SIGNAL 1
 
@@ -57,6 +60,8 @@ typedef struct icm_config {
     double noise_strength;      /* eps0 */
     uint64_t seed;              /* Philox key of the white noise */
     int use_graph;              /* FIXED only: replay the forward as a captured hipGraph */
+    int boundary;               /* ICM_BOUNDARY_*: spline end condition (envelopes with >= 4
+                                   knots; fewer knots keep natural ends) */
 } icm_config;
 
 typedef struct icm_plan_s* icm_plan;

@@ -106,9 +106,17 @@ void ocm_boundary_fix(int n, const int* pos, double* val, int is_max)
 
 /* ----------------------------------------------------- tridiagonal system */
 
-/* Row g of the natural-spline system on knots (pos, val), n rows.  Rows
- * outside [0, n) are inert identity rows (the reference pads with them,
- * cudaICEEMDAN.cu:293-308). */
+/* Boundary condition of the spline (natural by default; not-a-knot is the
+ * option the reference started but never wired, cudaICEEMDAN.cu:345-380). */
+static int g_bc = OCM_BC_NATURAL;
+void ocm_set_boundary(int bc) { g_bc = bc; }
+
+/* Row g of the spline system on knots (pos, val), n rows.  Rows outside
+ * [0, n) and the two end rows are inert identity rows (the reference pads
+ * with them, cudaICEEMDAN.cu:293-308).  Not-a-knot (n >= 4): d_0 = d_1 and
+ * d_{n-3} = d_{n-2}; c_0 is eliminated from row 1 and c_{n-1} from row n-2,
+ * which keeps both rows diagonally dominant, and the end values are
+ * recovered after the solve (ocm_nak_end). */
 static void spline_row(int n, const int* pos, const double* y, long g,
                        double* dl, double* d, double* du, double* f)
 {
@@ -122,6 +130,26 @@ static void spline_row(int n, const int* pos, const double* y, long g,
     *du = (double)hr;
     *d = (double)(2 * (hl + hr));
     *f = 3.0 * ((y[g + 1] - y[g]) / (double)hr - (y[g] - y[g - 1]) / (double)hl);
+    if (g_bc == OCM_BC_NOT_A_KNOT && n >= 4) {
+        if (g == 1) {
+            *dl = 0.0;
+            *d = (double)(3 * hl + 2 * hr) + (double)hl * (double)hl / (double)hr;
+            *du = (double)hr - (double)hl * (double)hl / (double)hr;
+        }
+        if (g == n - 2) {
+            *dl = (double)hl - (double)hr * (double)hr / (double)hl;
+            *d = (double)(2 * hl + 3 * hr) + (double)hr * (double)hr / (double)hl;
+            *du = 0.0;
+        }
+    }
+}
+
+/* Not-a-knot end value from its two neighbours: c_e = c_1 (1 + r) - c_2 r,
+ * r = h_e / h_next (h_e: interval at the end, h_next: the one after it). */
+static double nak_end(double c1, double c2, int h_e, int h_next)
+{
+    double r = (double)h_e / (double)h_next;
+    return c1 * (1.0 + r) - c2 * r;
 }
 
 void ocm_solve_window(int n, const int* pos, const double* y, int B, long g0, double* x)
@@ -239,6 +267,15 @@ void ocm_spline_solve(int n, const int* pos, const double* val, int B, int H, do
             long g = g0 + r;
             if (g >= 0 && g < n) c[g] = x[r];
         }
+        if (g_bc == OCM_BC_NOT_A_KNOT && n >= 4) {
+            /* the window that keeps an end knot recovers it from its own rows */
+            long r0 = -g0, r1 = (long)(n - 1) - g0;
+            if (r0 >= H && r0 < H + K)
+                c[0] = nak_end(x[r0 + 1], x[r0 + 2], pos[1] - pos[0], pos[2] - pos[1]);
+            if (r1 >= H && r1 < H + K)
+                c[n - 1] = nak_end(x[r1 - 1], x[r1 - 2], pos[n - 1] - pos[n - 2],
+                                   pos[n - 2] - pos[n - 3]);
+        }
     }
     free(x);
 }
@@ -261,6 +298,10 @@ void ocm_spline_solve_thomas(int n, const int* pos, const double* val, double* c
     }
     c[n - 1] = dp[n - 1];
     for (int g = n - 2; g >= 0; --g) c[g] = dp[g] - cp[g] * c[g + 1];
+    if (g_bc == OCM_BC_NOT_A_KNOT && n >= 4) {
+        c[0] = nak_end(c[1], c[2], pos[1] - pos[0], pos[2] - pos[1]);
+        c[n - 1] = nak_end(c[n - 2], c[n - 3], pos[n - 1] - pos[n - 2], pos[n - 2] - pos[n - 3]);
+    }
     free(cp);
 }
 

@@ -24,6 +24,11 @@ extern "C" {
 #define OCM_STOP_FIXED   0   /* exactly S subtractions, then export */
 #define OCM_STOP_RILLING 1   /* reference rule, cap passes (cudaICEEMDAN.cu:494-544) */
 
+/* spline boundary condition (process-wide setting of the oracle) */
+#define OCM_BC_NATURAL     0
+#define OCM_BC_NOT_A_KNOT  1
+void ocm_set_boundary(int bc);
+
 /* Strict local extrema with both endpoints forced (cudaICEEMDAN.cu:92-148). */
 int ocm_extrema(const double* h, int N, int is_max, int* pos, double* val);
 

@@ -13,6 +13,8 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 STOP_FIXED = 0
 STOP_RILLING = 1
+BC_NATURAL = 0
+BC_NOT_A_KNOT = 1
 
 # default windowed-solve geometry of the FP64 path: 64 lanes x B rows, H halo
 GEOM_F64 = (8, 32)
@@ -32,8 +34,23 @@ def lib():
     return _lib
 
 
+class _boundary:
+    """Sets the oracle's spline boundary condition for the duration of a call."""
+
+    def __init__(self, bc):
+        self.bc = int(bc)
+
+    def __enter__(self):
+        lib().ocm_set_boundary(self.bc)
+
+    def __exit__(self, *exc):
+        lib().ocm_set_boundary(BC_NATURAL)
+
+
 def _setup(L):
     dp = C.POINTER(C.c_double)
+    L.ocm_set_boundary.argtypes = [C.c_int]
+    L.ocm_set_boundary.restype = None
     ip = C.POINTER(C.c_int)
     L.ocm_extrema.argtypes = [dp, C.c_int, C.c_int, ip, dp]
     L.ocm_extrema.restype = C.c_int
@@ -92,19 +109,21 @@ def boundary_fix(pos, val, is_max):
     return val
 
 
-def spline_solve(pos, val, geom=GEOM_F64):
+def spline_solve(pos, val, geom=GEOM_F64, bc=BC_NATURAL):
     pos = np.ascontiguousarray(pos, np.int32)
     val = _f64(val)
     c = np.empty(pos.size, np.float64)
-    lib().ocm_spline_solve(pos.size, _i(pos), _d(val), geom[0], geom[1], _d(c))
+    with _boundary(bc):
+        lib().ocm_spline_solve(pos.size, _i(pos), _d(val), geom[0], geom[1], _d(c))
     return c
 
 
-def spline_solve_thomas(pos, val):
+def spline_solve_thomas(pos, val, bc=BC_NATURAL):
     pos = np.ascontiguousarray(pos, np.int32)
     val = _f64(val)
     c = np.empty(pos.size, np.float64)
-    lib().ocm_spline_solve_thomas(pos.size, _i(pos), _d(val), _d(c))
+    with _boundary(bc):
+        lib().ocm_spline_solve_thomas(pos.size, _i(pos), _d(val), _d(c))
     return c
 
 
@@ -162,7 +181,7 @@ def sift_mode(x, stop_mode, S_or_cap, geom=GEOM_F64, trace_passes=0):
 
 
 def iceemdan(y, noise, M, stop_mode=STOP_FIXED, S_or_cap=10, noise_cap=None, eps0=0.2,
-             geom=GEOM_F64, nthreads=0, trace_passes=0, want_noise_modes=False):
+             geom=GEOM_F64, nthreads=0, trace_passes=0, want_noise_modes=False, bc=BC_NATURAL):
     """Full ICEEMDAN on one channel.  noise: (I, N) standard normals.
 
     Returns dict(imfs=(M, N), trace=(2, M-1, P, I, 2) knot counts,
@@ -178,9 +197,10 @@ def iceemdan(y, noise, M, stop_mode=STOP_FIXED, S_or_cap=10, noise_cap=None, eps
     trace = np.zeros((2, M - 1, max(P, 1), I, 2), np.int32)
     stops = np.zeros((2, M - 1, I), np.int32)
     nm = np.empty((M - 1, I, N), np.float64) if want_noise_modes else None
-    rc = lib().ocm_iceemdan(_d(y), N, I, M, stop_mode, S_or_cap, noise_cap, eps0, _d(noise),
-                            geom[0], geom[1], nthreads, _d(imfs), _i(trace) if P else None, P,
-                            _i(stops), _d(nm) if nm is not None else None)
+    with _boundary(bc):
+        rc = lib().ocm_iceemdan(_d(y), N, I, M, stop_mode, S_or_cap, noise_cap, eps0, _d(noise),
+                                geom[0], geom[1], nthreads, _d(imfs), _i(trace) if P else None,
+                                P, _i(stops), _d(nm) if nm is not None else None)
     if rc != 0:
         raise ValueError(f"oracle iceemdan failed: {rc}")
     out = dict(imfs=imfs, stops=stops)

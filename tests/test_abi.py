@@ -81,7 +81,7 @@ def test_host_validation(icm):
     assert _cfg(icm) == 0
     bad = [dict(lengths=[3]), dict(num_noise=0), dict(num_imfs=1),
            dict(max_sifts=0), dict(stop_mode=7), dict(dtype_code=5),
-           dict(noise_strength=float("nan")), dict(lengths=[20_000_000]),
+           dict(noise_strength=float("nan")), dict(lengths=[20_000_000]), dict(boundary=2),
            dict(num_noise=70000)]
     for b in bad:
         assert _cfg(icm, **b) == -1, b

@@ -143,3 +143,26 @@ def test_golden_fixtures_on_gpu(icm, dev):
     g = plan.backward([torch.tensor(G, device=dev)])[0].cpu().numpy()
     plan.close()
     assert relerr(g, data["grad_eeg/grad"]) < 1e-9
+
+
+@pytest.mark.parametrize("stop_mode", [0, 1])
+def test_not_a_knot_fp64_bitexact(icm, orc, dev, stop_mode):
+    """Not-a-knot spline ends: the HIP path equals the oracle bit for bit
+    (FIXED and RILLING); the adjoint is natural-ends only."""
+    import torch
+    N, I, M = 3000, 6, 5
+    x = _eeg("eegSampleDataCH4.f32", 30504)[7000:7000 + N]
+    print(f"case: not-a-knot stop_mode={stop_mode}", flush=True)
+    w = icm.noise(I, N, seed=11, dtype=torch.float64, device=dev)
+    S = 10 if stop_mode == 0 else 60
+    plan = icm.Plan([N], I, M, dtype=torch.float64, stop_mode=stop_mode, max_sifts=S,
+                    noise_max_sifts=S if stop_mode == 0 else 300, use_graph=False, device=dev,
+                    boundary=icm.BOUNDARY_NOT_A_KNOT)
+    got = plan.forward([torch.tensor(x, device=dev)], noise=w)[0].cpu().numpy()
+    ref = orc.iceemdan(x, w.cpu().numpy(), M, stop_mode=stop_mode, S_or_cap=S,
+                       noise_cap=S if stop_mode == 0 else 300, bc=orc.BC_NOT_A_KNOT)["imfs"]
+    assert np.array_equal(got, ref), relerr(got, ref)
+    if stop_mode == 0:
+        with pytest.raises(icm.ICMError):
+            plan.backward([torch.zeros((M, N), dtype=torch.float64, device=dev)])
+    plan.close()

@@ -313,6 +313,43 @@ def test_iceemdan_rilling_stops(orc):
     assert np.max(np.abs(imfs.sum(0) - xt)) < 1e-12
 
 
+# ---------------------------------------------------------- not-a-knot
+
+@pytest.mark.parametrize("n", [4, 5, 9, 40, 3000])
+def test_not_a_knot_matches_scipy(orc, n):
+    """c_k (half the second derivative) of the not-a-knot spline equals
+    scipy's CubicSpline(bc_type='not-a-knot'); the windowed solve (several
+    windows at n = 3000) equals the global one."""
+    rng = np.random.default_rng(n)
+    pos = np.concatenate([[0], np.cumsum(rng.integers(1, 40, n - 1))]).astype(np.int32)
+    val = rng.standard_normal(n)
+    cs = CubicSpline(pos.astype(np.float64), val, bc_type="not-a-knot")
+    ref = cs(pos.astype(np.float64), 2) / 2.0
+    ct = orc.spline_solve_thomas(pos, val, bc=orc.BC_NOT_A_KNOT)
+    scale = np.max(np.abs(ref))
+    assert np.max(np.abs(ct - ref)) <= 1e-11 * scale
+    cw = orc.spline_solve(pos, val, bc=orc.BC_NOT_A_KNOT)
+    assert np.max(np.abs(cw - ct)) <= 1e-12 * scale
+    # the third derivative is continuous at x_1 and x_{n-2}
+    d = np.diff(ct) / (3.0 * np.diff(pos))
+    assert abs(d[0] - d[1]) <= 1e-9 * np.max(np.abs(d))
+    assert abs(d[-1] - d[-2]) <= 1e-9 * np.max(np.abs(d))
+    # the setting does not leak into later natural solves
+    cn = orc.spline_solve_thomas(pos, val)
+    assert cn[0] == 0.0 and cn[-1] == 0.0
+
+
+def test_iceemdan_not_a_knot_completeness(orc):
+    N, I, M = 2048, 6, 5
+    x = three_tone(N, seed=5)
+    w = orc.noise(2, I, N)
+    nak = orc.iceemdan(x, w, M, S_or_cap=10, bc=orc.BC_NOT_A_KNOT)["imfs"]
+    nat = orc.iceemdan(x, w, M, S_or_cap=10)["imfs"]
+    xt = x / np.sqrt(np.var(x, ddof=1))
+    assert np.max(np.abs(nak.sum(0) - xt)) < 1e-12
+    assert not np.array_equal(nak, nat)
+
+
 # ---------------------------------------------------------------- adjoint
 
 def _frozen_envelope(orc, N, pos, flo, fhi):
