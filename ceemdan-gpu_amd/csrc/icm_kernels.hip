@@ -907,13 +907,14 @@ __device__ __forceinline__ void wave_sync()
 }
 
 // Copy knots [glo, ghi) of (series s, type) from the tile slots into LDS
-// (sp/sy index = g - lo).  One wavefront; tpre and tcnt are 64 ints of LDS
-// scratch each.  Tiles are taken 64 at a time; where they hold many knots
-// each (dense early modes) the wave copies one tile's contiguous slots at a
-// time, otherwise every lane finds the tile of its own knot by binary search.
-template <typename real>
+// (sp/sy index = g - lo).  One wavefront; tpre is 64 ints of LDS scratch.
+// Tiles are taken 64 at a time (their knot prefixes in LDS); each lane finds
+// the tiles of its knots g + lane + 64 j by binary search first, then issues
+// all its loads before storing any, so the loads overlap.  NIT >= the knots
+// of one window divided by 64.
+template <typename real, int NIT>
 __device__ void gather_knots(const SeriesDev& sd, const KnotsDev<real>& k, int s, int type,
-                             long lo, int glo, int ghi, int* sp, real* sy, int* tpre, int* tcnt)
+                             long lo, int glo, int ghi, int* sp, real* sy, int* tpre)
 {
     const int lane = threadIdx.x & 63;
     const int tb = sd.tile0[s], ntl = sd.ntl[s];
@@ -937,39 +938,42 @@ __device__ void gather_knots(const SeriesDev& sd, const KnotsDev<real>& k, int s
         const int p = valid ? k.pre[2 * (tb + tl) + type] : 0x7FFFFFFF;
         const int c = valid ? k.cnt[2 * (tb + tl) + type] : 0;
         tpre[lane] = p;
-        tcnt[lane] = c;
         const uint64_t vm = wave_ballot(valid);
         if (vm == 0ull) break;   // counts inconsistent with n: leave the rest
         const int last = 63 - __builtin_clzll(vm);
         const int pend = __shfl(p + c, last, kWave);
-        // tiles of this group that hold knots of [g, gend)
         const int gend = min(ghi, pend);
-        const uint64_t used = wave_ballot(valid && p < gend && p + c > g);
         wave_sync();
-        if ((gend - g) >= 16 * __popcll(used)) {
-            uint64_t m = used;
-            while (m) {
-                const int j = __builtin_ctzll(m);
-                m &= m - 1;
-                const int pj = tpre[j], cj = tcnt[j];
-                const int k0 = max(pj, g), k1 = min(pj + cj, gend);
-                const int64_t base = so + (int64_t)(tb + tile + j) * sd.cap - pj;
-                for (int gi = k0 + lane; gi < k1; gi += kWave) {
-                    sp[gi - lo] = k.x[base + gi];
-                    sy[gi - lo] = k.y[base + gi];
+        for (int gb = g; gb < gend; gb += kWave * NIT) {
+            int64_t slot[NIT];
+#pragma unroll
+            for (int j = 0; j < NIT; ++j) {
+                const int gi = gb + lane + kWave * j;
+                slot[j] = -1;
+                if (gi < gend) {
+                    int x = 0, y = last;
+                    while (x < y) {
+                        const int mid = (x + y + 1) >> 1;
+                        if (tpre[mid] <= gi) x = mid;
+                        else y = mid - 1;
+                    }
+                    slot[j] = so + (int64_t)(tb + tile + x) * sd.cap + (gi - tpre[x]);
                 }
             }
-        } else {
-            for (int gi = g + lane; gi < gend; gi += kWave) {
-                int x = 0, y = last;
-                while (x < y) {
-                    const int mid = (x + y + 1) >> 1;
-                    if (tpre[mid] <= gi) x = mid;
-                    else y = mid - 1;
+            int xv[NIT];
+            real yv[NIT];
+#pragma unroll
+            for (int j = 0; j < NIT; ++j) {
+                xv[j] = slot[j] >= 0 ? k.x[slot[j]] : 0;
+                yv[j] = slot[j] >= 0 ? k.y[slot[j]] : (real)0;
+            }
+#pragma unroll
+            for (int j = 0; j < NIT; ++j) {
+                const int gi = gb + lane + kWave * j;
+                if (slot[j] >= 0) {
+                    sp[gi - lo] = xv[j];
+                    sy[gi - lo] = yv[j];
                 }
-                const int64_t slot = so + (int64_t)(tb + tile + x) * sd.cap + (gi - tpre[x]);
-                sp[gi - lo] = k.x[slot];
-                sy[gi - lo] = k.y[slot];
             }
         }
         g = gend;
@@ -1067,7 +1071,7 @@ __global__ __launch_bounds__(kThreads) void k_solve(SolveArgs<real> a)
     constexpr int NW = kThreads / kWave;
     __shared__ int s_pos[NW][W + 2];
     __shared__ real s_y[NW][W + 2];
-    __shared__ int s_tpre[NW][kWave], s_tcnt[NW][kWave];
+    __shared__ int s_tpre[NW][kWave];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     int* sp = s_pos[wv];
     real* sy = s_y[wv];
@@ -1084,7 +1088,8 @@ __global__ __launch_bounds__(kThreads) void k_solve(SolveArgs<real> a)
         const long lo = g0 - 1;
         const int glo = (int)max(lo, 0L);
         const int ghi = (int)min(lo + W + 2, (long)n);
-        gather_knots<real>(a.sd, a.k, s, type, lo, glo, ghi, sp, sy, s_tpre[wv], s_tcnt[wv]);
+        gather_knots<real, (W + 2 + kWave - 1) / kWave>(a.sd, a.k, s, type, lo, glo, ghi, sp, sy,
+                                                         s_tpre[wv]);
         wave_sync();
         // endpoint fix-up when n >= 6 (setBoundaryMax/Min, cudaICEEMDAN.cu:209-291);
         // idempotent, so every window that holds an end applies it
