@@ -29,6 +29,7 @@ struct SeriesDev {
     const int* nidx;          // [nseries] noise series whose modes this series uses
     int nseries;
     int ntiles;
+    int max_tiles;            // most tiles of one series
     int64_t kstride;          // per-type stride of the compact c array
     int64_t sstride;          // per-type stride of the knot slots (tiles x cap)
     int cap;                  // knot slots per tile and type

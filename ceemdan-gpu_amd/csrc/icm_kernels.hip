@@ -422,6 +422,136 @@ __global__ __launch_bounds__(kPT) void k_knot_prefix_b(SeriesDev sd, KnotsDev<re
     }
 }
 
+// Series of at most 1024 tiles: the same outputs in one pass, both knot
+// types and both scan directions at once (one load round, two workgroup
+// barriers, one gather round).
+template <typename real>
+__global__ __launch_bounds__(kPT) void k_knot_prefix_1(SeriesDev sd, KnotsDev<real> k, PassCtl pc,
+                                                       const int* active)
+{
+    constexpr int NWV = kPT / kWave;
+    constexpr int INF = 0x7FFFFFFF;
+    __shared__ int s_tot[2][NWV], s_ex_tot[2][NWV], s_all[2];
+    __shared__ LastTwo s_lt[2][NWV], s_ex_lt[2][NWV];
+    __shared__ int s_min[2][NWV], s_ex_min[2][NWV];
+    __shared__ unsigned s_wbase;
+    const int s = blockIdx.x;
+    if (active && !active[s]) return;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int tb = sd.tile0[s], nt = sd.ntl[s];
+    const int cap = sd.cap;
+    const int t = tid;
+    int c[2];
+    LastTwo sc[2];
+    int inc[2], mn[2];
+#pragma unroll
+    for (int ty = 0; ty < 2; ++ty) {
+        c[ty] = t < nt ? k.cnt[2 * (tb + t) + ty] : 0;
+        sc[ty].c = min(c[ty], 2);
+        sc[ty].l1 = c[ty] >= 1 ? (int64_t)(tb + t) * cap + (c[ty] - 1) : -1;
+        sc[ty].l2 = c[ty] >= 2 ? (int64_t)(tb + t) * cap + (c[ty] - 2) : -1;
+        inc[ty] = c[ty];
+        mn[ty] = c[ty] > 0 ? t : INF;
+    }
+    for (int o = 1; o < kWave; o <<= 1) {
+#pragma unroll
+        for (int ty = 0; ty < 2; ++ty) {
+            const int v = shfl_up(inc[ty], o);
+            const LastTwo w = lt_shfl_up(sc[ty], o);
+            const int m = shfl_down(mn[ty], o);
+            if (lane >= o) {
+                inc[ty] += v;
+                sc[ty] = lt_combine(w, sc[ty]);
+            }
+            if (lane + o < kWave) mn[ty] = min(mn[ty], m);
+        }
+    }
+    int xi[2], xm[2];
+    LastTwo xl[2];
+#pragma unroll
+    for (int ty = 0; ty < 2; ++ty) {
+        xi[ty] = shfl_up(inc[ty], 1);
+        xl[ty] = lt_shfl_up(sc[ty], 1);
+        xm[ty] = shfl_down(mn[ty], 1);
+        if (lane == kWave - 1) {
+            s_tot[ty][wv] = inc[ty];
+            s_lt[ty][wv] = sc[ty];
+        }
+        if (lane == 0) s_min[ty][wv] = mn[ty];
+    }
+    __syncthreads();
+    if (wv == 0) {
+        // scans of the wave totals by the first NWV lanes
+#pragma unroll
+        for (int ty = 0; ty < 2; ++ty) {
+            const int v = lane < NWV ? s_tot[ty][lane] : 0;
+            const LastTwo l = lane < NWV ? s_lt[ty][lane] : lt_none();
+            const int m0 = lane < NWV ? s_min[ty][lane] : INF;
+            int vi = v, mi = m0;
+            LastTwo li = l;
+            for (int o = 1; o < NWV; o <<= 1) {
+                const int a1 = shfl_up(vi, o);
+                const LastTwo b1 = lt_shfl_up(li, o);
+                const int c1 = shfl_down(mi, o);
+                if (lane >= o) {
+                    vi += a1;
+                    li = lt_combine(b1, li);
+                }
+                if (lane + o < NWV) mi = min(mi, c1);
+            }
+            const LastTwo le = lt_shfl_up(li, 1);
+            const int me = shfl_down(mi, 1);
+            if (lane < NWV) {
+                s_ex_tot[ty][lane] = vi - v;
+                s_ex_lt[ty][lane] = lane > 0 ? le : lt_none();
+                s_ex_min[ty][lane] = lane < NWV - 1 ? me : INF;
+            }
+            if (lane == NWV - 1) s_all[ty] = vi;
+        }
+    }
+    __syncthreads();
+    if (t < nt) {
+#pragma unroll
+        for (int ty = 0; ty < 2; ++ty) {
+            const int64_t so = ty ? sd.sstride : 0;
+            const int ex = s_ex_tot[ty][wv] + (lane > 0 ? xi[ty] : 0);
+            const LastTwo before = lt_combine(s_ex_lt[ty][wv], lane > 0 ? xl[ty] : lt_none());
+            const int after = min(s_ex_min[ty][wv], lane < kWave - 1 ? xm[ty] : INF);
+            const int64_t sl = so + (int64_t)(tb + (after == INF ? 0 : after)) * cap;
+            const int x1 = before.l1 >= 0 ? k.x[so + before.l1] : -1;
+            const real y1 = before.l1 >= 0 ? k.y[so + before.l1] : (real)0;
+            const int x2 = before.l2 >= 0 ? k.x[so + before.l2] : -1;
+            const real y2 = before.l2 >= 0 ? k.y[so + before.l2] : (real)0;
+            const int x3 = after == INF ? -1 : k.x[sl];
+            const real y3 = after == INF ? (real)0 : k.y[sl];
+            k.pre[2 * (tb + t) + ty] = ex;
+            const int64_t o = 6 * (int64_t)(tb + t) + 3 * ty;
+            k.nbx[o] = x1;
+            k.nby[o] = y1;
+            k.nbx[o + 1] = x2;
+            k.nby[o + 1] = y2;
+            k.nbx[o + 2] = x3;
+            k.nby[o + 2] = y3;
+        }
+    }
+    const int n0 = s_all[0], n1 = s_all[1];
+    const int wmx = (n0 + pc.K - 1) / pc.K;
+    const int wmn = (n1 + pc.K - 1) / pc.K;
+    if (tid == 0) {
+        k.n[2 * s] = n0;
+        k.n[2 * s + 1] = n1;
+        s_wbase = atomicAdd(&pc.wincount[pc.epoch], (unsigned)(wmx + wmn));
+    }
+    __syncthreads();
+    const unsigned wbase = s_wbase;
+    for (int w = tid; w < wmx + wmn; w += kPT) {
+        const int type = w >= wmx;
+        const int wi = type ? w - wmx : w;
+        pc.winlist[2 * (wbase + w)] = s;
+        pc.winlist[2 * (wbase + w) + 1] = (wi << 1) | type;
+    }
+}
+
 // ------------------------------------------------------------- stage init
 
 template <typename real>
@@ -2495,8 +2625,12 @@ template <typename real>
 hipError_t launch_knot_prefix(const SeriesDev& sd, const KnotsDev<real>& k, const PassCtl& pc,
                               const int* active, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_knot_prefix_b<real>), dim3(sd.nseries), dim3(kPT), 0, st, sd, k, pc,
-                       active);
+    if (sd.max_tiles <= kPT)
+        hipLaunchKernelGGL((k_knot_prefix_1<real>), dim3(sd.nseries), dim3(kPT), 0, st, sd, k, pc,
+                           active);
+    else
+        hipLaunchKernelGGL((k_knot_prefix_b<real>), dim3(sd.nseries), dim3(kPT), 0, st, sd, k, pc,
+                           active);
     return hipGetLastError();
 }
 

@@ -166,7 +166,7 @@ struct DevSeries {
     int *tile_order = nullptr, *tile_series = nullptr, *tile0 = nullptr, *ntl = nullptr, *len = nullptr, *chan = nullptr,
         *nidx = nullptr;
     int64_t *off = nullptr, *kbase = nullptr, *eoff = nullptr;
-    int nseries = 0, ntiles = 0, max_len = 0;
+    int nseries = 0, ntiles = 0, max_len = 0, max_tiles = 0;
     int64_t samples = 0, kcap = 0, maxwin = 0;
     SeriesDev view(int64_t kstride, int64_t sstride, int cap) const
     {
@@ -176,6 +176,7 @@ struct DevSeries {
         v.tile_order = tile_order; v.tile_series = tile_series; v.tile0 = tile0; v.ntl = ntl; v.off = off; v.len = len;
         v.kbase = kbase; v.chan = chan; v.eoff = eoff; v.nidx = nidx;
         v.nseries = nseries; v.ntiles = ntiles; v.kstride = kstride;
+        v.max_tiles = max_tiles;
         return v;
     }
 };
@@ -318,6 +319,8 @@ struct PlanBase {
         if ((rc = upload(&d.eoff, h.eoff))) return rc;
         d.nseries = (int)h.len.size();
         d.ntiles = h.ntiles;
+        d.max_tiles = 0;
+        for (int nt : h.ntl) d.max_tiles = std::max(d.max_tiles, nt);
         d.max_len = h.max_len;
         d.samples = h.samples;
         d.kcap = h.kcap;

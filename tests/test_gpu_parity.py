@@ -166,3 +166,16 @@ def test_not_a_knot_fp64_bitexact(icm, orc, dev, stop_mode):
         with pytest.raises(icm.ICMError):
             plan.backward([torch.zeros((M, N), dtype=torch.float64, device=dev)])
     plan.close()
+
+
+def test_long_series_bitexact(icm, orc, dev):
+    """A series of more than 1024 tiles (300,000 samples) takes the chunked
+    knot-prefix kernel; FP64 result equal to the oracle."""
+    import torch
+    N, I, M, S = 300000, 2, 3, 3
+    x = _sig().eeg_like(N, seed=77)
+    print(f"case: long series N={N}", flush=True)
+    w = icm.noise(I, N, seed=5, dtype=torch.float64, device=dev)
+    got = icm.iceemdan(torch.tensor(x, device=dev), I, M, max_sifts=S, noise=w).cpu().numpy()
+    ref = orc.iceemdan(x, w.cpu().numpy(), M, S_or_cap=S)["imfs"]
+    assert np.array_equal(got, ref), relerr(got, ref)
