@@ -1534,8 +1534,10 @@ __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs
                         const int j = cb[ty][r] + lanes_below(bm[ty][r]) + 2;
                         const int g = pre[ty] - 2 + j;
                         real y = cur.h[r];
-                        if (g == 0) y = fx[ty][0];
-                        if (g == nn[ty] - 1) y = fx[ty][1];
+                        if (!FULL) {   // knots 0 and n-1 lie in the series' end tiles
+                            if (g == 0) y = fx[ty][0];
+                            if (g == nn[ty] - 1) y = fx[ty][1];
+                        }
                         tp[ty][j] = p;
                         tc[ty][j].a = y;
                     }
