@@ -42,7 +42,8 @@ class icm_config(C.Structure):
                 ("lengths", C.POINTER(C.c_int64)), ("num_noise", C.c_int),
                 ("num_imfs", C.c_int), ("stop_mode", C.c_int), ("max_sifts", C.c_int),
                 ("noise_max_sifts", C.c_int), ("noise_strength", C.c_double),
-                ("seed", C.c_uint64), ("use_graph", C.c_int), ("boundary", C.c_int)]
+                ("seed", C.c_uint64), ("use_graph", C.c_int), ("boundary", C.c_int),
+                ("keep_records", C.c_int)]
 
 
 class icm_plan_info(C.Structure):
@@ -150,7 +151,7 @@ def _require_cuda(*ts):
 
 def make_config(lengths, num_noise, num_imfs, dtype_code=DTYPE_F64, stop_mode=STOP_FIXED,
                 max_sifts=10, noise_max_sifts=None, noise_strength=0.2, seed=1234, use_graph=1,
-                boundary=BOUNDARY_NATURAL):
+                boundary=BOUNDARY_NATURAL, keep_records=False):
     cfg = icm_config()
     lib().icm_config_default(C.byref(cfg))
     lens = (C.c_int64 * len(lengths))(*[int(v) for v in lengths])
@@ -168,6 +169,7 @@ def make_config(lengths, num_noise, num_imfs, dtype_code=DTYPE_F64, stop_mode=ST
     cfg.seed = int(seed)
     cfg.use_graph = int(use_graph)
     cfg.boundary = int(boundary)
+    cfg.keep_records = 1 if keep_records else 0
     return cfg, lens
 
 
@@ -192,7 +194,7 @@ class Plan:
 
     def __init__(self, lengths, num_noise, num_imfs, dtype=None, stop_mode=STOP_FIXED,
                  max_sifts=10, noise_max_sifts=None, noise_strength=0.2, seed=1234,
-                 use_graph=True, device=0, boundary=BOUNDARY_NATURAL):
+                 use_graph=True, device=0, boundary=BOUNDARY_NATURAL, keep_records=False):
         torch = _torch()
         if dtype is None:
             dtype = torch.float64
@@ -204,7 +206,7 @@ class Plan:
         self.device = torch.device("cuda", device) if isinstance(device, int) else device
         cfg, self._lens = make_config(self.lengths, num_noise, num_imfs, _dtype_code(dtype),
                                       stop_mode, max_sifts, noise_max_sifts, noise_strength,
-                                      seed, 1 if use_graph else 0, boundary)
+                                      seed, 1 if use_graph else 0, boundary, keep_records)
         self._cfg = cfg
         h = C.c_void_p()
         with torch.cuda.device(self.device):

@@ -415,6 +415,10 @@ int64_t plan_bytes_estimate(const icm_config* c)
     b += (int64_t)kEpochs * 4 + 2 * P.maxwin * 4;
     b += 5 * P.chan_total * s + P.chan_total * M * s;
     b += (int64_t)(M - 1) * P.chan_total * s;     // residue history
+    if (c->keep_records && c->stop_mode == ICM_STOP_FIXED)   // per-pass knot records
+        b += (int64_t)(M - 1) * c->max_sifts *
+             (2 * P.kstride * 4 + 4 * (int64_t)P.hsig.ntiles * 4 + 6 * (int64_t)P.hsig.len.size() * 4 +
+              2 * P.hsig.maxwin * 4);
     return b;
 }
 
@@ -444,7 +448,9 @@ struct Plan : PlanBase {
         int* winlist = nullptr;
         unsigned* wincount = nullptr;
     };
-    std::vector<Rec> rec;
+    std::vector<Rec> rec;                  // re-sift records of one stage (backward)
+    std::vector<std::vector<Rec>> krec;    // per stage, kept by the forward (keep_records)
+    std::vector<char> krec_valid;
     real *mu = nullptr, *mom = nullptr, *edge = nullptr, *ga = nullptr, *gr = nullptr,
          *gacc = nullptr, *dots = nullptr, *gy = nullptr;
 
@@ -492,6 +498,12 @@ struct Plan : PlanBase {
             if ((rc = alloc(&subs, nser_max))) return rc;
             if ((rc = alloc(&sx_cnt, ntiles_max))) return rc;
             if ((rc = alloc(&sx_mark, ntiles_max))) return rc;
+        }
+        if (cfg.keep_records && cfg.stop_mode == ICM_STOP_FIXED) {
+            krec.resize(M - 1);
+            for (int k = 0; k < M - 1; ++k)
+                if ((rc = alloc_recs(krec[k]))) return rc;
+            krec_valid.assign(M - 1, 0);
         }
         if (hipHostMalloc((void**)&h_n_active, sizeof(unsigned), hipHostMallocDefault) != hipSuccess)
             return ICM_ERR_OUT_OF_MEMORY;
@@ -556,7 +568,7 @@ struct Plan : PlanBase {
     // The stage output (h after S subtractions) ends in `out` if given
     // (series offsets), else in buffer S % 2.
     int stage_fixed(int phase, int stage, const DevSeries& d, const InitArgs<real>& ia, int S,
-                    real* out, hipStream_t st, Rec* record = nullptr)
+                    real* out, hipStream_t st, Rec* record = nullptr, bool full = false)
     {
         const SeriesDev sd = d.view(kstride, sstride, cap);
         int rc;
@@ -593,7 +605,8 @@ struct Plan : PlanBase {
             if ((rc = prof_begin(1, 0, st))) return rc;
             ICM_TRY(launch_solve<real>(sa, solve_grid(), st));
             if ((rc = prof_end(st))) return rc;
-            if (record && j == S - 1) break;   // the stage output itself is not needed
+            // a re-sift for the adjoint does not need the stage output itself
+            if (record && !full && j == S - 1) break;
             SiftArgs<real> a{};
             a.sd = sd;
             a.segs = d.segs;
@@ -750,8 +763,11 @@ struct Plan : PlanBase {
         const int* parity = nullptr;
         int fixed_parity = 0;
         if (cfg.stop_mode == ICM_STOP_FIXED) {
-            if ((rc = stage_fixed(ICM_PHASE_SIGNAL, k, dsig, ia, cfg.max_sifts, nullptr, st)))
+            Rec* R = krec.empty() ? nullptr : krec[k].data();
+            if ((rc = stage_fixed(ICM_PHASE_SIGNAL, k, dsig, ia, cfg.max_sifts, nullptr, st, R,
+                                  true)))
                 return rc;
+            if (R) krec_valid[k] = 1;
             fixed_parity = cfg.max_sifts & 1;
         } else {
             if ((rc = stage_rilling(ICM_PHASE_SIGNAL, k, dsig, ia, cfg.max_sifts, st))) return rc;
@@ -898,6 +914,7 @@ struct Plan : PlanBase {
     {
         *h_err = 0;
         have_forward = false;
+        std::fill(krec_valid.begin(), krec_valid.end(), (char)0);
         ICM_TRY(launch_fill_u32((uint32_t*)err, 0u, 1, st));
         int rc;
         if ((rc = inputs_and_noise(sig, noise, st))) return rc;
@@ -1022,6 +1039,7 @@ struct Plan : PlanBase {
         int rc;
         *h_err = 0;
         have_forward = false;
+        std::fill(krec_valid.begin(), krec_valid.end(), (char)0);
         ICM_TRY(launch_fill_u32((uint32_t*)err, 0u, 1, st));
         if ((rc = inputs_and_noise(nullptr, noise, st))) return rc;
         if ((rc = reset_control(st))) return rc;
@@ -1035,10 +1053,9 @@ struct Plan : PlanBase {
         return ICM_OK;
     }
 
-    int backward_alloc()
+    int alloc_recs(std::vector<Rec>& out)
     {
-        if (!rec.empty()) return ICM_OK;
-        const int S = cfg.max_sifts, M = cfg.num_imfs;
+        const int S = cfg.max_sifts;
         int rc;
         std::vector<Rec> rr(S);
         int* px_all = nullptr;
@@ -1052,6 +1069,15 @@ struct Plan : PlanBase {
             if ((rc = alloc(&rr[j].winlist, 2 * dsig.maxwin))) return rc;
             if ((rc = alloc(&rr[j].wincount, 1))) return rc;
         }
+        out.swap(rr);
+        return ICM_OK;
+    }
+
+    int backward_alloc()
+    {
+        if (mu) return ICM_OK;
+        const int M = cfg.num_imfs;
+        int rc;
         if ((rc = alloc(&mu, dsig.samples))) return rc;
         if ((rc = alloc(&mom, 8 * kstride))) return rc;
         if ((rc = alloc(&edge, 16 * (int64_t)dsig.ntiles))) return rc;
@@ -1060,7 +1086,6 @@ struct Plan : PlanBase {
         if ((rc = alloc(&gacc, chan_total))) return rc;
         if ((rc = alloc(&dots, (int64_t)dsig.nseries))) return rc;
         if ((rc = alloc(&gy, chan_total))) return rc;
-        rec.swap(rr);
         return ICM_OK;
     }
 
@@ -1103,17 +1128,23 @@ struct Plan : PlanBase {
                 ia.chan_sig = rk;
                 ia.scale = sq_chan;
             }
-            if ((rc = stage_fixed(ICM_PHASE_SIGNAL, k, dsig, ia, S, nullptr, st, rec.data())))
-                return rc;
+            Rec* R = nullptr;
+            if (!krec.empty() && krec_valid[k]) {
+                R = krec[k].data();   // kept by the forward
+            } else {
+                if (rec.empty() && (rc = alloc_recs(rec))) return rc;
+                R = rec.data();
+                if ((rc = stage_fixed(ICM_PHASE_SIGNAL, k, dsig, ia, S, nullptr, st, R))) return rc;
+            }
             const real* lam = gr + (int64_t)(k + 1) * chan_total;
             auto rec_of = [&](int j) {
                 PassRec r{};
                 if (j >= 0 && j < S) {
-                    r.px = rec[j].px;
-                    r.cnt = rec[j].cnt;
-                    r.pre = rec[j].pre;
-                    r.n = rec[j].n;
-                    r.fixf = rec[j].fixf;
+                    r.px = R[j].px;
+                    r.cnt = R[j].cnt;
+                    r.pre = R[j].pre;
+                    r.n = R[j].n;
+                    r.fixf = R[j].fixf;
                 }
                 return r;
             };
@@ -1125,8 +1156,8 @@ struct Plan : PlanBase {
                 a.mom = mom;
                 a.edge = edge;
                 a.ga = ga;
-                a.winlist = rec[j].winlist;
-                a.wincount = rec[j].wincount;
+                a.winlist = R[j].winlist;
+                a.wincount = R[j].wincount;
                 a.err = err;
                 return a;
             };

@@ -62,6 +62,9 @@ typedef struct icm_config {
     int use_graph;              /* FIXED only: replay the forward as a captured hipGraph */
     int boundary;               /* ICM_BOUNDARY_*: spline end condition (envelopes with >= 4
                                    knots; fewer knots keep natural ends) */
+    int keep_records;           /* FIXED: the forward keeps every pass's knot structure so that
+                                   icm_backward need not re-sift (memory: about
+                                   (M-1) * max_sifts * 4 * I * sum(N) bytes) */
 } icm_config;
 
 typedef struct icm_plan_s* icm_plan;
