@@ -59,11 +59,15 @@ struct Segment {
     int pad;
 };
 
+// window item of the solve: series, (window << 1) | type, first tile or -1
+constexpr int kWinItem = 3;
+
 struct PassCtl {
     uint32_t epoch;        // launch id (unique within one forward)
-    int* winlist;          // [2 * maxwin] window items for the next solve
+    int* winlist;          // [kWinItem * maxwin] window items for the next solve
     unsigned* wincount;    // [epochs] number of window items
     int K;                 // rows kept per window of the solve
+    int H;                 // halo rows on each side of a window
     int* err;              // device error word
 };
 

@@ -174,108 +174,11 @@ __device__ __forceinline__ LastTwo lt_shfl_up(const LastTwo& v, int o)
     return r;
 }
 
-// Per-series exclusive scan of the tile knot counts (the CUB ExclusiveSum of
-// cudaICEEMDAN.cu:859-883, over tiles instead of samples), the series knot
-// counts, the solve's window items, and for every tile the slots of the two
-// knots before it and of the first knot after it.  One wavefront per series.
-template <typename real>
-__global__ void k_knot_prefix(SeriesDev sd, KnotsDev<real> k, PassCtl pc, const int* active)
-{
-    const int s = blockIdx.x;
-    const int lane = threadIdx.x;
-    if (active && !active[s]) return;
-    const int tb = sd.tile0[s], nt = sd.ntl[s];
-    const int cap = sd.cap;
-    int run[2] = {0, 0};
-    LastTwo carry[2];
-    for (int ty = 0; ty < 2; ++ty) {
-        carry[ty].c = 0;
-        carry[ty].l1 = -1;
-        carry[ty].l2 = -1;
-    }
-    for (int base = 0; base < nt; base += kWave) {
-        const int t = base + lane;
-#pragma unroll
-        for (int ty = 0; ty < 2; ++ty) {
-            const int c = t < nt ? k.cnt[2 * (tb + t) + ty] : 0;
-            int inc = c;
-            for (int o = 1; o < kWave; o <<= 1) {
-                const int v = shfl_up(inc, o);
-                if (lane >= o) inc += v;
-            }
-            LastTwo own;
-            own.c = min(c, 2);
-            own.l1 = c >= 1 ? (int64_t)(tb + t) * cap + (c - 1) : -1;
-            own.l2 = c >= 2 ? (int64_t)(tb + t) * cap + (c - 2) : -1;
-            LastTwo scan = own;
-            for (int o = 1; o < kWave; o <<= 1) {
-                const LastTwo v = lt_shfl_up(scan, o);
-                if (lane >= o) scan = lt_combine(v, scan);
-            }
-            LastTwo ex = lt_shfl_up(scan, 1);
-            if (lane == 0) {
-                ex.c = 0;
-                ex.l1 = -1;
-                ex.l2 = -1;
-            }
-            const LastTwo before = lt_combine(carry[ty], ex);
-            if (t < nt) {
-                k.pre[2 * (tb + t) + ty] = run[ty] + inc - c;
-                const int64_t o = 6 * (int64_t)(tb + t) + 3 * ty;
-                const int64_t so = ty ? sd.sstride : 0;
-                k.nbx[o] = before.l1 >= 0 ? k.x[so + before.l1] : -1;
-                k.nby[o] = before.l1 >= 0 ? k.y[so + before.l1] : (real)0;
-                k.nbx[o + 1] = before.l2 >= 0 ? k.x[so + before.l2] : -1;
-                k.nby[o + 1] = before.l2 >= 0 ? k.y[so + before.l2] : (real)0;
-            }
-            run[ty] += __shfl(inc, kWave - 1, kWave);
-            LastTwo last;
-            last.c = __shfl(scan.c, kWave - 1, kWave);
-            last.l1 = __shfl(scan.l1, kWave - 1, kWave);
-            last.l2 = __shfl(scan.l2, kWave - 1, kWave);
-            carry[ty] = lt_combine(carry[ty], last);
-        }
-    }
-    // first knot after each tile: suffix minimum over non-empty tiles
-    for (int ty = 0; ty < 2; ++ty) {
-        int nxt = 0x7FFFFFFF;
-        for (int top = ((nt - 1) / kWave) * kWave; top >= 0; top -= kWave) {
-            const int t = top + lane;
-            const int c = t < nt ? k.cnt[2 * (tb + t) + ty] : 0;
-            int key = c > 0 ? t : 0x7FFFFFFF;
-            for (int o = 1; o < kWave; o <<= 1) {
-                const int v = shfl_down(key, o);
-                if (lane + o < kWave) key = min(key, v);
-            }
-            int ex = shfl_down(key, 1);
-            if (lane == kWave - 1) ex = 0x7FFFFFFF;
-            const int after = min(ex, nxt);
-            if (t < nt) {
-                const int64_t o = 6 * (int64_t)(tb + t) + 3 * ty + 2;
-                const int64_t sl = (ty ? sd.sstride : 0) + (int64_t)(tb + after) * cap;
-                k.nbx[o] = after == 0x7FFFFFFF ? -1 : k.x[sl];
-                k.nby[o] = after == 0x7FFFFFFF ? (real)0 : k.y[sl];
-            }
-            nxt = min(nxt, __shfl(key, 0, kWave));
-        }
-    }
-    const int wmx = (run[0] + pc.K - 1) / pc.K;
-    const int wmn = (run[1] + pc.K - 1) / pc.K;
-    unsigned wbase = 0;
-    if (lane == 0) {
-        k.n[2 * s] = run[0];
-        k.n[2 * s + 1] = run[1];
-        wbase = atomicAdd(&pc.wincount[pc.epoch], (unsigned)(wmx + wmn));
-    }
-    wbase = __shfl(wbase, 0, kWave);
-    for (int w = lane; w < wmx + wmn; w += kWave) {
-        const int type = w >= wmx;
-        const int wi = type ? w - wmx : w;
-        pc.winlist[2 * (wbase + w)] = s;
-        pc.winlist[2 * (wbase + w) + 1] = (wi << 1) | type;
-    }
-}
-
+// Knot prefix: per-series exclusive scan of the tile knot counts (the CUB
+// ExclusiveSum of cudaICEEMDAN.cu:859-883, over tiles instead of samples),
+// the series knot counts, the solve's window items, and for every tile the
+// slots of the two knots before it and of the first knot after it.
+//
 // The same outputs with one 1024-thread workgroup per series and one tile per
 // thread (chunks of 1024 tiles): workgroup scans forward (counts, last two
 // knots) and backward (next knot), carried from chunk to chunk.
@@ -417,8 +320,9 @@ __global__ __launch_bounds__(kPT) void k_knot_prefix_b(SeriesDev sd, KnotsDev<re
     for (int w = tid; w < wmx + wmn; w += kPT) {
         const int type = w >= wmx;
         const int wi = type ? w - wmx : w;
-        pc.winlist[2 * (wbase + w)] = s;
-        pc.winlist[2 * (wbase + w) + 1] = (wi << 1) | type;
+        pc.winlist[kWinItem * (wbase + w)] = s;
+        pc.winlist[kWinItem * (wbase + w) + 1] = (wi << 1) | type;
+        pc.winlist[kWinItem * (wbase + w) + 2] = -1;   // no tile hint: the solve searches
     }
 }
 
@@ -434,6 +338,7 @@ __global__ __launch_bounds__(kPT) void k_knot_prefix_1(SeriesDev sd, KnotsDev<re
     __shared__ int s_tot[2][NWV], s_ex_tot[2][NWV], s_all[2];
     __shared__ LastTwo s_lt[2][NWV], s_ex_lt[2][NWV];
     __shared__ int s_min[2][NWV], s_ex_min[2][NWV];
+    __shared__ int s_pre[2][kPT];
     __shared__ unsigned s_wbase;
     const int s = blockIdx.x;
     if (active && !active[s]) return;
@@ -525,6 +430,7 @@ __global__ __launch_bounds__(kPT) void k_knot_prefix_1(SeriesDev sd, KnotsDev<re
             const int x3 = after == INF ? -1 : k.x[sl];
             const real y3 = after == INF ? (real)0 : k.y[sl];
             k.pre[2 * (tb + t) + ty] = ex;
+            s_pre[ty][t] = ex;
             const int64_t o = 6 * (int64_t)(tb + t) + 3 * ty;
             k.nbx[o] = x1;
             k.nby[o] = y1;
@@ -547,8 +453,18 @@ __global__ __launch_bounds__(kPT) void k_knot_prefix_1(SeriesDev sd, KnotsDev<re
     for (int w = tid; w < wmx + wmn; w += kPT) {
         const int type = w >= wmx;
         const int wi = type ? w - wmx : w;
-        pc.winlist[2 * (wbase + w)] = s;
-        pc.winlist[2 * (wbase + w) + 1] = (wi << 1) | type;
+        // the last tile whose knots start at or before the window's first
+        // gathered knot, so the solve needs no search
+        const int glo = max(wi * pc.K - pc.H - 1, 0);
+        int x = 0, y = nt - 1;
+        while (x < y) {
+            const int mid = (x + y + 1) >> 1;
+            if (s_pre[type][mid] <= glo) x = mid;
+            else y = mid - 1;
+        }
+        pc.winlist[kWinItem * (wbase + w)] = s;
+        pc.winlist[kWinItem * (wbase + w) + 1] = (wi << 1) | type;
+        pc.winlist[kWinItem * (wbase + w) + 2] = x;
     }
 }
 
@@ -1044,13 +960,13 @@ __device__ __forceinline__ void wave_sync()
 // of one window divided by 64.
 template <typename real, int NIT>
 __device__ void gather_knots(const SeriesDev& sd, const KnotsDev<real>& k, int s, int type,
-                             long lo, int glo, int ghi, int* sp, real* sy, int* tpre)
+                             long lo, int glo, int ghi, int* sp, real* sy, int* tpre, int hint)
 {
     const int lane = threadIdx.x & 63;
     const int tb = sd.tile0[s], ntl = sd.ntl[s];
     const int64_t so = type ? sd.sstride : 0;
-    // 64-ary search for the tile holding knot glo
-    int a = 0, b = ntl;
+    // the tile holding knot glo: given by the prefix kernel, else a 64-ary search
+    int a = hint >= 0 ? hint : 0, b = hint >= 0 ? hint + 1 : ntl;
     while (b - a > 1) {
         const int step = (b - a + kWave - 1) / kWave;
         const int idx = a + lane * step;
@@ -1207,8 +1123,9 @@ __global__ __launch_bounds__(kThreads) void k_solve(SolveArgs<real> a)
     real* sy = s_y[wv];
     const unsigned total = *a.wincount;
     for (unsigned item = blockIdx.x * NW + wv; item < total; item += gridDim.x * NW) {
-        const int s = a.winlist[2 * item];
-        const int code = a.winlist[2 * item + 1];
+        const int s = a.winlist[kWinItem * item];
+        const int code = a.winlist[kWinItem * item + 1];
+        const int hint = a.winlist[kWinItem * item + 2];
         const int type = code & 1;
         const int w = code >> 1;
         if (a.active && !a.active[s]) continue;
@@ -1219,7 +1136,7 @@ __global__ __launch_bounds__(kThreads) void k_solve(SolveArgs<real> a)
         const int glo = (int)max(lo, 0L);
         const int ghi = (int)min(lo + W + 2, (long)n);
         gather_knots<real, (W + 2 + kWave - 1) / kWave>(a.sd, a.k, s, type, lo, glo, ghi, sp, sy,
-                                                         s_tpre[wv]);
+                                                         s_tpre[wv], hint);
         wave_sync();
         // endpoint fix-up when n >= 6 (setBoundaryMax/Min, cudaICEEMDAN.cu:209-291);
         // idempotent, so every window that holds an end applies it
@@ -2172,8 +2089,8 @@ __global__ __launch_bounds__(kThreads) void k_adj_knot(AdjArgs<real> a)
     real* sf = s_f[wv];
     const unsigned total = *a.wincount;
     for (unsigned item = blockIdx.x * NW + wv; item < total; item += gridDim.x * NW) {
-        const int s = a.winlist[2 * item];
-        const int code = a.winlist[2 * item + 1];
+        const int s = a.winlist[kWinItem * item];
+        const int code = a.winlist[kWinItem * item + 1];
         const int type = code & 1;
         const int w = code >> 1;
         const int n0 = a.ps.n[2 * s], n1 = a.ps.n[2 * s + 1];

@@ -412,13 +412,13 @@ int64_t plan_bytes_estimate(const icm_config* c)
     b += 2 * 2 * P.sstride * (4 + s);              // knot slots x,y ping-pong
     b += 2 * P.kstride * s;                        // c
     b += 2 * 2 * 2 * P.ntiles_max * 4 + 2 * 2 * P.nser_max * (4 + 2 * s);
-    b += (int64_t)kEpochs * 4 + 2 * P.maxwin * 4;
+    b += (int64_t)kEpochs * 4 + (int64_t)kWinItem * P.maxwin * 4;
     b += 5 * P.chan_total * s + P.chan_total * M * s;
     b += (int64_t)(M - 1) * P.chan_total * s;     // residue history
     if (c->keep_records && c->stop_mode == ICM_STOP_FIXED)   // per-pass knot records
         b += (int64_t)(M - 1) * c->max_sifts *
              (2 * P.kstride * 4 + 4 * (int64_t)P.hsig.ntiles * 4 + 6 * (int64_t)P.hsig.len.size() * 4 +
-              2 * P.hsig.maxwin * 4);
+              (int64_t)kWinItem * P.hsig.maxwin * 4);
     return b;
 }
 
@@ -476,7 +476,7 @@ struct Plan : PlanBase {
         }
         if ((rc = alloc(&kc, 2 * kstride))) return rc;
         if ((rc = alloc(&wincount, kEpochs))) return rc;
-        if ((rc = alloc(&winlist, 2 * maxwin))) return rc;
+        if ((rc = alloc(&winlist, (int64_t)kWinItem * maxwin))) return rc;
         if ((rc = alloc(&err, 1))) return rc;
         if ((rc = alloc(&n_active, 1))) return rc;
         if ((rc = alloc(&y, chan_total))) return rc;
@@ -522,6 +522,7 @@ struct Plan : PlanBase {
         pc.winlist = winlist;
         pc.wincount = wincount;
         pc.K = solve_K(cfg.dtype);
+        pc.H = solve_H(cfg.dtype);
         pc.err = err;
         return pc;
     }
@@ -599,7 +600,8 @@ struct Plan : PlanBase {
                 ICM_TRY(launch_copy<float>((float*)R.cnt, (const float*)kcnt[b], 2 * (int64_t)d.ntiles, st));
                 ICM_TRY(launch_copy<float>((float*)R.pre, (const float*)kpre[b], 2 * (int64_t)d.ntiles, st));
                 ICM_TRY(launch_copy<float>((float*)R.n, (const float*)nk[b], 2 * (int64_t)d.nseries, st));
-                ICM_TRY(launch_copy<float>((float*)R.winlist, (const float*)winlist, 2 * d.maxwin, st));
+                ICM_TRY(launch_copy<float>((float*)R.winlist, (const float*)winlist,
+                                           (int64_t)kWinItem * d.maxwin, st));
                 ICM_TRY(launch_copy<float>((float*)R.wincount, (const float*)(wincount + e_emit), 1, st));
             }
             if ((rc = prof_begin(1, 0, st))) return rc;
@@ -1066,7 +1068,7 @@ struct Plan : PlanBase {
             if ((rc = alloc(&rr[j].pre, 2 * (int64_t)dsig.ntiles))) return rc;
             if ((rc = alloc(&rr[j].n, 2 * (int64_t)dsig.nseries))) return rc;
             if ((rc = alloc(&rr[j].fixf, 4 * (int64_t)dsig.nseries))) return rc;
-            if ((rc = alloc(&rr[j].winlist, 2 * dsig.maxwin))) return rc;
+            if ((rc = alloc(&rr[j].winlist, (int64_t)kWinItem * dsig.maxwin))) return rc;
             if ((rc = alloc(&rr[j].wincount, 1))) return rc;
         }
         out.swap(rr);
