@@ -1935,18 +1935,19 @@ __global__ __launch_bounds__(kThreads) void k_adj_tile(AdjTileArgs<real> a)
         }
         // run number of sample u: knots of the tile strictly before it
         // (sample 0 of the series joins run 1, the interval (x_0, x_1])
+        // the four samples of a lane lie in knot word ub >> 6: its bits and
+        // the knots of the words before it
+        const int wd = ub >> 6;
+        const unsigned long long wbits = bits[wd];
+        int wbefore = 0;
+#pragma unroll
+        for (int w = 0; w < T / 64; ++w)
+            if (w < wd) wbefore += __popcll(bits[w]);
         auto key_of = [&](int u) -> int {
             if (u >= nt) return NOKEY;
             if (t0 == 0 && u == 0) return 1;
-            int c = 0;
-#pragma unroll
-            for (int w = 0; w < T / 64; ++w) {
-                const unsigned long long b = bits[w];
-                const int lo = w * 64;
-                if (u >= lo + 64) c += __popcll(b);
-                else if (u > lo) c += __popcll(b << (64 - (u - lo)));
-            }
-            return c;
+            const int b = u & 63;
+            return wbefore + (b ? __popcll(wbits << (64 - b)) : 0);
         };
         auto dst_of = [&](int r) -> real* {
             if (r >= 1 && r < cnt) return a.mom + 4 * (kb + pre + r - 1);
