@@ -65,6 +65,58 @@ __device__ __forceinline__ double shift_next(double v, double fill)
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// DPP row moves (GFX9 encodings): row_shr:n = 0x110 + n shifts inside each
+// 16-lane row, row_bcast:15 = 0x142 sends lane 15 of a row to the next row,
+// row_bcast:31 = 0x143 sends lane 31 to rows 2 and 3.  Lanes with no source
+// lane, or in a row left out of ROWS, keep their own `fill`.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_row_i(int v, int fill)
+{
+    return __builtin_amdgcn_update_dpp(fill, v, CTRL, ROWS, 0xF, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_row(float v, float fill)
+{
+    return __int_as_float(dpp_row_i<CTRL, ROWS>(__float_as_int(v), __float_as_int(fill)));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_row(double v, double fill)
+{
+    const long long b = __double_as_longlong(v), f = __double_as_longlong(fill);
+    const int lo = dpp_row_i<CTRL, ROWS>((int)b, (int)f);
+    const int hi = dpp_row_i<CTRL, ROWS>((int)(b >> 32), (int)(f >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+template <int CTRL, int ROWS, typename real, int NV>
+__device__ __forceinline__ void seg_scan_step(real (&v)[NV], int key)
+{
+    // a lane that receives nothing sees key + 1, which never matches
+    const bool take = dpp_row_i<CTRL, ROWS>(key, key + 1) == key;
+#pragma unroll
+    for (int z = 0; z < NV; ++z) {
+        const real w = dpp_row<CTRL, ROWS>(v[z], (real)0);
+        v[z] = take ? v[z] + w : v[z];
+    }
+}
+
+// Inclusive scan of NV values per lane over the runs of equal `key`, for
+// keys that do not decrease with the lane number: lane i ends with the sum
+// over the lanes j <= i of its run.  Steps of 1, 2, 4, 8 lanes inside each
+// row, then the last lane of row 0 (2) into row 1 (3), then lane 31 into
+// rows 2 and 3; with non-decreasing keys, "the source lane has my key"
+// means every lane between belongs to my run.  Every lane executes it.
+template <typename real, int NV>
+__device__ __forceinline__ void seg_scan_keys(real (&v)[NV], int key)
+{
+    seg_scan_step<0x111, 0xF>(v, key);
+    seg_scan_step<0x112, 0xF>(v, key);
+    seg_scan_step<0x114, 0xF>(v, key);
+    seg_scan_step<0x118, 0xF>(v, key);
+    seg_scan_step<0x142, 0xA>(v, key);
+    seg_scan_step<0x143, 0xC>(v, key);
+}
+
 // value of lane `l` (wave-uniform result)
 __device__ __forceinline__ float lane_value(float v, int l)
 {

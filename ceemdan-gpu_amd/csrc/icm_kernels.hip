@@ -1997,25 +1997,16 @@ __global__ __launch_bounds__(kThreads) void k_adj_tile(AdjTileArgs<real> a)
                 for (int z = 0; z < 4; ++z) cur[z] = m[q][z];
             }
         }
-        // segmented inclusive scan of the last runs over the lanes
+        // segmented inclusive scan of the last runs over the lanes (the
+        // last-run keys do not decrease with the lane)
         real sc[4] = {cur[0], cur[1], cur[2], cur[3]};
-#pragma unroll
-        for (int o = 1; o < kWave; o <<= 1) {
-            const int ku = shfl_up(kcur, o);
-            const bool take = lane >= o && ku == kcur;
-#pragma unroll
-            for (int z = 0; z < 4; ++z) {
-                const real w = shfl_up(sc[z], o);
-                if (take) sc[z] = sc[z] + w;
-            }
-        }
-        // shuffles run on every lane (a disabled source lane gives no defined value)
-        const int kup = shfl_up(kcur, 1);
-        const int kprev = lane > 0 ? kup : NOKEY + 1;
+        seg_scan_keys(sc, kcur);
+        // lane moves run on every lane
+        const int kprev = dpp_prev_i(kcur, NOKEY + 1);
         real prevs[4];
 #pragma unroll
-        for (int z = 0; z < 4; ++z) prevs[z] = shfl_up(sc[z], 1);
-        const int knext_first = shfl_down(kfirst, 1);
+        for (int z = 0; z < 4; ++z) prevs[z] = shift_prev(sc[z], (real)0);
+        const int knext_first = dpp_next_i(kfirst, NOKEY + 1);
         if (closed && kfirst != NOKEY) {
             // the first run ends inside this lane
             real tot[4];
