@@ -15,6 +15,15 @@ constexpr int kWave = 64;
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
+// Index of this wavefront in its workgroup as a scalar value.  The compiler
+// cannot prove threadIdx.x >> 6 uniform over the wave; without this, every
+// per-wave quantity derived from it (work item, series, base pointers,
+// lengths) would be loaded per lane and kept in vector registers.
+__device__ __forceinline__ int wave_index()
+{
+    return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+
 // number of set bits of `mask` below this lane
 __device__ __forceinline__ int lanes_below(uint64_t mask)
 {
@@ -116,6 +125,18 @@ __device__ __forceinline__ void seg_scan_keys(real (&v)[NV], int key)
     seg_scan_step<0x142, 0xA>(v, key);
     seg_scan_step<0x143, 0xC>(v, key);
 }
+
+// Wave-uniform load through the scalar data cache (constant address space).
+// Only for data that the running kernel does not write.
+template <typename T>
+__device__ __forceinline__ T load_scalar(const T* p)
+{
+    return *(const __attribute__((address_space(4))) T*)p;
+}
+
+// Wait until every vector memory access of this wave has completed
+// (s_waitcnt vmcnt(0); expcnt and lgkmcnt left at their maxima).
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 // value of lane `l` (wave-uniform result)
 __device__ __forceinline__ float lane_value(float v, int l)

@@ -572,10 +572,10 @@ __global__ __launch_bounds__(kThreads) void k_sift(SiftArgs<real> a)
 
     auto load_meta = [&](int t, TileMeta& m) {
         const int gt = tb + t;
-        m.pre[0] = a.kin.pre[2 * gt];
-        m.pre[1] = a.kin.pre[2 * gt + 1];
-        m.cnt[0] = a.kin.cnt[2 * gt];
-        m.cnt[1] = a.kin.cnt[2 * gt + 1];
+        m.pre[0] = load_scalar(a.kin.pre + 2 * gt);
+        m.pre[1] = load_scalar(a.kin.pre + 2 * gt + 1);
+        m.cnt[0] = load_scalar(a.kin.cnt + 2 * gt);
+        m.cnt[1] = load_scalar(a.kin.cnt + 2 * gt + 1);
     };
     auto load_tile = [&](int t, const TileMeta& m, real* hr, real* cr, int* nbx, real* nby) {
         const int t0 = t * T;
@@ -1118,7 +1118,7 @@ __global__ __launch_bounds__(kThreads) void k_solve(SolveArgs<real> a)
     __shared__ int s_pos[NW][W + 2];
     __shared__ real s_y[NW][W + 2];
     __shared__ int s_tpre[NW][kWave];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wv = wave_index(), lane = threadIdx.x & 63;
     int* sp = s_pos[wv];
     real* sy = s_y[wv];
     const unsigned total = *a.wincount;
@@ -1328,7 +1328,7 @@ __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs
     constexpr int NW = kThreads / kWave;
     __shared__ int s_tp[NW][2][kWKC];
     __shared__ Coef<real> s_tc[NW][2][kWKC];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = wave_index();
     const int sid = blockIdx.x * NW + wv;
     if (sid >= nsegs) return;
     const SeriesDev& sd = a.sd;
@@ -1378,20 +1378,24 @@ __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs
     int mpre[2], mcnt[2], npre[2] = {0, 0}, ncnt[2] = {0, 0};
     {
         const int gt = tb + seg.t_first;
-        mpre[0] = a.kin.pre[2 * gt];
-        mpre[1] = a.kin.pre[2 * gt + 1];
-        mcnt[0] = a.kin.cnt[2 * gt];
-        mcnt[1] = a.kin.cnt[2 * gt + 1];
+        mpre[0] = load_scalar(a.kin.pre + 2 * gt);
+        mpre[1] = load_scalar(a.kin.pre + 2 * gt + 1);
+        mcnt[0] = load_scalar(a.kin.cnt + 2 * gt);
+        mcnt[1] = load_scalar(a.kin.cnt + 2 * gt + 1);
     }
     if (seg.count > 1) {
         const int gt = tb + seg.t_first + 1;
-        npre[0] = a.kin.pre[2 * gt];
-        npre[1] = a.kin.pre[2 * gt + 1];
-        ncnt[0] = a.kin.cnt[2 * gt];
-        ncnt[1] = a.kin.cnt[2 * gt + 1];
+        npre[0] = load_scalar(a.kin.pre + 2 * gt);
+        npre[1] = load_scalar(a.kin.pre + 2 * gt + 1);
+        ncnt[0] = load_scalar(a.kin.cnt + 2 * gt);
+        ncnt[1] = load_scalar(a.kin.cnt + 2 * gt + 1);
     }
     WTile<real> cur, nxt;
     load(seg.t_first, mpre, mcnt, cur);
+    // the first tile is needed at once; waiting here also tells the compiler
+    // that nothing loaded before the loop is outstanding inside it, so the
+    // loop body never waits on the prefetch of the next tile
+    wait_vmem();
 
     for (int i = 0; i < seg.count; ++i) {
         const int t = seg.t_first + i;
@@ -1404,10 +1408,10 @@ __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs
             load(t + 1, npre, ncnt, nxt);
             if (i + 2 < seg.count) {
                 const int g2 = gt + 2;
-                n2pre[0] = a.kin.pre[2 * g2];
-                n2pre[1] = a.kin.pre[2 * g2 + 1];
-                n2cnt[0] = a.kin.cnt[2 * g2];
-                n2cnt[1] = a.kin.cnt[2 * g2 + 1];
+                n2pre[0] = load_scalar(a.kin.pre + 2 * g2);
+                n2pre[1] = load_scalar(a.kin.pre + 2 * g2 + 1);
+                n2cnt[0] = load_scalar(a.kin.cnt + 2 * g2);
+                n2cnt[1] = load_scalar(a.kin.cnt + 2 * g2 + 1);
             }
         }
 
@@ -1603,7 +1607,7 @@ __global__ __launch_bounds__(kThreads) void k_init_w(SeriesDev sd, InitArgs<real
                                                      KnotsDev<real> kout)
 {
     using P = typename Promote<real>::type;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = wave_index();
     const int gt = blockIdx.x * (kThreads / kWave) + wv;
     if (gt >= sd.ntiles) return;
     const int s = sd.tile_series[gt];
@@ -1861,7 +1865,7 @@ __global__ __launch_bounds__(kThreads) void k_adj_tile(AdjTileArgs<real> a)
     __shared__ real s_delta[NW][T];
     __shared__ int s_tp[NW][KC];
     __shared__ unsigned long long s_bits[NW][T / 64];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = wave_index();
     const int gt = blockIdx.x * NW + wv;
     const SeriesDev& sd = a.sd;
     if (gt >= sd.ntiles) return;
@@ -2075,7 +2079,7 @@ __global__ __launch_bounds__(kThreads) void k_adj_knot(AdjArgs<real> a)
     __shared__ int s_pos[NW][W + 4];       // knots lo-1 .. lo+W+2 (index + 1)
     __shared__ real s_m[NW][W + 4][4];     // moments of the interval starting at the knot
     __shared__ real s_f[NW][W + 2];        // gc, then the solution gf
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wv = wave_index(), lane = threadIdx.x & 63;
     int* sp = s_pos[wv];
     real(*sm)[4] = s_m[wv];
     real* sf = s_f[wv];
