@@ -143,6 +143,12 @@ __device__ __forceinline__ float lane_value(float v, int l)
 {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
+__device__ __forceinline__ uint64_t lane_value(uint64_t v, int l)
+{
+    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, l);
+    const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+    return ((uint64_t)hi << 32) | lo;
+}
 __device__ __forceinline__ double lane_value(double v, int l)
 {
     const long long b = __double_as_longlong(v);

@@ -48,6 +48,8 @@ struct KnotsDev {
     real* fix;   // [nseries][2][2] values of knots 0 and n-1 after the endpoint fix (k_solve)
     int* nbx;    // [ntiles][2][3] positions of the knots pre-1, pre-2 and pre+cnt of the
     real* nby;   //  tile (-1 = none) and their values (k_knot_prefix)
+    uint64_t* mask;   // [ntiles][2][4] 256-sample tiles: knot bits of the tile's four
+                      //  64-sample rows per type, written with the knots
 };
 
 // A run of consecutive tiles of one series, sifted by one workgroup that

@@ -1312,6 +1312,16 @@ constexpr int kWR = kWT / kWave;       // 64-sample chunks per tile
 constexpr int kWKC = kWT / 2 + 8;      // table entries per type (>= cnt + 3)
 constexpr int kWNC = (kWKC + kWave - 1) / kWave;
 
+// Lane 0 stores the eight wave-uniform row masks m[type][row] (no lane-
+// dependent indexing, which would put the array in scratch memory).
+__device__ __forceinline__ void store_masks(uint64_t* dst, const uint64_t (&m)[2][kWR], int lane)
+{
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < 2 * kWR; ++q) dst[q] = m[q / kWR][q % kWR];
+    }
+}
+
 template <typename real>
 struct WTile {
     real h[kWR];
@@ -1319,10 +1329,11 @@ struct WTile {
     real c[2][kWNC];      // c of knots pre-2+j, j = lane + 64 k
     int nbx;              // lanes 0..5: positions of knots pre-1, pre-2, pre+cnt per type
     real nby;
+    uint64_t mk;          // lanes 0..7: knot bitmasks [type][row] of h (other lanes: copies)
 };
 
 
-template <typename real>
+template <typename real, bool RILL>
 __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs)
 {
     constexpr int NW = kThreads / kWave;
@@ -1373,6 +1384,7 @@ __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs
             w.nbx = a.kin.nbx[o];
             w.nby = a.kin.nby[o];
         }
+        w.mk = a.kin.mask[8 * (int64_t)(tb + t) + (lane & (2 * kWR - 1))];
     };
 
     int mpre[2], mcnt[2], npre[2] = {0, 0}, ncnt[2] = {0, 0};
@@ -1419,27 +1431,20 @@ __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs
         // take a copy of the body without the per-lane end tests
         auto tile = [&](auto full_c) -> bool {
             constexpr bool FULL = decltype(full_c)::value;
-            // ---- knot flags of h (find_extrema_shfl_*, cudaICEEMDAN.cu:92-148)
+            // ---- knot flags of h (find_extrema_shfl_*, cudaICEEMDAN.cu:92-148):
+            // the previous pass (or the stage start) found them when it wrote
+            // h, from the same neighbour values, and stored them as bitmasks
             uint64_t bm[2][kWR];
             int cb[2][kWR];
             int tot[2] = {0, 0};
 #pragma unroll
-            for (int r = 0; r < kWR; ++r) {
-                const int u = r * kWave + lane;
-                const int p = t0 + u;
-                const real lft = shift_prev(cur.h[r], r == 0 ? cur.hl : lane_value(cur.h[r > 0 ? r - 1 : 0], kWave - 1));
-                const real rgt = shift_next(cur.h[r], r == kWR - 1 ? cur.hr : lane_value(cur.h[r < kWR - 1 ? r + 1 : r], 0));
-                const bool valid = FULL || u < nt;
-                const bool end = !FULL && (p == 0 || p == N - 1);
-                const real v = cur.h[r];
-                bm[0][r] = wave_ballot(valid && (end || ((v > lft) && (v > rgt))));
-                bm[1][r] = wave_ballot(valid && (end || ((v < lft) && (v < rgt))));
+            for (int ty = 0; ty < 2; ++ty)
 #pragma unroll
-                for (int ty = 0; ty < 2; ++ty) {
+                for (int r = 0; r < kWR; ++r) {
+                    bm[ty][r] = lane_value(cur.mk, ty * kWR + r);
                     cb[ty][r] = tot[ty];
                     tot[ty] += __popcll(bm[ty][r]);
                 }
-            }
             if (tot[0] != cnt[0] || tot[1] != cnt[1] || cnt[0] + 3 > kWKC || cnt[1] + 3 > kWKC) {
                 if (lane == 0) atomicOr(a.pc.err, ICM_DEVERR_KNOTCAP);
                 return false;
@@ -1451,7 +1456,7 @@ __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs
                 const int p = t0 + r * kWave + lane;
 #pragma unroll
                 for (int ty = 0; ty < 2; ++ty) {
-                    if ((bm[ty][r] >> lane) & 1ull) {
+                    if (__builtin_amdgcn_inverse_ballot_w64(bm[ty][r])) {
                         const int j = cb[ty][r] + lanes_below(bm[ty][r]) + 2;
                         const int g = pre[ty] - 2 + j;
                         real y = cur.h[r];
@@ -1518,14 +1523,14 @@ __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs
                     const real mm = use_mean ? (U + L) / (real)2 : (real)0;
                     hn[r] = cur.h[r] - mm;
                     hout[p] = hn[r];
-                    if (a.sx_cnt) {
+                    if (RILL) {
                         const real sx = fabs((U + L) / (U - L));
                         sxc += (sx > (real)0.05) ? 1 : 0;
                         sxm = sxm || (sx > (real)0.5);
                     }
                 }
             }
-            if (a.sx_cnt) {
+            if (RILL) {
                 for (int o = 32; o >= 1; o >>= 1) sxc += __shfl_xor(sxc, o, kWave);
                 const uint64_t mk = wave_ballot(sxm);
                 if (lane == 0) {
@@ -1553,8 +1558,10 @@ __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs
                     hnr = cur.hr - (use_mean ? (U + L) / (real)2 : (real)0);
                 }
                 // ---- knots of h_new into the tile's slots (select_extrema_*, :150-207)
+                // and their row bitmasks (the next pass's knot flags)
                 const int64_t slot0 = (int64_t)gt * sd.cap;
                 int run[2] = {0, 0};
+                uint64_t em[2][kWR];
 #pragma unroll
                 for (int r = 0; r < kWR; ++r) {
                     const int u = r * kWave + lane;
@@ -1564,18 +1571,20 @@ __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs
                     const bool valid = FULL || u < nt;
                     const bool end = !FULL && (p == 0 || p == N - 1);
                     const real v = hn[r];
-                    const uint64_t fm[2] = {wave_ballot(valid && (end || ((v > lft) && (v > rgt)))),
-                                            wave_ballot(valid && (end || ((v < lft) && (v < rgt))))};
+                    const bool f[2] = {valid && (end || ((v > lft) && (v > rgt))),
+                                       valid && (end || ((v < lft) && (v < rgt)))};
 #pragma unroll
                     for (int ty = 0; ty < 2; ++ty) {
-                        if ((fm[ty] >> lane) & 1ull) {
-                            const int64_t o = (ty ? sd.sstride : 0) + slot0 + run[ty] + lanes_below(fm[ty]);
+                        em[ty][r] = wave_ballot(f[ty]);
+                        if (f[ty]) {
+                            const int64_t o = (ty ? sd.sstride : 0) + slot0 + run[ty] + lanes_below(em[ty][r]);
                             a.kout.x[o] = p;
                             a.kout.y[o] = v;
                         }
-                        run[ty] += __popcll(fm[ty]);
+                        run[ty] += __popcll(em[ty][r]);
                     }
                 }
+                store_masks(a.kout.mask + 8 * (int64_t)gt, em, lane);
                 if (lane == 0) {
                     a.kout.cnt[2 * gt] = run[0];
                     a.kout.cnt[2 * gt + 1] = run[1];
@@ -1649,6 +1658,7 @@ __global__ __launch_bounds__(kThreads) void k_init_w(SeriesDev sd, InitArgs<real
     const real xr = t0 + kWT < N ? val(t0 + kWT) : (real)0;
     const int64_t slot0 = (int64_t)gt * sd.cap;
     int run[2] = {0, 0};
+    uint64_t em[2][kWR];
 #pragma unroll
     for (int r = 0; r < kWR; ++r) {
         const int u = r * kWave + lane;
@@ -1658,18 +1668,20 @@ __global__ __launch_bounds__(kThreads) void k_init_w(SeriesDev sd, InitArgs<real
         const bool valid = u < nt;
         const bool end = p == 0 || p == N - 1;
         const real v = x[r];
-        const uint64_t fm[2] = {wave_ballot(valid && (end || ((v > lft) && (v > rgt)))),
-                                wave_ballot(valid && (end || ((v < lft) && (v < rgt))))};
+        const bool f[2] = {valid && (end || ((v > lft) && (v > rgt))),
+                           valid && (end || ((v < lft) && (v < rgt)))};
 #pragma unroll
         for (int ty = 0; ty < 2; ++ty) {
-            if ((fm[ty] >> lane) & 1ull) {
-                const int64_t o = (ty ? sd.sstride : 0) + slot0 + run[ty] + lanes_below(fm[ty]);
+            em[ty][r] = wave_ballot(f[ty]);
+            if (f[ty]) {
+                const int64_t o = (ty ? sd.sstride : 0) + slot0 + run[ty] + lanes_below(em[ty][r]);
                 kout.x[o] = p;
                 kout.y[o] = v;
             }
-            run[ty] += __popcll(fm[ty]);
+            run[ty] += __popcll(em[ty][r]);
         }
     }
+    store_masks(kout.mask + 8 * (int64_t)gt, em, lane);
     if (lane == 0) {
         kout.cnt[2 * gt] = run[0];
         kout.cnt[2 * gt + 1] = run[1];
@@ -2554,8 +2566,12 @@ hipError_t launch_sift(const SiftArgs<real>& a, int nsegs, hipStream_t st)
 {
     const int T = 2 * (a.sd.cap - 2);
     if (T == 256)
-        hipLaunchKernelGGL((k_sift_w<real>), dim3(blocks_for(nsegs, kThreads / kWave)),
-                           dim3(kThreads), 0, st, a, nsegs);
+        if (a.sx_cnt)
+            hipLaunchKernelGGL((k_sift_w<real, true>), dim3(blocks_for(nsegs, kThreads / kWave)),
+                               dim3(kThreads), 0, st, a, nsegs);
+        else
+            hipLaunchKernelGGL((k_sift_w<real, false>), dim3(blocks_for(nsegs, kThreads / kWave)),
+                               dim3(kThreads), 0, st, a, nsegs);
     else if (T == 512)
         hipLaunchKernelGGL((k_sift<real, 512>), dim3(nsegs), dim3(kThreads), 0, st, a);
     else if (T == 1024)

@@ -412,6 +412,7 @@ int64_t plan_bytes_estimate(const icm_config* c)
     b += 2 * 2 * P.sstride * (4 + s);              // knot slots x,y ping-pong
     b += 2 * P.kstride * s;                        // c
     b += 2 * 2 * 2 * P.ntiles_max * 4 + 2 * 2 * P.nser_max * (4 + 2 * s);
+    b += 2 * P.ntiles_max * (6 * (4 + s) + 8 * 8);  // neighbour knots, knot bitmasks
     b += (int64_t)kEpochs * 4 + (int64_t)kWinItem * P.maxwin * 4;
     b += 5 * P.chan_total * s + P.chan_total * M * s;
     b += (int64_t)(M - 1) * P.chan_total * s;     // residue history
@@ -435,6 +436,7 @@ struct Plan : PlanBase {
     real* kfix[2] = {nullptr, nullptr};
     int* knbx[2] = {nullptr, nullptr};
     real* knby[2] = {nullptr, nullptr};
+    uint64_t* kmask[2] = {nullptr, nullptr};
     real* kc = nullptr;
     real *y = nullptr, *xt = nullptr, *r = nullptr, *imf = nullptr;
     real *var_chan = nullptr, *var_noise = nullptr;
@@ -473,6 +475,7 @@ struct Plan : PlanBase {
             if ((rc = alloc(&kfix[b], 4 * nser_max))) return rc;
             if ((rc = alloc(&knbx[b], 6 * ntiles_max))) return rc;
             if ((rc = alloc(&knby[b], 6 * ntiles_max))) return rc;
+            if ((rc = alloc(&kmask[b], 8 * ntiles_max))) return rc;
         }
         if ((rc = alloc(&kc, 2 * kstride))) return rc;
         if ((rc = alloc(&wincount, kEpochs))) return rc;
@@ -537,6 +540,7 @@ struct Plan : PlanBase {
         k.fix = kfix[b];
         k.nbx = knbx[b];
         k.nby = knby[b];
+        k.mask = kmask[b];
         return k;
     }
 
