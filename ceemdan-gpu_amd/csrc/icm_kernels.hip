@@ -2088,13 +2088,10 @@ __global__ __launch_bounds__(kThreads) void k_adj_knot(AdjArgs<real> a)
     constexpr int W = kWave * B;
     constexpr int K = W - 2 * H;
     constexpr int NW = kThreads / kWave;
-    __shared__ int s_pos[NW][W + 4];       // knots lo-1 .. lo+W+2 (index + 1)
-    __shared__ real s_m[NW][W + 4][4];     // moments of the interval starting at the knot
-    __shared__ real s_f[NW][W + 2];        // gc, then the solution gf
+    __shared__ real s_f[NW][W];            // knot-value gradients of the window rows
     const int wv = wave_index(), lane = threadIdx.x & 63;
-    int* sp = s_pos[wv];
-    real(*sm)[4] = s_m[wv];
     real* sf = s_f[wv];
+    const real third = (real)1 / (real)3;
     const unsigned total = *a.wincount;
     for (unsigned item = blockIdx.x * NW + wv; item < total; item += gridDim.x * NW) {
         const int s = a.winlist[kWinItem * item];
@@ -2106,110 +2103,98 @@ __global__ __launch_bounds__(kThreads) void k_adj_knot(AdjArgs<real> a)
         const int n = type ? n1 : n0;
         const int64_t kb = a.sd.kbase[s] + (type ? a.sd.kstride : 0);
         const long g0 = (long)w * K - H;
-        const long lo = g0 - 2;                // LDS index i <-> knot lo + i
-        for (int i = lane; i < W + 4; i += kWave) {
-            const long g = lo + i;
-            if (g >= 0 && g < n) {
-                sp[i] = a.ps.px[kb + g];
-                const real* m = a.mom + 4 * (kb + g);
-                if (g < n - 1) {
-                    sm[i][0] = m[0];
-                    sm[i][1] = m[1];
-                    sm[i][2] = m[2];
-                    sm[i][3] = m[3];
-                } else {
-                    sm[i][0] = sm[i][1] = sm[i][2] = sm[i][3] = (real)0;
-                }
-            }
+        // lane l owns window rows r = l*B + q (knot gb + q); everything a row
+        // needs comes from its own lane's registers or a neighbour lane's
+        const long gb = g0 + (long)lane * B;
+        // positions of knots gb-1 .. gb+B (xk[j] <-> knot gb-1+j), clamped
+        // into the series (values past its ends are never used)
+        int xk[B + 2];
+#pragma unroll
+        for (int j = 0; j < B + 2; ++j) {
+            const long g = gb - 1 + j;
+            xk[j] = a.ps.px[kb + min(max(g, 0L), (long)n - 1)];
         }
-        wave_sync();
-        // gc of window rows r (knot g = g0 + r, LDS index r + 2)
-        for (int r = lane; r < W; r += kWave) {
-            const long g = g0 + r;
-            real gc = (real)0;
-            if (g >= 1 && g <= n - 2) {
-                const int i = r + 2;
-                const real hr = (real)(sp[i + 1] - sp[i]);
-                const real hl = (real)(sp[i] - sp[i - 1]);
-                gc = sm[i][2] - (real)2 * hr / (real)3 * sm[i][1] - sm[i][3] / ((real)3 * hr);
-                gc = gc - hl / (real)3 * sm[i - 1][1] + sm[i - 1][3] / ((real)3 * hl);
+        // interval gb-1+j (j = 0..B): its moments feed its left knot (row j-1)
+        // and its right knot (row j) through the transposed coefficient
+        // formulas (cudaICEEMDAN.cu:394-398).  gc = right-hand side of the
+        // transposed solve, vp = the part of the knot gradient without the
+        // solution; ih = reciprocal interval length.  (The adjoint is compared
+        // to tolerance, not bit for bit: one reciprocal per interval.)
+        real gc[B], vp[B], ih[B + 1];
+        real gr_prev = (real)0, vr_prev = (real)0;
+#pragma unroll
+        for (int j = 0; j <= B; ++j) {
+            const long gg = gb - 1 + j;
+            const bool in = gg >= 0 && gg <= n - 2;
+            const real* m = a.mom + 4 * (kb + (in ? gg : 0));
+            const real m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3];
+            const real h = (real)(xk[j + 1] - xk[j]);
+            const real ihh = in ? (real)1 / h : (real)0;
+            ih[j] = ihh;
+            const real gl = in ? m2 - (real)2 * third * h * m1 - m3 * ihh * third : (real)0;
+            const real gr = in ? m3 * ihh * third - h * third * m1 : (real)0;
+            const real vl = in ? m0 - m1 * ihh : (real)0;
+            const real vr = in ? m1 * ihh : (real)0;
+            if (j >= 1) {
+                const int q = j - 1;
+                const long g = gb + q;
+                gc[q] = (g >= 1 && g <= n - 2) ? gl + gr_prev : (real)0;
+                vp[q] = vl + vr_prev;
             }
-            sf[r] = gc;
+            gr_prev = gr;
+            vr_prev = vr;
         }
-        wave_sync();
         real xs[B];
         partition_pcr<real, B>(
             lane,
             [&](int r, real& dl, real& d, real& du, real& f) {
-                const long g = g0 + r;
+                const int q = r - lane * B;
+                const long g = gb + q;
                 if (g <= 0 || g >= n - 1) {
                     dl = (real)0; d = (real)1; du = (real)0; f = (real)0;
                     return;
                 }
-                const int i = r + 2;
-                const int hl = sp[i] - sp[i - 1];
-                const int hr = sp[i + 1] - sp[i];
+                const int hl = xk[q + 1] - xk[q];
+                const int hr = xk[q + 2] - xk[q + 1];
                 dl = (real)hl;
                 du = (real)hr;
                 d = (real)(2 * (hl + hr));
-                f = sf[r];
+                f = gc[q];
             },
             xs);
-        wave_sync();
-#pragma unroll
-        for (int q = 0; q < B; ++q) sf[lane * B + q] = xs[q];
-        wave_sync();
+        // solution of the rows just outside this lane's block
+        const real xprev = shift_prev(xs[B - 1], (real)0);
+        const real xnext = shift_next(xs[0], (real)0);
         // base knot gradient of every row with both neighbours in the window
-        real gav[B];
 #pragma unroll
         for (int q = 0; q < B; ++q) {
             const int r = lane * B + q;
-            const long g = g0 + r;
+            const long g = gb + q;
             real v = (real)0;
             if (g >= 0 && g < n && r >= 1 && r <= W - 2) {
-                const int i = r + 2;
-                if (g <= n - 2) {
-                    const real hr = (real)(sp[i + 1] - sp[i]);
-                    v = sm[i][0] - sm[i][1] / hr;
-                }
-                if (g >= 1) {
-                    const real hl = (real)(sp[i] - sp[i - 1]);
-                    v = v + sm[i - 1][1] / hl;
-                }
-                if (g - 1 >= 1 && g - 1 <= n - 2) {
-                    const real hl = (real)(sp[i] - sp[i - 1]);
-                    v = v + (real)3 * sf[r - 1] / hl;
-                }
-                if (g >= 1 && g <= n - 2) {
-                    const real hl = (real)(sp[i] - sp[i - 1]);
-                    const real hr = (real)(sp[i + 1] - sp[i]);
-                    v = v - ((real)3 * sf[r] / hr + (real)3 * sf[r] / hl);
-                }
-                if (g + 1 >= 1 && g + 1 <= n - 2) {
-                    const real hr = (real)(sp[i + 1] - sp[i]);
-                    v = v + (real)3 * sf[r + 1] / hr;
-                }
+                const real sm1 = q == 0 ? xprev : xs[q > 0 ? q - 1 : 0];
+                const real sp1 = q == B - 1 ? xnext : xs[q < B - 1 ? q + 1 : q];
+                v = vp[q];
+                if (g - 1 >= 1 && g - 1 <= n - 2) v = v + (real)3 * sm1 * ih[q];
+                if (g >= 1 && g <= n - 2) v = v - (real)3 * xs[q] * (ih[q + 1] + ih[q]);
+                if (g + 1 >= 1 && g + 1 <= n - 2) v = v + (real)3 * sp1 * ih[q + 1];
             }
-            gav[q] = v;
+            sf[r] = v;
         }
-        wave_sync();
-#pragma unroll
-        for (int q = 0; q < B; ++q) sf[lane * B + q] = gav[q];
         wave_sync();
         // endpoint fix: a_0 = a_1 (1 + rho) - a_2 rho when it replaced a_0
         if (lane == 0 && n >= 6) {
+            const int* xp = a.ps.px + kb;
             const long r0 = 0 - g0;
             if (r0 >= 1 && r0 + 2 <= W - 2 && a.ps.fixf[4 * s + 2 * type]) {
-                const int i = (int)r0 + 2;
-                const real rho = (real)(sp[i + 1] - sp[i]) / (real)(sp[i + 2] - sp[i + 1]);
+                const real rho = (real)(xp[1] - xp[0]) / (real)(xp[2] - xp[1]);
                 sf[r0 + 1] = sf[r0 + 1] + sf[r0] * ((real)1 + rho);
                 sf[r0 + 2] = sf[r0 + 2] - sf[r0] * rho;
                 sf[r0] = (real)0;
             }
             const long r1 = (long)(n - 1) - g0;
             if (r1 - 2 >= 1 && r1 <= W - 2 && a.ps.fixf[4 * s + 2 * type + 1]) {
-                const int i = (int)r1 + 2;
-                const real rho = (real)(sp[i] - sp[i - 1]) / (real)(sp[i - 1] - sp[i - 2]);
+                const real rho = (real)(xp[n - 1] - xp[n - 2]) / (real)(xp[n - 2] - xp[n - 3]);
                 sf[r1 - 1] = sf[r1 - 1] + sf[r1] * ((real)1 + rho);
                 sf[r1 - 2] = sf[r1 - 2] - sf[r1] * rho;
                 sf[r1] = (real)0;
