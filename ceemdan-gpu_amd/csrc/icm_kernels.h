@@ -161,17 +161,12 @@ struct AdjTileArgs {
 };
 
 template <typename real>
-hipError_t launch_adjoint_pass(const AdjArgs<real>& a, int solve_grid, hipStream_t st);
-template <typename real>
 hipError_t launch_adj_tile(const AdjTileArgs<real>& a, hipStream_t st);
 template <typename real>
 hipError_t launch_adj_span_knot(const AdjArgs<real>& a, int solve_grid, hipStream_t st);
 template <typename real>
 hipError_t launch_adj_seed(const real* gimf, int M, int64_t N, int64_t stride, real* gr,
                            hipStream_t st);
-template <typename real>
-hipError_t launch_adj_broadcast(const SeriesDev& sd, const real* gr_next, const int64_t* chan_off,
-                                real* mu, int max_len, hipStream_t st);
 template <typename real>
 hipError_t launch_adj_stage(int stage, int nchan, int I, int max_len, const int* chan_len,
                             const int64_t* chan_off, const SeriesDev& sd, const real* gr_next,

@@ -29,127 +29,6 @@ template <typename real> struct Promote { using type = double; };
 
 // ------------------------------------------------------------ tile helpers
 
-template <typename real, int T>
-struct TileGeom {
-    int gt, s, t, N, t0, nt;
-};
-
-template <typename real, int T>
-__device__ __forceinline__ TileGeom<real, T> tile_geom(const SeriesDev& sd)
-{
-    TileGeom<real, T> g;
-    g.gt = sd.tile_order[blockIdx.x];
-    g.s = sd.tile_series[g.gt];
-    g.t = g.gt - sd.tile0[g.s];
-    g.N = sd.len[g.s];
-    g.t0 = g.t * T;
-    g.nt = min(T, g.N - g.t0);
-    return g;
-}
-
-// Extrema flags over the tile samples u = p - t0 in [0, nt), from LDS
-// samples sh[u + 2] (sh covers p in [t0-2, t0+nt]).  mask[type*NCH + chunk].
-// Strict comparison with both neighbours, endpoints always knots
-// (find_extrema_shfl_max/min, cudaICEEMDAN.cu:92-148).
-template <typename real, int T>
-__device__ __forceinline__ void tile_flags(const real* sh, int N, int t0, int nt, uint64_t* mask)
-{
-    constexpr int NCH = T / 64;
-    const int tid = threadIdx.x, lane = tid & 63;
-#pragma unroll
-    for (int r = 0; r < T / kThreads; ++r) {
-        const int u = r * kThreads + tid;
-        const int p = t0 + u;
-        bool fmx = false, fmn = false;
-        if (u < nt) {
-            if (p == 0 || p == N - 1) {
-                fmx = true;
-                fmn = true;
-            } else {
-                const real v = sh[u + 2], lft = sh[u + 1], rgt = sh[u + 3];
-                fmx = (v > lft) && (v > rgt);
-                fmn = (v < lft) && (v < rgt);
-            }
-        }
-        const uint64_t bm = wave_ballot(fmx);
-        const uint64_t bn = wave_ballot(fmn);
-        if (lane == 0) {
-            mask[u >> 6] = bm;
-            mask[NCH + (u >> 6)] = bn;
-        }
-    }
-}
-
-// chunk exclusive prefixes: pre[type*(NCH+1) + chunk], total at [.. + NCH]
-template <int T>
-__device__ __forceinline__ void chunk_prefix(const uint64_t* mask, int* pre)
-{
-    constexpr int NCH = T / 64;
-    const int tid = threadIdx.x;
-    if (tid < 2) {
-        int run = 0;
-        for (int ch = 0; ch < NCH; ++ch) {
-            pre[tid * (NCH + 1) + ch] = run;
-            run += __popcll(mask[tid * NCH + ch]);
-        }
-        pre[tid * (NCH + 1) + NCH] = run;
-    }
-}
-
-template <int T>
-__device__ __forceinline__ int rank_in_tile(const uint64_t* mask, const int* pre, int type, int u)
-{
-    constexpr int NCH = T / 64;
-    const int ch = u >> 6;
-    const uint64_t m = mask[type * NCH + ch];
-    const uint64_t below = (u & 63) ? (m & ((1ull << (u & 63)) - 1ull)) : 0ull;
-    return pre[type * (NCH + 1) + ch] + __popcll(below);
-}
-
-// Knots of the new samples (sh over [t0-2, t0+nt], new values valid on
-// [t0-1, t0+nt]) into the tile's own slots: select_extrema_max/min
-// (cudaICEEMDAN.cu:150-207) without the global scan; k_knot_prefix turns the
-// per-tile counts into series offsets afterwards.
-template <typename real, int T>
-__device__ void emit_tile(const SeriesDev& sd, const KnotsDev<real>& kout,
-                          const TileGeom<real, T>& g, const real* sh, uint64_t* mask, int* cpre)
-{
-    constexpr int NCH = T / 64;
-    const int tid = threadIdx.x;
-    tile_flags<real, T>(sh, g.N, g.t0, g.nt, mask);
-    __syncthreads();
-    chunk_prefix<T>(mask, cpre);
-    __syncthreads();
-    const int64_t slot0 = (int64_t)g.gt * sd.cap;
-#pragma unroll
-    for (int r = 0; r < T / kThreads; ++r) {
-        const int u = r * kThreads + tid;
-        if (u < g.nt) {
-            const int ch = u >> 6;
-            const int bit = u & 63;
-            const real v = sh[u + 2];
-            if ((mask[ch] >> bit) & 1ull) {
-                const int64_t o = slot0 + rank_in_tile<T>(mask, cpre, 0, u);
-                kout.x[o] = g.t0 + u;
-                kout.y[o] = v;
-            }
-            if ((mask[NCH + ch] >> bit) & 1ull) {
-                const int64_t o = sd.sstride + slot0 + rank_in_tile<T>(mask, cpre, 1, u);
-                kout.x[o] = g.t0 + u;
-                kout.y[o] = v;
-            }
-        }
-    }
-    if (tid == 0) {
-        kout.cnt[2 * g.gt] = cpre[NCH];
-        kout.cnt[2 * g.gt + 1] = cpre[(NCH + 1) + NCH];
-    }
-}
-
-// ------------------------------------------------------------- knot prefix
-
-// Summary of the knots seen so far along a series: how many (capped at 2)
-// and the slots of the last two.  combine(A, B) = A followed by B.
 struct LastTwo {
     int c;
     int64_t l1, l2;
@@ -487,36 +366,7 @@ __device__ __forceinline__ real init_value(const InitArgs<real>& ia, const Serie
     return base + (ia.eps0 * e) * (real)ia.scale[c];
 }
 
-template <typename real, int T>
-__global__ __launch_bounds__(kThreads) void k_init(SeriesDev sd, InitArgs<real> ia, real* h_out,
-                                                   KnotsDev<real> kout)
-{
-    constexpr int NCH = T / 64;
-    __shared__ real sh[T + 3];
-    __shared__ uint64_t mask[2 * NCH];
-    __shared__ int cpre[2 * (NCH + 1)];
-    const TileGeom<real, T> g = tile_geom<real, T>(sd);
-    real* out = h_out + sd.off[g.s];
-    for (int v = threadIdx.x; v < g.nt + 2; v += kThreads) {
-        const int p = g.t0 - 1 + v;
-        if (p >= 0 && p < g.N) {
-            const real x = init_value<real>(ia, sd, g.s, p);
-            sh[v + 1] = x;
-            if (v >= 1 && v <= g.nt) out[p] = x;
-        }
-    }
-    __syncthreads();
-    emit_tile<real, T>(sd, kout, g, sh, mask, cpre);
-}
-
 // ---------------------------------------------------------------- sift pass
-
-// Per-tile inputs that depend only on the tile index (prefetched one tile
-// ahead of the samples, two ahead of use).
-struct TileMeta {
-    int pre[2];
-    int cnt[2];
-};
 
 // Interval coefficients S(t) = a + (b + (c + d t) t) t, t = p - pos.
 template <typename real>
@@ -528,379 +378,6 @@ template <typename real>
 __device__ __forceinline__ real horner(const Coef<real>& q, real t)
 {
     return q.a + (q.b + (q.c + q.d * t) * t) * t;
-}
-
-// One workgroup sifts a segment of consecutive tiles of one series.  Lane
-// mapping: main sample u = r*256 + tid of the tile, so the extrema ballots of
-// a wave are the knot flags of its lanes' samples and a sample's interval is
-// (knots before its chunk) + mbcnt(ballot) - 1.
-template <typename real, int T>
-__global__ __launch_bounds__(kThreads) void k_sift(SiftArgs<real> a)
-{
-    constexpr int NR = T / kThreads;                          // main samples per thread
-    constexpr int NCH = T / kWave;                            // 64-sample chunks per tile
-    constexpr int KC = T / 2 + 8;                             // knot slots per type
-    constexpr int NH = (T + 3 + kThreads - 1) / kThreads;     // samples prefetched per thread
-    constexpr int NC = (2 * KC + kThreads - 1) / kThreads;    // c values prefetched per thread
-    __shared__ real sh[T + 3];                 // h over [t0-2, t1]; h_new after the update
-    __shared__ int s_cnt[2][NCH];              // knots per chunk
-    __shared__ int skp[2][KC];                 // knot positions of [klo, khi]
-    __shared__ Coef<real> sco[2][KC];          // knot value (a), c, then b and d
-    __shared__ real scr[2 * KC];               // prefetched c of [lo, hi] of both types
-    __shared__ int snbx[6];
-    __shared__ real snby[6];
-    __shared__ int s_flag[2];
-    __shared__ int s_red[2 * (kThreads / kWave)];
-
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wv = tid >> 6;
-    const SeriesDev& sd = a.sd;
-    const Segment seg = a.segs[blockIdx.x];
-    const int s = seg.s;
-    if (a.active && !a.active[s]) return;
-    const int N = sd.len[s];
-    const int tb = sd.tile0[s];
-    const int64_t kb = sd.kbase[s];
-    const real* h = a.h_in + sd.off[s];
-    real* hout = a.h_out + sd.off[s];
-    const int n0 = a.kin.n[2 * s], n1 = a.kin.n[2 * s + 1];
-    const real fix00 = a.kin.fix[4 * s], fix01 = a.kin.fix[4 * s + 1];
-    const real fix10 = a.kin.fix[4 * s + 2], fix11 = a.kin.fix[4 * s + 3];
-    const bool use_mean = (n0 > 3) && (n1 > 3);
-    const real* c0p = a.c + kb;
-    const real* c1p = a.c + sd.kstride + kb;
-
-    auto load_meta = [&](int t, TileMeta& m) {
-        const int gt = tb + t;
-        m.pre[0] = load_scalar(a.kin.pre + 2 * gt);
-        m.pre[1] = load_scalar(a.kin.pre + 2 * gt + 1);
-        m.cnt[0] = load_scalar(a.kin.cnt + 2 * gt);
-        m.cnt[1] = load_scalar(a.kin.cnt + 2 * gt + 1);
-    };
-    auto load_tile = [&](int t, const TileMeta& m, real* hr, real* cr, int* nbx, real* nby) {
-        const int t0 = t * T;
-        const int nt = min(T, N - t0);
-#pragma unroll
-        for (int k = 0; k < NH; ++k) {
-            const int q = tid + k * kThreads;
-            const int p = t0 - 2 + q;
-            hr[k] = (q < nt + 3 && p >= 0 && p < N) ? h[p] : (real)0;
-        }
-        // c of knots [max(pre-2,0), min(pre+cnt, n-1)] per type, back to back
-        const int lo0 = max(m.pre[0] - 2, 0);
-        const int len0 = min(m.pre[0] + m.cnt[0], n0 - 1) - lo0 + 1;
-        const int lo1 = max(m.pre[1] - 2, 0);
-        const int len1 = min(m.pre[1] + m.cnt[1], n1 - 1) - lo1 + 1;
-#pragma unroll
-        for (int k = 0; k < NC; ++k) {
-            const int j = tid + k * kThreads;
-            real v = (real)0;
-            if (j < len0) v = c0p[lo0 + j];
-            else if (j - len0 < len1) v = c1p[lo1 + (j - len0)];
-            cr[k] = v;
-        }
-        if (tid < 6) {
-            const int64_t o = 6 * (int64_t)(tb + t) + tid;
-            *nbx = a.kin.nbx[o];
-            *nby = a.kin.nby[o];
-        }
-    };
-
-    real hcur[NH], hnext[NH], ccur[NC], cnext[NC];
-    int nbx_cur = 0, nbx_next = 0;
-    real nby_cur = (real)0, nby_next = (real)0;
-    TileMeta mcur, mnext;
-    load_meta(seg.t_first, mcur);
-    if (seg.count > 1) load_meta(seg.t_first + 1, mnext);
-    load_tile(seg.t_first, mcur, hcur, ccur, &nbx_cur, &nby_cur);
-
-    for (int i = 0; i < seg.count; ++i) {
-        const int t = seg.t_first + i;
-        const int gt = tb + t;
-        const int t0 = t * T;
-        const int nt = min(T, N - t0);
-        // stage the prefetched tile, then start fetching the next one
-#pragma unroll
-        for (int k = 0; k < NH; ++k) {
-            const int q = tid + k * kThreads;
-            if (q < T + 3) sh[q] = hcur[k];
-        }
-#pragma unroll
-        for (int k = 0; k < NC; ++k) {
-            const int j = tid + k * kThreads;
-            if (j < 2 * KC) scr[j] = ccur[k];
-        }
-        if (tid < 6) {
-            snbx[tid] = nbx_cur;
-            snby[tid] = nby_cur;
-        }
-        const TileMeta m = mcur;
-        TileMeta mnext2 = mnext;
-        if (i + 1 < seg.count) {
-            load_tile(t + 1, mnext, hnext, cnext, &nbx_next, &nby_next);
-            if (i + 2 < seg.count) load_meta(t + 2, mnext2);
-        }
-        __syncthreads();
-
-        // ---- extrema of h over the tile (find_extrema_shfl_*, :92-148)
-        uint64_t bm[2][NR];
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            const int u = r * kThreads + tid;
-            const int p = t0 + u;
-            bool fmx = false, fmn = false;
-            if (u < nt) {
-                if (p == 0 || p == N - 1) {
-                    fmx = fmn = true;
-                } else {
-                    const real v = sh[u + 2], lft = sh[u + 1], rgt = sh[u + 3];
-                    fmx = (v > lft) && (v > rgt);
-                    fmn = (v < lft) && (v < rgt);
-                }
-            }
-            bm[0][r] = wave_ballot(fmx);
-            bm[1][r] = wave_ballot(fmn);
-            if (lane == 0) {
-                s_cnt[0][r * (kThreads / kWave) + wv] = __popcll(bm[0][r]);
-                s_cnt[1][r * (kThreads / kWave) + wv] = __popcll(bm[1][r]);
-            }
-        }
-        if (tid == 0) {
-            int fmx = 0, fmn = 0;
-            const int p = t0 - 1;
-            if (p == 0) {
-                fmx = fmn = 1;
-            } else if (p > 0) {
-                fmx = (sh[1] > sh[0]) && (sh[1] > sh[2]);
-                fmn = (sh[1] < sh[0]) && (sh[1] < sh[2]);
-            }
-            s_flag[0] = fmx;
-            s_flag[1] = fmn;
-        }
-        __syncthreads();
-
-        // knots before each of this thread's chunks, and the tile totals
-        int cb[2][NR];
-        int tot[2] = {0, 0};
-#pragma unroll
-        for (int r = 0; r < NR; ++r) cb[0][r] = cb[1][r] = 0;
-#pragma unroll
-        for (int ch = 0; ch < NCH; ++ch) {
-            const int x0 = s_cnt[0][ch], x1 = s_cnt[1][ch];
-#pragma unroll
-            for (int r = 0; r < NR; ++r)
-                if (ch < r * (kThreads / kWave) + wv) {
-                    cb[0][r] += x0;
-                    cb[1][r] += x1;
-                }
-            tot[0] += x0;
-            tot[1] += x1;
-        }
-        const int fprev[2] = {s_flag[0], s_flag[1]};
-        int klo[2], khi[2], lo[2], len[2];
-        const int nn[2] = {n0, n1};
-#pragma unroll
-        for (int ty = 0; ty < 2; ++ty) {
-            // interval of sample p: pos[k] < p <= pos[k+1], p = 0 -> 0 (:418-433)
-            klo[ty] = t0 >= 1 ? m.pre[ty] - fprev[ty] - 1 : 0;
-            khi[ty] = t0 + nt < N ? m.pre[ty] + tot[ty] : nn[ty] - 1;
-            lo[ty] = max(m.pre[ty] - 2, 0);
-            len[ty] = min(m.pre[ty] + m.cnt[ty], nn[ty] - 1) - lo[ty] + 1;
-        }
-        const int cnt0 = khi[0] - klo[0] + 1, cnt1 = khi[1] - klo[1] + 1;
-        if (cnt0 > KC || cnt1 > KC || cnt0 < 2 || cnt1 < 2 || khi[0] >= n0 || khi[1] >= n1 ||
-            klo[0] < lo[0] || klo[1] < lo[1] || khi[0] >= lo[0] + len[0] ||
-            khi[1] >= lo[1] + len[1] || tot[0] != m.cnt[0] || tot[1] != m.cnt[1]) {
-            if (tid == 0) atomicOr(a.pc.err, ICM_DEVERR_KNOTCAP);
-            return;
-        }
-        // ---- knot table of [klo, khi]: the lanes owning the tile's knots,
-        // then the (at most three) knots outside it
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            const int u = r * kThreads + tid;
-#pragma unroll
-            for (int ty = 0; ty < 2; ++ty) {
-                if ((bm[ty][r] >> lane) & 1ull) {
-                    const int gk = m.pre[ty] + cb[ty][r] + lanes_below(bm[ty][r]);
-                    const int idx = gk - klo[ty];
-                    real y = sh[u + 2];
-                    if (gk == 0) y = ty ? fix10 : fix00;
-                    if (gk == nn[ty] - 1) y = ty ? fix11 : fix01;
-                    skp[ty][idx] = t0 + u;
-                    sco[ty][idx].a = y;
-                    sco[ty][idx].c = scr[(ty ? len[0] : 0) + gk - lo[ty]];
-                }
-            }
-        }
-        if (tid < 6) {
-            const int ty = tid / 3, which = tid % 3;
-            const int gk = which == 0 ? m.pre[ty] - 1
-                                      : (which == 1 ? m.pre[ty] - 2 : m.pre[ty] + tot[ty]);
-            if (gk >= klo[ty] && gk <= khi[ty] && snbx[tid] >= 0) {
-                real y = snby[tid];
-                if (gk == 0) y = ty ? fix10 : fix00;
-                if (gk == nn[ty] - 1) y = ty ? fix11 : fix01;
-                skp[ty][gk - klo[ty]] = snbx[tid];
-                sco[ty][gk - klo[ty]].a = y;
-                sco[ty][gk - klo[ty]].c = scr[(ty ? len[0] : 0) + gk - lo[ty]];
-            }
-        }
-        __syncthreads();
-        // ---- interval coefficients (spline_coefficients, :394-398)
-        for (int j = tid; j < cnt0 + cnt1 - 2; j += kThreads) {
-            const int ty = (j >= cnt0 - 1);
-            const int k = ty ? j - (cnt0 - 1) : j;
-            const real hk = (real)(skp[ty][k + 1] - skp[ty][k]);
-            const real a0 = sco[ty][k].a, a1 = sco[ty][k + 1].a;
-            const real c0 = sco[ty][k].c, c1 = sco[ty][k + 1].c;
-            sco[ty][k].b = (a1 - a0) / hk - hk * ((real)2 * c0 + c1) / (real)3;
-            sco[ty][k].d = (c1 - c0) / ((real)3 * hk);
-        }
-        __syncthreads();
-
-        // ---- envelopes, mean, h_new = h - m (interpolate :403-439,
-        // averageUppperLower :441-454, updateRealizations :527-544)
-        int sxc = 0, sxm = 0;
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            const int u = r * kThreads + tid;
-            if (u < nt) {
-                const int p = t0 + u;
-                int ku = 0, kl = 0;
-                if (p > 0) {
-                    ku = m.pre[0] + cb[0][r] + lanes_below(bm[0][r]) - 1;
-                    kl = m.pre[1] + cb[1][r] + lanes_below(bm[1][r]) - 1;
-                }
-                ku -= klo[0];
-                kl -= klo[1];
-                const real U = horner(sco[0][ku], (real)(p - skp[0][ku]));
-                const real L = horner(sco[1][kl], (real)(p - skp[1][kl]));
-                const real mm = use_mean ? (U + L) / (real)2 : (real)0;
-                const real hn = sh[u + 2] - mm;
-                sh[u + 2] = hn;
-                hout[p] = hn;
-                if (a.sx_cnt) {
-                    const real sx = fabs((U + L) / (U - L));
-                    sxc += (sx > (real)0.05) ? 1 : 0;
-                    sxm |= (sx > (real)0.5) ? 1 : 0;
-                }
-            }
-        }
-        // the samples just outside the tile (t0-1 and t1), needed for the
-        // extrema of h_new at the tile edges
-        if (tid < 2) {
-            const int p = tid == 0 ? t0 - 1 : t0 + nt;
-            if (p >= 0 && p < N) {
-                int ku, kl;
-                if (p == 0) {
-                    ku = kl = 0;
-                } else if (tid == 0) {
-                    ku = m.pre[0] - fprev[0] - 1;
-                    kl = m.pre[1] - fprev[1] - 1;
-                } else {
-                    ku = m.pre[0] + tot[0] - 1;
-                    kl = m.pre[1] + tot[1] - 1;
-                }
-                ku -= klo[0];
-                kl -= klo[1];
-                const real U = horner(sco[0][ku], (real)(p - skp[0][ku]));
-                const real L = horner(sco[1][kl], (real)(p - skp[1][kl]));
-                const real mm = use_mean ? (U + L) / (real)2 : (real)0;
-                const int q = tid == 0 ? 1 : nt + 2;
-                sh[q] = sh[q] - mm;
-            }
-        }
-        if (a.sx_cnt) {
-            // workgroup sums of the Rilling partials (integers: order-free)
-            for (int o = 32; o >= 1; o >>= 1) {
-                sxc += __shfl_xor(sxc, o, kWave);
-                sxm |= __shfl_xor(sxm, o, kWave);
-            }
-            if (lane == 0) {
-                s_red[2 * wv] = sxc;
-                s_red[2 * wv + 1] = sxm;
-            }
-        }
-        __syncthreads();
-        if (a.sx_cnt && tid == 0) {
-            int c = 0, mk = 0;
-            for (int w = 0; w < kThreads / kWave; ++w) {
-                c += s_red[2 * w];
-                mk |= s_red[2 * w + 1];
-            }
-            a.sx_cnt[gt] = c;
-            a.sx_mark[gt] = mk;
-        }
-
-        if (a.emit) {
-            // ---- knots of h_new into the tile's slots (select_extrema_*, :150-207)
-#pragma unroll
-            for (int r = 0; r < NR; ++r) {
-                const int u = r * kThreads + tid;
-                const int p = t0 + u;
-                bool fmx = false, fmn = false;
-                if (u < nt) {
-                    if (p == 0 || p == N - 1) {
-                        fmx = fmn = true;
-                    } else {
-                        const real v = sh[u + 2], lft = sh[u + 1], rgt = sh[u + 3];
-                        fmx = (v > lft) && (v > rgt);
-                        fmn = (v < lft) && (v < rgt);
-                    }
-                }
-                bm[0][r] = wave_ballot(fmx);
-                bm[1][r] = wave_ballot(fmn);
-                if (lane == 0) {
-                    s_cnt[0][r * (kThreads / kWave) + wv] = __popcll(bm[0][r]);
-                    s_cnt[1][r * (kThreads / kWave) + wv] = __popcll(bm[1][r]);
-                }
-            }
-            __syncthreads();
-            int tot2[2] = {0, 0};
-#pragma unroll
-            for (int r = 0; r < NR; ++r) cb[0][r] = cb[1][r] = 0;
-#pragma unroll
-            for (int ch = 0; ch < NCH; ++ch) {
-                const int x0 = s_cnt[0][ch], x1 = s_cnt[1][ch];
-#pragma unroll
-                for (int r = 0; r < NR; ++r)
-                    if (ch < r * (kThreads / kWave) + wv) {
-                        cb[0][r] += x0;
-                        cb[1][r] += x1;
-                    }
-                tot2[0] += x0;
-                tot2[1] += x1;
-            }
-            const int64_t slot0 = (int64_t)gt * sd.cap;
-#pragma unroll
-            for (int r = 0; r < NR; ++r) {
-                const int u = r * kThreads + tid;
-#pragma unroll
-                for (int ty = 0; ty < 2; ++ty) {
-                    if ((bm[ty][r] >> lane) & 1ull) {
-                        const int64_t o = (ty ? sd.sstride : 0) + slot0 + cb[ty][r] +
-                                          lanes_below(bm[ty][r]);
-                        a.kout.x[o] = t0 + u;
-                        a.kout.y[o] = sh[u + 2];
-                    }
-                }
-            }
-            if (tid == 0) {
-                a.kout.cnt[2 * gt] = tot2[0];
-                a.kout.cnt[2 * gt + 1] = tot2[1];
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < NH; ++k) hcur[k] = hnext[k];
-#pragma unroll
-        for (int k = 0; k < NC; ++k) ccur[k] = cnext[k];
-        nbx_cur = nbx_next;
-        nby_cur = nby_next;
-        mcur = mnext;
-        mnext = mnext2;
-    }
 }
 
 // ------------------------------------------------------- windowed spline solve
@@ -1698,169 +1175,6 @@ __global__ __launch_bounds__(kThreads) void k_init_w(SeriesDev sd, InitArgs<real
 // the forward's windowed solver is reused), transposed right-hand side and
 // endpoint fix, and a scatter to the knot samples.
 
-// 1. moments sum_p mu_p t^m, m = 0..3, of the runs of samples of each knot
-// interval inside a tile (interval k = (x_k, x_{k+1}], sample 0 in interval 0).
-// A sample's run is the number of the tile's knots strictly before it (from
-// a bitmap of the knots); runs are summed by a segmented scan (wave shuffles,
-// then a carry across waves and 256-sample chunks).  Runs bounded by two
-// knots of the tile are complete intervals; the first and last run of a tile
-// go to the edge array for k_adj_span.
-template <typename real, int T>
-__global__ __launch_bounds__(kThreads) void k_adj_mom(AdjArgs<real> a)
-{
-    constexpr int KC = T / 2 + 2;
-    constexpr int NWV = kThreads / kWave;
-    constexpr int NWORD = T / 64;
-    constexpr int NOKEY = 1 << 28;
-    __shared__ int skp[2][KC];
-    __shared__ unsigned long long sbits[2][NWORD];
-    __shared__ int swpre[2][NWORD];
-    __shared__ int s_tail_key[2][NWV], s_head_key[2][NWV];
-    __shared__ real s_tail[2][NWV][4];
-    __shared__ int s_in_key[2][NWV];
-    __shared__ real s_in[2][NWV][4];
-    __shared__ int s_cc_key[2];
-    __shared__ real s_cc[2][4];
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wv = tid >> 6;
-    const SeriesDev& sd = a.sd;
-    const int gt = blockIdx.x;
-    const int s = sd.tile_series[gt];
-    const int t = gt - sd.tile0[s];
-    const int N = sd.len[s];
-    const int t0 = t * T;
-    const int nt = min(T, N - t0);
-    const int n0 = a.ps.n[2 * s], n1 = a.ps.n[2 * s + 1];
-    if (!(n0 > 3 && n1 > 3)) return;   // the pass did not subtract: identity
-    const real* mu = a.mu + sd.off[s];
-    int cnt[2], pre[2], left[2];
-    int64_t kb[2];
-#pragma unroll
-    for (int ty = 0; ty < 2; ++ty) {
-        kb[ty] = sd.kbase[s] + (ty ? sd.kstride : 0);
-        cnt[ty] = a.ps.cnt[2 * gt + ty];
-        pre[ty] = a.ps.pre[2 * gt + ty];
-        left[ty] = pre[ty] > 0 ? a.ps.px[kb[ty] + pre[ty] - 1] : 0;
-    }
-    for (int i = tid; i < 2 * NWORD; i += kThreads) sbits[i / NWORD][i % NWORD] = 0ull;
-    if (tid < 2) {
-        s_cc_key[tid] = NOKEY;
-        for (int m = 0; m < 4; ++m) s_cc[tid][m] = (real)0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int ty = 0; ty < 2; ++ty)
-        for (int i = tid; i < cnt[ty]; i += kThreads) {
-            const int x = a.ps.px[kb[ty] + pre[ty] + i];
-            skp[ty][i] = x;
-            const int u = x - t0;
-            atomicOr(&sbits[ty][u >> 6], 1ull << (u & 63));
-        }
-    __syncthreads();
-    if (tid < 2) {
-        int run = 0;
-        for (int w = 0; w < NWORD; ++w) {
-            swpre[tid][w] = run;
-            run += __popcll(sbits[tid][w]);
-        }
-    }
-    // empty edge runs (a knot on the tile's first / last sample)
-    if (tid < 2) {
-        const int ty = tid;
-        real* e = a.edge + 4 * (4 * (int64_t)gt + 2 * ty);
-        if (cnt[ty] > 0 && (t0 == 0 || skp[ty][0] == t0))
-            for (int m = 0; m < 4; ++m) e[m] = (real)0;
-        if (cnt[ty] > 0 && skp[ty][cnt[ty] - 1] == t0 + nt - 1)
-            for (int m = 0; m < 4; ++m) e[4 + m] = (real)0;
-    }
-    __syncthreads();
-
-    auto key_of = [&](int ty, int u) -> int {
-        // knots of the tile strictly before sample u; sample 0 joins run 1
-        if (t0 == 0 && u == 0) return 1;
-        const unsigned long long below = (u & 63) ? (sbits[ty][u >> 6] << (64 - (u & 63))) : 0ull;
-        return swpre[ty][u >> 6] + __popcll(below);
-    };
-
-    for (int c0 = 0; c0 < T; c0 += kThreads) {
-        const int u = c0 + tid;
-        const bool valid = u < nt;
-        int key[2];
-        real v[2][4];
-#pragma unroll
-        for (int ty = 0; ty < 2; ++ty) {
-            key[ty] = valid ? key_of(ty, u) : NOKEY + 1 + ty;
-            real g = (real)0, tt = (real)0;
-            if (valid) {
-                const int r = key[ty];
-                const int lp = (t0 == 0 && u == 0) ? 0 : (r >= 1 ? skp[ty][r - 1] : left[ty]);
-                tt = (real)(t0 + u - lp);
-                g = mu[t0 + u];
-            }
-            v[ty][0] = g;
-            v[ty][1] = g * tt;
-            v[ty][2] = v[ty][1] * tt;
-            v[ty][3] = v[ty][2] * tt;
-        }
-        // segmented inclusive scan inside the wave
-#pragma unroll
-        for (int o = 1; o < kWave; o <<= 1) {
-#pragma unroll
-            for (int ty = 0; ty < 2; ++ty) {
-                const int ku = shfl_up(key[ty], o);
-                const bool take = lane >= o && ku == key[ty];
-#pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                    const real w = shfl_up(v[ty][m], o);
-                    if (take) v[ty][m] = v[ty][m] + w;
-                }
-            }
-        }
-        if (lane == kWave - 1)
-            for (int ty = 0; ty < 2; ++ty) {
-                s_tail_key[ty][wv] = key[ty];
-                for (int m = 0; m < 4; ++m) s_tail[ty][wv][m] = v[ty][m];
-            }
-        if (lane == 0)
-            for (int ty = 0; ty < 2; ++ty) s_head_key[ty][wv] = key[ty];
-        __syncthreads();
-        // carries into each wave (and into the next chunk), one thread per type
-        if (tid < 2) {
-            const int ty = tid;
-            int ck = s_cc_key[ty];
-            real cv[4];
-            for (int m = 0; m < 4; ++m) cv[m] = s_cc[ty][m];
-            for (int w = 0; w < NWV; ++w) {
-                s_in_key[ty][w] = ck;
-                for (int m = 0; m < 4; ++m) s_in[ty][w][m] = cv[m];
-                const int tk = s_tail_key[ty][w];
-                const bool chain = (s_head_key[ty][w] == tk) && (ck == tk);
-                for (int m = 0; m < 4; ++m) cv[m] = s_tail[ty][w][m] + (chain ? cv[m] : (real)0);
-                ck = tk;
-            }
-            s_cc_key[ty] = ck;
-            for (int m = 0; m < 4; ++m) s_cc[ty][m] = cv[m];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int ty = 0; ty < 2; ++ty) {
-            if (!valid) continue;
-            if (s_in_key[ty][wv] == key[ty])
-                for (int m = 0; m < 4; ++m) v[ty][m] = v[ty][m] + s_in[ty][wv][m];
-            // run ends here: write its moments
-            const bool end = (u == nt - 1) || (key_of(ty, u + 1) != key[ty]);
-            if (!end) continue;
-            const int r = key[ty];
-            real* dst;
-            if (r >= 1 && r < cnt[ty]) dst = a.mom + 4 * (kb[ty] + pre[ty] + r - 1);
-            else dst = a.edge + 4 * (4 * (int64_t)gt + 2 * ty + (r == 0 ? 0 : 1));
-            for (int m = 0; m < 4; ++m) dst[m] = v[ty][m];
-        }
-        // the carry words are rewritten by the next chunk
-        __syncthreads();
-    }
-}
-
 // 1'. Fused per-tile step for 256-sample tiles: one wavefront per tile,
 // lane l owns the four consecutive samples 4l .. 4l+3.  Optionally
 // mu <- broadcast, then mu <- mu - (G_max^T + G_min^T)/2 for the scatter
@@ -2209,42 +1523,6 @@ __global__ __launch_bounds__(kThreads) void k_adj_knot(AdjArgs<real> a)
     }
 }
 
-// 4. mu <- mu - (scatter of ga_max + scatter of ga_min) / 2
-template <typename real, int T>
-__global__ __launch_bounds__(kThreads) void k_adj_scatter(AdjArgs<real> a)
-{
-    __shared__ real sd_acc[T];
-    const int tid = threadIdx.x;
-    const SeriesDev& sd = a.sd;
-    const int gt = blockIdx.x;
-    const int s = sd.tile_series[gt];
-    const int t = gt - sd.tile0[s];
-    const int N = sd.len[s];
-    const int t0 = t * T;
-    const int nt = min(T, N - t0);
-    const int n0 = a.ps.n[2 * s], n1 = a.ps.n[2 * s + 1];
-    if (!(n0 > 3 && n1 > 3)) return;
-    real* mu = a.mu + sd.off[s];
-    for (int u = tid; u < T; u += kThreads) sd_acc[u] = (real)0;
-    __syncthreads();
-    for (int ty = 0; ty < 2; ++ty) {
-        const int64_t kb = sd.kbase[s] + (ty ? sd.kstride : 0);
-        const int cnt = a.ps.cnt[2 * gt + ty];
-        const int pre = a.ps.pre[2 * gt + ty];
-        for (int i = tid; i < cnt; i += kThreads) {
-            const int x = a.ps.px[kb + pre + i];
-            sd_acc[x - t0] = sd_acc[x - t0] + a.ga[kb + pre + i];
-        }
-        __syncthreads();
-    }
-    for (int u = tid; u < nt; u += kThreads) {
-        const int p = t0 + u;
-        mu[p] = mu[p] - sd_acc[u] / (real)2;
-    }
-}
-
-// dL/dr_k from dL/dIMF of one channel ([M][N]): gr_0 = g_0,
-// gr_k = g_k - g_{k-1} (IMF_0 = x~ - r_1, IMF_k = r_k - r_{k+1}, IMF_{M-1} = r_{M-1})
 template <typename real>
 __global__ void k_adj_seed(const real* gimf, int M, int N, int64_t stride, real* gr)
 {
@@ -2256,16 +1534,6 @@ __global__ void k_adj_seed(const real* gimf, int M, int N, int64_t stride, real*
 }
 
 // stage level: mu of every realization starts as dL/dr_{k+1} of its channel
-template <typename real>
-__global__ void k_adj_broadcast(SeriesDev sd, const real* gr_next, const int64_t* chan_off, real* mu)
-{
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    for (int s = blockIdx.y; s < sd.nseries; s += gridDim.y) {
-        if (p >= sd.len[s]) continue;
-        mu[sd.off[s] + p] = gr_next[chan_off[sd.chan[s]] + p];
-    }
-}
-
 // g_x_i = (lambda - mu_i) / I; gacc = sum_i g_x_i (sequential in i)
 template <typename real>
 __global__ void k_adj_accum(int I, const int* chan_len, const int64_t* chan_off, SeriesDev sd,
@@ -2518,18 +1786,8 @@ template <typename real>
 hipError_t launch_init(const SeriesDev& sd, const InitArgs<real>& ia, real* h_out,
                        const KnotsDev<real>& kout, const PassCtl&, hipStream_t st)
 {
-    const int T = 2 * (sd.cap - 2);
-    if (T == 256)
-        hipLaunchKernelGGL((k_init_w<real>), dim3(blocks_for(sd.ntiles, kThreads / kWave)),
-                           dim3(kThreads), 0, st, sd, ia, h_out, kout);
-    else if (T == 512)
-        hipLaunchKernelGGL((k_init<real, 512>), dim3(sd.ntiles), dim3(kThreads), 0, st, sd, ia,
-                           h_out, kout);
-    else if (T == 1024)
-        hipLaunchKernelGGL((k_init<real, 1024>), dim3(sd.ntiles), dim3(kThreads), 0, st, sd, ia,
-                           h_out, kout);
-    else
-        return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_init_w<real>), dim3(blocks_for(sd.ntiles, kThreads / kWave)),
+                       dim3(kThreads), 0, st, sd, ia, h_out, kout);
     return hipGetLastError();
 }
 
@@ -2549,20 +1807,12 @@ hipError_t launch_knot_prefix(const SeriesDev& sd, const KnotsDev<real>& k, cons
 template <typename real>
 hipError_t launch_sift(const SiftArgs<real>& a, int nsegs, hipStream_t st)
 {
-    const int T = 2 * (a.sd.cap - 2);
-    if (T == 256)
-        if (a.sx_cnt)
-            hipLaunchKernelGGL((k_sift_w<real, true>), dim3(blocks_for(nsegs, kThreads / kWave)),
-                               dim3(kThreads), 0, st, a, nsegs);
-        else
-            hipLaunchKernelGGL((k_sift_w<real, false>), dim3(blocks_for(nsegs, kThreads / kWave)),
-                               dim3(kThreads), 0, st, a, nsegs);
-    else if (T == 512)
-        hipLaunchKernelGGL((k_sift<real, 512>), dim3(nsegs), dim3(kThreads), 0, st, a);
-    else if (T == 1024)
-        hipLaunchKernelGGL((k_sift<real, 1024>), dim3(nsegs), dim3(kThreads), 0, st, a);
+    if (a.sx_cnt)
+        hipLaunchKernelGGL((k_sift_w<real, true>), dim3(blocks_for(nsegs, kThreads / kWave)),
+                           dim3(kThreads), 0, st, a, nsegs);
     else
-        return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_sift_w<real, false>), dim3(blocks_for(nsegs, kThreads / kWave)),
+                           dim3(kThreads), 0, st, a, nsegs);
     return hipGetLastError();
 }
 
@@ -2658,34 +1908,6 @@ hipError_t launch_select_copy(const SeriesDev& sd, int max_len, const real* hA, 
 }
 
 template <typename real>
-hipError_t launch_adjoint_pass(const AdjArgs<real>& a, int solve_grid, hipStream_t st)
-{
-    const int T = 2 * (a.sd.cap - 2);
-    if (T == 256)
-        hipLaunchKernelGGL((k_adj_mom<real, 256>), dim3(a.sd.ntiles), dim3(kThreads), 0, st, a);
-    else if (T == 512)
-        hipLaunchKernelGGL((k_adj_mom<real, 512>), dim3(a.sd.ntiles), dim3(kThreads), 0, st, a);
-    else if (T == 1024)
-        hipLaunchKernelGGL((k_adj_mom<real, 1024>), dim3(a.sd.ntiles), dim3(kThreads), 0, st, a);
-    else
-        return hipErrorInvalidValue;
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_adj_span<real>), dim3(blocks_for(a.sd.ntiles, 256)), dim3(256), 0, st, a);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL((k_adj_knot<real, Geom<real>::B, Geom<real>::H>), dim3(solve_grid),
-                       dim3(kThreads), 0, st, a);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (T == 256)
-        hipLaunchKernelGGL((k_adj_scatter<real, 256>), dim3(a.sd.ntiles), dim3(kThreads), 0, st, a);
-    else if (T == 512)
-        hipLaunchKernelGGL((k_adj_scatter<real, 512>), dim3(a.sd.ntiles), dim3(kThreads), 0, st, a);
-    else
-        hipLaunchKernelGGL((k_adj_scatter<real, 1024>), dim3(a.sd.ntiles), dim3(kThreads), 0, st, a);
-    return hipGetLastError();
-}
-
-template <typename real>
 hipError_t launch_adj_tile(const AdjTileArgs<real>& a, hipStream_t st)
 {
     if (2 * (a.sd.cap - 2) != kWT) return hipErrorInvalidValue;
@@ -2711,16 +1933,6 @@ hipError_t launch_adj_seed(const real* gimf, int M, int64_t N, int64_t stride, r
 {
     hipLaunchKernelGGL((k_adj_seed<real>), dim3(blocks_for(N, 256)), dim3(256), 0, st, gimf, M,
                        (int)N, stride, gr);
-    return hipGetLastError();
-}
-
-template <typename real>
-hipError_t launch_adj_broadcast(const SeriesDev& sd, const real* gr_next, const int64_t* chan_off,
-                                real* mu, int max_len, hipStream_t st)
-{
-    const unsigned gy = (unsigned)(sd.nseries < 65535 ? sd.nseries : 65535);
-    hipLaunchKernelGGL((k_adj_broadcast<real>), dim3(blocks_for(max_len, 256), gy), dim3(256), 0,
-                       st, sd, gr_next, chan_off, mu);
     return hipGetLastError();
 }
 
@@ -2791,13 +2003,10 @@ hipError_t launch_check_index(const int* d_x, int64_t N, int* err, hipStream_t s
                                                  const int*, real*, hipStream_t);
 
 #define ICM_INSTANTIATE_ADJ(real)                                                                  \
-    template hipError_t launch_adjoint_pass<real>(const AdjArgs<real>&, int, hipStream_t);         \
     template hipError_t launch_adj_tile<real>(const AdjTileArgs<real>&, hipStream_t);              \
     template hipError_t launch_adj_span_knot<real>(const AdjArgs<real>&, int, hipStream_t);        \
     template hipError_t launch_adj_seed<real>(const real*, int, int64_t, int64_t, real*,           \
                                               hipStream_t);                                        \
-    template hipError_t launch_adj_broadcast<real>(const SeriesDev&, const real*, const int64_t*,  \
-                                                   real*, int, hipStream_t);                       \
     template hipError_t launch_adj_stage<real>(int, int, int, int, const int*, const int64_t*,     \
                                                const SeriesDev&, const real*, const real*,         \
                                                const real*, const real*, real, real*, real*,       \

@@ -45,14 +45,8 @@ const uint32_t kEpochs = 65536;
 
 // --------------------------------------------------------------- geometry
 
-// samples per tile of the tile kernels: 256 (one wavefront per tile in the
-// sift pass), or 512 / 1024 (one workgroup per tile)
-int tile_of(int)
-{
-    const char* env = std::getenv("ICM_TILE");
-    const int t = env ? std::atoi(env) : 256;
-    return (t == 512 || t == 1024) ? t : 256;
-}
+// samples per tile of the tile kernels (one wavefront per tile)
+int tile_of(int) { return 256; }
 // windowed solve geometry (both precisions): 64 lanes x 8 rows, 32 halo rows
 // on each side; the dropped coupling decays by >= 3.7x per row, so 32 rows
 // are below FP64 round-off
@@ -1167,26 +1161,20 @@ struct Plan : PlanBase {
                 a.err = err;
                 return a;
             };
-            if (2 * (cap - 2) == 256) {
-                // fused per-tile steps: (broadcast | scatter of pass j+1) + moments of pass j
-                for (int j = S - 1; j >= -1; --j) {
-                    AdjTileArgs<real> ta{};
-                    ta.sd = sd;
-                    ta.mu = mu;
-                    ta.bcast = j == S - 1 ? lam : nullptr;
-                    ta.chan_off = d_chan_off;
-                    ta.sc = rec_of(j + 1);
-                    ta.ga = ga;
-                    ta.mo = rec_of(j);
-                    ta.mom = mom;
-                    ta.edge = edge;
-                    ICM_TRY(launch_adj_tile<real>(ta, st));
-                    if (j >= 0) ICM_TRY(launch_adj_span_knot<real>(adj_args(j), solve_grid(), st));
-                }
-            } else {
-                ICM_TRY(launch_adj_broadcast<real>(sd, lam, d_chan_off, mu, dsig.max_len, st));
-                for (int j = S - 1; j >= 0; --j)
-                    ICM_TRY(launch_adjoint_pass<real>(adj_args(j), solve_grid(), st));
+            // fused per-tile steps: (broadcast | scatter of pass j+1) + moments of pass j
+            for (int j = S - 1; j >= -1; --j) {
+                AdjTileArgs<real> ta{};
+                ta.sd = sd;
+                ta.mu = mu;
+                ta.bcast = j == S - 1 ? lam : nullptr;
+                ta.chan_off = d_chan_off;
+                ta.sc = rec_of(j + 1);
+                ta.ga = ga;
+                ta.mo = rec_of(j);
+                ta.mom = mom;
+                ta.edge = edge;
+                ICM_TRY(launch_adj_tile<real>(ta, st));
+                if (j >= 0) ICM_TRY(launch_adj_span_knot<real>(adj_args(j), solve_grid(), st));
             }
             ICM_TRY(launch_adj_stage<real>(k, C, I, max_chan_len, d_chan_len, d_chan_off, sd, lam,
                                            mu, Ek, rk, eps0, gr + (int64_t)k * chan_total, gacc,

@@ -211,27 +211,3 @@ def test_invalid_arguments_raise(icm, dev):
         plan.forward([torch.zeros(99, dtype=torch.float64, device=dev)])
     plan.close()
 
-
-def test_tile_size_does_not_change_results(icm, dev):
-    """512-sample tiles (one workgroup per tile in the sift pass and the
-    adjoint) give the same bits as the default 256-sample wave tiles."""
-    import torch
-    N, I, M, S = 5000, 6, 5, 10
-    x = torch.tensor(_sig().eeg_like(N, seed=21), dtype=torch.float64, device=dev)
-    G = torch.tensor(np.random.default_rng(4).standard_normal((M, N)), device=dev)
-    out = {}
-    for tile in ("256", "512"):
-        print(f"case: tile {tile}", flush=True)
-        os.environ["ICM_TILE"] = tile
-        try:
-            plan = icm.Plan([N], I, M, dtype=torch.float64, max_sifts=S, use_graph=False,
-                            device=dev)
-        finally:
-            del os.environ["ICM_TILE"]
-        assert plan.info()["tile"] == int(tile)
-        imfs = plan.forward([x])[0].clone()
-        g = plan.backward([G])[0].clone()
-        plan.close()
-        out[tile] = (imfs, g)
-    assert torch.equal(out["256"][0], out["512"][0])
-    assert torch.allclose(out["256"][1], out["512"][1], rtol=1e-12, atol=1e-14)
