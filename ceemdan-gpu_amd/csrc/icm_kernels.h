@@ -31,25 +31,22 @@ struct SeriesDev {
     int ntiles;
     int max_tiles;            // most tiles of one series
     int64_t kstride;          // per-type stride of the compact c array
-    int64_t sstride;          // per-type stride of the knot slots (tiles x cap)
-    int cap;                  // knot slots per tile and type
+    int cap;                  // most knots of one type in a tile (T/2 + 2)
 };
 
-// Knots of one sift-pass input, kept where the tile that found them wrote
-// them (no cross-workgroup compaction): tile gt's maxima are slots
-// [gt*cap, gt*cap + cnt[2*gt]) of type 0, its minima the same of type 1.
+// Knots of one sift-pass input.  A knot is a set bit of its tile's row
+// bitmasks, written by the kernel that wrote the samples (stage start or the
+// previous sift pass); its position is the bit's sample index and its value
+// the sample of the pass input there.  Nothing else is stored per knot.
 template <typename real>
 struct KnotsDev {
-    int* x;      // [2][sstride] sample index in the series
-    real* y;     // [2][sstride] sample value
-    int* cnt;    // [ntiles][2]  knots found in the tile
+    uint64_t* mask;   // [ntiles][2][4] knot bits of the tile's four 64-sample rows per type
+    int* cnt;    // [ntiles][2]  knots in the tile (popcounts of its masks)
     int* pre;    // [ntiles][2]  knots before the tile in its series (k_knot_prefix)
     int* n;      // [nseries][2] knots of the series (k_knot_prefix)
     real* fix;   // [nseries][2][2] values of knots 0 and n-1 after the endpoint fix (k_solve)
     int* nbx;    // [ntiles][2][3] positions of the knots pre-1, pre-2 and pre+cnt of the
     real* nby;   //  tile (-1 = none) and their values (k_knot_prefix)
-    uint64_t* mask;   // [ntiles][2][4] 256-sample tiles: knot bits of the tile's four
-                      //  64-sample rows per type, written with the knots
 };
 
 // A run of consecutive tiles of one series, sifted by one workgroup that
@@ -109,7 +106,8 @@ struct SiftArgs {
 template <typename real>
 struct SolveArgs {
     SeriesDev sd;
-    KnotsDev<real> k;       // knots (y of knots 0 and n-1 fixed up in place)
+    KnotsDev<real> k;       // knots of the pass input
+    const real* h;          // the pass input samples (the knots' values)
     real* c;                // [2][kstride] out
     const int* winlist;
     const unsigned* wincount;
@@ -186,8 +184,8 @@ template <typename real>
 hipError_t launch_solve(const SolveArgs<real>& a, int grid, hipStream_t st);
 
 template <typename real>
-hipError_t launch_knot_prefix(const SeriesDev& sd, const KnotsDev<real>& k, const PassCtl& pc,
-                              const int* active, hipStream_t st);
+hipError_t launch_knot_prefix(const SeriesDev& sd, const KnotsDev<real>& k, const real* h,
+                              const PassCtl& pc, const int* active, hipStream_t st);
 template <typename real>
 hipError_t launch_rilling_decide(const SeriesDev& sd, const int* nk, const int* sx_cnt,
                                  const int* sx_mark, int* active, int* stop_pass, int* subs,

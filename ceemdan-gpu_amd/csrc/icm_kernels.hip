@@ -29,6 +29,73 @@ template <typename real> struct Promote { using type = double; };
 
 // ------------------------------------------------------------ tile helpers
 
+constexpr int kWT = 256;               // samples per tile
+constexpr int kWR = kWT / kWave;       // 64-sample chunks per tile
+constexpr int kWKC = kWT / 2 + 8;      // table entries per type (>= cnt + 3)
+constexpr int kWNC = (kWKC + kWave - 1) / kWave;
+
+// ------------------------------------------------------------- knot bitmasks
+// A tile's knots of one type are the set bits of four 64-bit row masks
+// m[r] (bit b of row r = sample 64 r + b of the tile).
+
+// tile offset of the last set bit, -1 if none
+__device__ __forceinline__ int mask_last(const uint64_t (&m)[kWR])
+{
+    int last = -1;
+#pragma unroll
+    for (int r = 0; r < kWR; ++r) last = m[r] ? kWave * r + 63 - __clzll(m[r]) : last;
+    return last;
+}
+
+// tile offset of the first set bit, -1 if none
+__device__ __forceinline__ int mask_first(const uint64_t (&m)[kWR])
+{
+    int first = -1;
+#pragma unroll
+    for (int r = kWR - 1; r >= 0; --r) first = m[r] ? kWave * r + __ffsll((long long)m[r]) - 1 : first;
+    return first;
+}
+
+// tile offset of the k-th set bit (k = 0 the first); k must be below the
+// number of set bits (any other k gives some offset in [0, 256))
+__device__ __forceinline__ int mask_select(const uint64_t (&m)[kWR], int k)
+{
+    static_assert(kWR == 4, "four rows per tile");
+    const int c0 = __popcll(m[0]), c1 = __popcll(m[1]), c2 = __popcll(m[2]);
+    int r = 0;
+    uint64_t w = m[0];
+    if (k >= c0) {
+        k -= c0; r = 1; w = m[1];
+        if (k >= c1) {
+            k -= c1; r = 2; w = m[2];
+            if (k >= c2) { k -= c2; r = 3; w = m[3]; }
+        }
+    }
+    // k-th set bit of w by halving
+    int pos = 0;
+    uint32_t v = (uint32_t)w;
+    int c = __popc(v);
+    if (k >= c) { k -= c; pos = 32; v = (uint32_t)(w >> 32); }
+    c = __popc(v & 0xFFFFu);
+    if (k >= c) { k -= c; pos += 16; v >>= 16; }
+    c = __popc(v & 0xFFu);
+    if (k >= c) { k -= c; pos += 8; v >>= 8; }
+    c = __popc(v & 0xFu);
+    if (k >= c) { k -= c; pos += 4; v >>= 4; }
+    c = __popc(v & 0x3u);
+    if (k >= c) { k -= c; pos += 2; v >>= 2; }
+    if (k >= (int)(v & 1u)) pos += 1;
+    return kWave * r + pos;
+}
+
+template <typename real>
+__device__ __forceinline__ void load_masks(const KnotsDev<real>& k, int64_t gt, int type,
+                                           uint64_t (&m)[kWR])
+{
+#pragma unroll
+    for (int r = 0; r < kWR; ++r) m[r] = k.mask[8 * gt + kWR * type + r];
+}
+
 struct LastTwo {
     int c;
     int64_t l1, l2;
@@ -56,7 +123,8 @@ __device__ __forceinline__ LastTwo lt_shfl_up(const LastTwo& v, int o)
 // Knot prefix: per-series exclusive scan of the tile knot counts (the CUB
 // ExclusiveSum of cudaICEEMDAN.cu:859-883, over tiles instead of samples),
 // the series knot counts, the solve's window items, and for every tile the
-// slots of the two knots before it and of the first knot after it.
+// positions and values of the two knots before it and of the first knot
+// after it (positions from the bitmasks, values from the pass input h).
 //
 // The same outputs with one 1024-thread workgroup per series and one tile per
 // thread (chunks of 1024 tiles): workgroup scans forward (counts, last two
@@ -73,8 +141,8 @@ __device__ __forceinline__ LastTwo lt_none()
 }
 
 template <typename real>
-__global__ __launch_bounds__(kPT) void k_knot_prefix_b(SeriesDev sd, KnotsDev<real> k, PassCtl pc,
-                                                       const int* active)
+__global__ __launch_bounds__(kPT) void k_knot_prefix_b(SeriesDev sd, KnotsDev<real> k, const real* h,
+                                                       PassCtl pc, const int* active)
 {
     constexpr int NWV = kPT / kWave;
     constexpr int INF = 0x7FFFFFFF;
@@ -86,20 +154,25 @@ __global__ __launch_bounds__(kPT) void k_knot_prefix_b(SeriesDev sd, KnotsDev<re
     if (active && !active[s]) return;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int tb = sd.tile0[s], nt = sd.ntl[s];
-    const int cap = sd.cap;
+    const real* hs = h + sd.off[s];
     int ntot[2] = {0, 0};
     for (int ty = 0; ty < 2; ++ty) {
-        const int64_t so = ty ? sd.sstride : 0;
         // forward: exclusive count and the last two knots before each tile
         int carry = 0;
         LastTwo carry_lt = lt_none();
         for (int base = 0; base < nt; base += kPT) {
             const int t = base + tid;
             const int c = t < nt ? k.cnt[2 * (tb + t) + ty] : 0;
-            LastTwo own;
-            own.c = min(c, 2);
-            own.l1 = c >= 1 ? (int64_t)(tb + t) * cap + (c - 1) : -1;
-            own.l2 = c >= 2 ? (int64_t)(tb + t) * cap + (c - 2) : -1;
+            LastTwo own = lt_none();
+            if (c > 0) {
+                uint64_t m[kWR];
+                load_masks(k, tb + t, ty, m);
+                const int b1 = mask_last(m);
+                m[b1 / kWave] ^= 1ull << (b1 % kWave);
+                own.c = min(c, 2);
+                own.l1 = (int64_t)t * kWT + b1;
+                own.l2 = c >= 2 ? (int64_t)t * kWT + mask_last(m) : -1;
+            }
             int inc = c;
             LastTwo sc = own;
             for (int o = 1; o < kWave; o <<= 1) {
@@ -149,10 +222,10 @@ __global__ __launch_bounds__(kPT) void k_knot_prefix_b(SeriesDev sd, KnotsDev<re
             if (t < nt) {
                 k.pre[2 * (tb + t) + ty] = ex;
                 const int64_t o = 6 * (int64_t)(tb + t) + 3 * ty;
-                k.nbx[o] = before.l1 >= 0 ? k.x[so + before.l1] : -1;
-                k.nby[o] = before.l1 >= 0 ? k.y[so + before.l1] : (real)0;
-                k.nbx[o + 1] = before.l2 >= 0 ? k.x[so + before.l2] : -1;
-                k.nby[o + 1] = before.l2 >= 0 ? k.y[so + before.l2] : (real)0;
+                k.nbx[o] = before.l1 >= 0 ? (int)before.l1 : -1;
+                k.nby[o] = before.l1 >= 0 ? hs[before.l1] : (real)0;
+                k.nbx[o + 1] = before.l2 >= 0 ? (int)before.l2 : -1;
+                k.nby[o + 1] = before.l2 >= 0 ? hs[before.l2] : (real)0;
             }
             carry += s_all;
             carry_lt = lt_combine(carry_lt, s_all_lt);
@@ -180,9 +253,14 @@ __global__ __launch_bounds__(kPT) void k_knot_prefix_b(SeriesDev sd, KnotsDev<re
             }
             if (t < nt) {
                 const int64_t o = 6 * (int64_t)(tb + t) + 3 * ty + 2;
-                const int64_t sl = so + (int64_t)(tb + after) * cap;
-                k.nbx[o] = after == INF ? -1 : k.x[sl];
-                k.nby[o] = after == INF ? (real)0 : k.y[sl];
+                int x3 = -1;
+                if (after != INF) {
+                    uint64_t m[kWR];
+                    load_masks(k, tb + after, ty, m);
+                    x3 = after * kWT + mask_first(m);
+                }
+                k.nbx[o] = x3;
+                k.nby[o] = x3 >= 0 ? hs[x3] : (real)0;
             }
             carry_min = all;
         }
@@ -209,8 +287,8 @@ __global__ __launch_bounds__(kPT) void k_knot_prefix_b(SeriesDev sd, KnotsDev<re
 // types and both scan directions at once (one load round, two workgroup
 // barriers, one gather round).
 template <typename real>
-__global__ __launch_bounds__(kPT) void k_knot_prefix_1(SeriesDev sd, KnotsDev<real> k, PassCtl pc,
-                                                       const int* active)
+__global__ __launch_bounds__(kPT) void k_knot_prefix_1(SeriesDev sd, KnotsDev<real> k, const real* h,
+                                                       PassCtl pc, const int* active)
 {
     constexpr int NWV = kPT / kWave;
     constexpr int INF = 0x7FFFFFFF;
@@ -218,22 +296,33 @@ __global__ __launch_bounds__(kPT) void k_knot_prefix_1(SeriesDev sd, KnotsDev<re
     __shared__ LastTwo s_lt[2][NWV], s_ex_lt[2][NWV];
     __shared__ int s_min[2][NWV], s_ex_min[2][NWV];
     __shared__ int s_pre[2][kPT];
+    __shared__ int s_first[2][kPT];
     __shared__ unsigned s_wbase;
     const int s = blockIdx.x;
     if (active && !active[s]) return;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int tb = sd.tile0[s], nt = sd.ntl[s];
-    const int cap = sd.cap;
+    const real* hs = h + sd.off[s];
     const int t = tid;
     int c[2];
     LastTwo sc[2];
     int inc[2], mn[2];
+    int first[2];         // tile offset of the tile's first knot per type
 #pragma unroll
     for (int ty = 0; ty < 2; ++ty) {
         c[ty] = t < nt ? k.cnt[2 * (tb + t) + ty] : 0;
-        sc[ty].c = min(c[ty], 2);
-        sc[ty].l1 = c[ty] >= 1 ? (int64_t)(tb + t) * cap + (c[ty] - 1) : -1;
-        sc[ty].l2 = c[ty] >= 2 ? (int64_t)(tb + t) * cap + (c[ty] - 2) : -1;
+        sc[ty] = lt_none();
+        first[ty] = -1;
+        if (c[ty] > 0) {
+            uint64_t m[kWR];
+            load_masks(k, tb + t, ty, m);
+            first[ty] = mask_first(m);
+            const int b1 = mask_last(m);
+            m[b1 / kWave] ^= 1ull << (b1 % kWave);
+            sc[ty].c = min(c[ty], 2);
+            sc[ty].l1 = (int64_t)t * kWT + b1;
+            sc[ty].l2 = c[ty] >= 2 ? (int64_t)t * kWT + mask_last(m) : -1;
+        }
         inc[ty] = c[ty];
         mn[ty] = c[ty] > 0 ? t : INF;
     }
@@ -262,6 +351,7 @@ __global__ __launch_bounds__(kPT) void k_knot_prefix_1(SeriesDev sd, KnotsDev<re
             s_lt[ty][wv] = sc[ty];
         }
         if (lane == 0) s_min[ty][wv] = mn[ty];
+        s_first[ty][t] = first[ty];
     }
     __syncthreads();
     if (wv == 0) {
@@ -297,17 +387,15 @@ __global__ __launch_bounds__(kPT) void k_knot_prefix_1(SeriesDev sd, KnotsDev<re
     if (t < nt) {
 #pragma unroll
         for (int ty = 0; ty < 2; ++ty) {
-            const int64_t so = ty ? sd.sstride : 0;
             const int ex = s_ex_tot[ty][wv] + (lane > 0 ? xi[ty] : 0);
             const LastTwo before = lt_combine(s_ex_lt[ty][wv], lane > 0 ? xl[ty] : lt_none());
             const int after = min(s_ex_min[ty][wv], lane < kWave - 1 ? xm[ty] : INF);
-            const int64_t sl = so + (int64_t)(tb + (after == INF ? 0 : after)) * cap;
-            const int x1 = before.l1 >= 0 ? k.x[so + before.l1] : -1;
-            const real y1 = before.l1 >= 0 ? k.y[so + before.l1] : (real)0;
-            const int x2 = before.l2 >= 0 ? k.x[so + before.l2] : -1;
-            const real y2 = before.l2 >= 0 ? k.y[so + before.l2] : (real)0;
-            const int x3 = after == INF ? -1 : k.x[sl];
-            const real y3 = after == INF ? (real)0 : k.y[sl];
+            const int x1 = before.l1 >= 0 ? (int)before.l1 : -1;
+            const real y1 = before.l1 >= 0 ? hs[before.l1] : (real)0;
+            const int x2 = before.l2 >= 0 ? (int)before.l2 : -1;
+            const real y2 = before.l2 >= 0 ? hs[before.l2] : (real)0;
+            const int x3 = after == INF ? -1 : after * kWT + s_first[ty][after];
+            const real y3 = after == INF ? (real)0 : hs[x3];
             k.pre[2 * (tb + t) + ty] = ex;
             s_pre[ty][t] = ex;
             const int64_t o = 6 * (int64_t)(tb + t) + 3 * ty;
@@ -429,19 +517,20 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_wave_barrier();
 }
 
-// Copy knots [glo, ghi) of (series s, type) from the tile slots into LDS
-// (sp/sy index = g - lo).  One wavefront; tpre is 64 ints of LDS scratch.
-// Tiles are taken 64 at a time (their knot prefixes in LDS); each lane finds
-// the tiles of its knots g + lane + 64 j by binary search first, then issues
-// all its loads before storing any, so the loads overlap.  NIT >= the knots
-// of one window divided by 64.
+// Positions and values of knots [glo, ghi) of (series s, type) into LDS
+// (sp/sy index = g - lo).  One wavefront; tpre (64 ints) and tmask (64 x 4
+// words) are LDS scratch.  Tiles are taken 64 at a time (their knot prefixes
+// and bitmasks in LDS); each lane finds the tile of its knots g + lane + 64 j
+// by binary search and the knot's bit in the tile's masks, then issues all
+// its sample loads (hs = the series' pass input) before storing any.
+// NIT >= the knots of one window divided by 64.
 template <typename real, int NIT>
-__device__ void gather_knots(const SeriesDev& sd, const KnotsDev<real>& k, int s, int type,
-                             long lo, int glo, int ghi, int* sp, real* sy, int* tpre, int hint)
+__device__ void gather_knots(const SeriesDev& sd, const KnotsDev<real>& k, const real* hs, int N,
+                             int s, int type, long lo, int glo, int ghi, int* sp, real* sy,
+                             int* tpre, uint64_t (*tmask)[kWR], int hint)
 {
     const int lane = threadIdx.x & 63;
     const int tb = sd.tile0[s], ntl = sd.ntl[s];
-    const int64_t so = type ? sd.sstride : 0;
     // the tile holding knot glo: given by the prefix kernel, else a 64-ary search
     int a = hint >= 0 ? hint : 0, b = hint >= 0 ? hint + 1 : ntl;
     while (b - a > 1) {
@@ -461,6 +550,12 @@ __device__ void gather_knots(const SeriesDev& sd, const KnotsDev<real>& k, int s
         const int p = valid ? k.pre[2 * (tb + tl) + type] : 0x7FFFFFFF;
         const int c = valid ? k.cnt[2 * (tb + tl) + type] : 0;
         tpre[lane] = p;
+        if (valid) {
+            uint64_t m[kWR];
+            load_masks(k, tb + tl, type, m);
+#pragma unroll
+            for (int r = 0; r < kWR; ++r) tmask[lane][r] = m[r];
+        }
         const uint64_t vm = wave_ballot(valid);
         if (vm == 0ull) break;   // counts inconsistent with n: leave the rest
         const int last = 63 - __builtin_clzll(vm);
@@ -468,11 +563,11 @@ __device__ void gather_knots(const SeriesDev& sd, const KnotsDev<real>& k, int s
         const int gend = min(ghi, pend);
         wave_sync();
         for (int gb = g; gb < gend; gb += kWave * NIT) {
-            int64_t slot[NIT];
+            int pos[NIT];
 #pragma unroll
             for (int j = 0; j < NIT; ++j) {
                 const int gi = gb + lane + kWave * j;
-                slot[j] = -1;
+                pos[j] = -1;
                 if (gi < gend) {
                     int x = 0, y = last;
                     while (x < y) {
@@ -480,21 +575,21 @@ __device__ void gather_knots(const SeriesDev& sd, const KnotsDev<real>& k, int s
                         if (tpre[mid] <= gi) x = mid;
                         else y = mid - 1;
                     }
-                    slot[j] = so + (int64_t)(tb + tile + x) * sd.cap + (gi - tpre[x]);
+                    uint64_t m[kWR];
+#pragma unroll
+                    for (int r = 0; r < kWR; ++r) m[r] = tmask[x][r];
+                    pos[j] = min((tile + x) * kWT + mask_select(m, gi - tpre[x]), N - 1);
                 }
             }
-            int xv[NIT];
+            // all sample loads are issued before any store
             real yv[NIT];
 #pragma unroll
-            for (int j = 0; j < NIT; ++j) {
-                xv[j] = slot[j] >= 0 ? k.x[slot[j]] : 0;
-                yv[j] = slot[j] >= 0 ? k.y[slot[j]] : (real)0;
-            }
+            for (int j = 0; j < NIT; ++j) yv[j] = pos[j] >= 0 ? hs[pos[j]] : (real)0;
 #pragma unroll
             for (int j = 0; j < NIT; ++j) {
                 const int gi = gb + lane + kWave * j;
-                if (slot[j] >= 0) {
-                    sp[gi - lo] = xv[j];
+                if (pos[j] >= 0) {
+                    sp[gi - lo] = pos[j];
                     sy[gi - lo] = yv[j];
                 }
             }
@@ -586,8 +681,9 @@ __device__ __forceinline__ void partition_pcr(int lane, RowFn row, real (&x)[B])
     for (int q = 0; q < B; ++q) x[q] = (q == B - 1) ? z : (al[q] + be[q] * zl) + ga[q] * z;
 }
 
+// (four waves per SIMD: the compiler is held to 128 registers)
 template <typename real, int B, int H>
-__global__ __launch_bounds__(kThreads) void k_solve(SolveArgs<real> a)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_solve(SolveArgs<real> a)
 {
     constexpr int W = kWave * B;
     constexpr int K = W - 2 * H;
@@ -595,6 +691,7 @@ __global__ __launch_bounds__(kThreads) void k_solve(SolveArgs<real> a)
     __shared__ int s_pos[NW][W + 2];
     __shared__ real s_y[NW][W + 2];
     __shared__ int s_tpre[NW][kWave];
+    __shared__ uint64_t s_tmask[NW][kWave][kWR];
     const int wv = wave_index(), lane = threadIdx.x & 63;
     int* sp = s_pos[wv];
     real* sy = s_y[wv];
@@ -612,8 +709,9 @@ __global__ __launch_bounds__(kThreads) void k_solve(SolveArgs<real> a)
         const long lo = g0 - 1;
         const int glo = (int)max(lo, 0L);
         const int ghi = (int)min(lo + W + 2, (long)n);
-        gather_knots<real, (W + 2 + kWave - 1) / kWave>(a.sd, a.k, s, type, lo, glo, ghi, sp, sy,
-                                                         s_tpre[wv], hint);
+        gather_knots<real, (W + 2 + kWave - 1) / kWave>(a.sd, a.k, a.h + a.sd.off[s], a.sd.len[s], s,
+                                                         type, lo, glo, ghi, sp, sy, s_tpre[wv],
+                                                         s_tmask[wv], hint);
         wave_sync();
         // endpoint fix-up when n >= 6 (setBoundaryMax/Min, cudaICEEMDAN.cu:209-291);
         // idempotent, so every window that holds an end applies it
@@ -784,10 +882,6 @@ __global__ __launch_bounds__(1024) void k_series_var(const real* base, const int
 // sample's interval), the knot table [pre-2, pre+cnt] of both types lives in
 // the wave's own LDS slice, and no workgroup barrier is used, so the four
 // waves of a workgroup run independent segments.
-constexpr int kWT = 256;               // samples per tile
-constexpr int kWR = kWT / kWave;       // 64-sample chunks per tile
-constexpr int kWKC = kWT / 2 + 8;      // table entries per type (>= cnt + 3)
-constexpr int kWNC = (kWKC + kWave - 1) / kWave;
 
 // Lane 0 stores the eight wave-uniform row masks m[type][row] (no lane-
 // dependent indexing, which would put the array in scratch memory).
@@ -1034,9 +1128,8 @@ __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs
                     const real L = horner(tc[1][jl], (real)(p - tp[1][jl]));
                     hnr = cur.hr - (use_mean ? (U + L) / (real)2 : (real)0);
                 }
-                // ---- knots of h_new into the tile's slots (select_extrema_*, :150-207)
-                // and their row bitmasks (the next pass's knot flags)
-                const int64_t slot0 = (int64_t)gt * sd.cap;
+                // ---- knots of h_new (select_extrema_*, :150-207): their row
+                // bitmasks and counts (the next pass's knots)
                 int run[2] = {0, 0};
                 uint64_t em[2][kWR];
 #pragma unroll
@@ -1053,11 +1146,6 @@ __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs
 #pragma unroll
                     for (int ty = 0; ty < 2; ++ty) {
                         em[ty][r] = wave_ballot(f[ty]);
-                        if (f[ty]) {
-                            const int64_t o = (ty ? sd.sstride : 0) + slot0 + run[ty] + lanes_below(em[ty][r]);
-                            a.kout.x[o] = p;
-                            a.kout.y[o] = v;
-                        }
                         run[ty] += __popcll(em[ty][r]);
                     }
                 }
@@ -1133,7 +1221,6 @@ __global__ __launch_bounds__(kThreads) void k_init_w(SeriesDev sd, InitArgs<real
     }
     const real xl = t0 >= 1 ? val(t0 - 1) : (real)0;
     const real xr = t0 + kWT < N ? val(t0 + kWT) : (real)0;
-    const int64_t slot0 = (int64_t)gt * sd.cap;
     int run[2] = {0, 0};
     uint64_t em[2][kWR];
 #pragma unroll
@@ -1150,11 +1237,6 @@ __global__ __launch_bounds__(kThreads) void k_init_w(SeriesDev sd, InitArgs<real
 #pragma unroll
         for (int ty = 0; ty < 2; ++ty) {
             em[ty][r] = wave_ballot(f[ty]);
-            if (f[ty]) {
-                const int64_t o = (ty ? sd.sstride : 0) + slot0 + run[ty] + lanes_below(em[ty][r]);
-                kout.x[o] = p;
-                kout.y[o] = v;
-            }
             run[ty] += __popcll(em[ty][r]);
         }
     }
@@ -1792,15 +1874,15 @@ hipError_t launch_init(const SeriesDev& sd, const InitArgs<real>& ia, real* h_ou
 }
 
 template <typename real>
-hipError_t launch_knot_prefix(const SeriesDev& sd, const KnotsDev<real>& k, const PassCtl& pc,
-                              const int* active, hipStream_t st)
+hipError_t launch_knot_prefix(const SeriesDev& sd, const KnotsDev<real>& k, const real* h,
+                              const PassCtl& pc, const int* active, hipStream_t st)
 {
     if (sd.max_tiles <= kPT)
-        hipLaunchKernelGGL((k_knot_prefix_1<real>), dim3(sd.nseries), dim3(kPT), 0, st, sd, k, pc,
-                           active);
+        hipLaunchKernelGGL((k_knot_prefix_1<real>), dim3(sd.nseries), dim3(kPT), 0, st, sd, k, h,
+                           pc, active);
     else
-        hipLaunchKernelGGL((k_knot_prefix_b<real>), dim3(sd.nseries), dim3(kPT), 0, st, sd, k, pc,
-                           active);
+        hipLaunchKernelGGL((k_knot_prefix_b<real>), dim3(sd.nseries), dim3(kPT), 0, st, sd, k, h,
+                           pc, active);
     return hipGetLastError();
 }
 
@@ -1980,7 +2062,8 @@ hipError_t launch_check_index(const int* d_x, int64_t N, int* err, hipStream_t s
                                           const KnotsDev<real>&, const PassCtl&, hipStream_t);     \
     template hipError_t launch_sift<real>(const SiftArgs<real>&, int, hipStream_t);                \
     template hipError_t launch_knot_prefix<real>(const SeriesDev&, const KnotsDev<real>&,          \
-                                                 const PassCtl&, const int*, hipStream_t);         \
+                                                 const real*, const PassCtl&, const int*,          \
+                                                 hipStream_t);                                     \
     template hipError_t launch_solve<real>(const SolveArgs<real>&, int, hipStream_t);              \
     template hipError_t launch_rilling_decide<real>(const SeriesDev&, const int*, const int*,      \
                                                     const int*, int*, int*, int*, int, int,        \

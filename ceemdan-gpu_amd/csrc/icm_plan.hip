@@ -162,10 +162,9 @@ struct DevSeries {
     int64_t *off = nullptr, *kbase = nullptr, *eoff = nullptr;
     int nseries = 0, ntiles = 0, max_len = 0, max_tiles = 0;
     int64_t samples = 0, kcap = 0, maxwin = 0;
-    SeriesDev view(int64_t kstride, int64_t sstride, int cap) const
+    SeriesDev view(int64_t kstride, int cap) const
     {
         SeriesDev v;
-        v.sstride = sstride;
         v.cap = cap;
         v.tile_order = tile_order; v.tile_series = tile_series; v.tile0 = tile0; v.ntl = ntl; v.off = off; v.len = len;
         v.kbase = kbase; v.chan = chan; v.eoff = eoff; v.nidx = nidx;
@@ -186,7 +185,6 @@ struct PlanBase {
     HostSeries hn, hsig;
     DevSeries dn, dsig;
     int64_t kstride = 0, maxwin = 0, ntiles_max = 0, nser_max = 0, samples_max = 0;
-    int64_t sstride = 0;   // knot slots per type (tiles x cap)
     int cap = 0;           // knot slots per tile and type
     int64_t chan_total = 0;
     std::vector<void*> allocs;
@@ -377,7 +375,6 @@ void layout(PlanBase& P)
     P.nser_max = std::max((int64_t)P.hn.len.size(), (int64_t)P.hsig.len.size());
     P.samples_max = std::max(P.hn.samples, P.hsig.samples);
     P.cap = T / 2 + 2;   // strict extrema are never adjacent, plus the two ends
-    P.sstride = P.ntiles_max * P.cap;
 }
 
 int64_t plan_bytes_estimate(const icm_config* c)
@@ -403,7 +400,6 @@ int64_t plan_bytes_estimate(const icm_config* c)
     b += 2 * P.samples_max * s;                    // h ping-pong
     b += (int64_t)(M - 1) * P.hn.samples * s;      // noise modes
     b += P.hn.samples * s;                         // noise residual
-    b += 2 * 2 * P.sstride * (4 + s);              // knot slots x,y ping-pong
     b += 2 * P.kstride * s;                        // c
     b += 2 * 2 * 2 * P.ntiles_max * 4 + 2 * 2 * P.nser_max * (4 + 2 * s);
     b += 2 * P.ntiles_max * (6 * (4 + s) + 8 * 8);  // neighbour knots, knot bitmasks
@@ -422,8 +418,6 @@ int64_t plan_bytes_estimate(const icm_config* c)
 template <typename real>
 struct Plan : PlanBase {
     real *hA = nullptr, *hB = nullptr, *modes = nullptr, *wres = nullptr;
-    int* kx[2] = {nullptr, nullptr};
-    real* ky[2] = {nullptr, nullptr};
     int* kcnt[2] = {nullptr, nullptr};
     int* kpre[2] = {nullptr, nullptr};
     int* nk[2] = {nullptr, nullptr};
@@ -461,8 +455,6 @@ struct Plan : PlanBase {
         if ((rc = alloc(&modes, (int64_t)(M - 1) * hn.samples))) return rc;
         if ((rc = alloc(&wres, hn.samples))) return rc;
         for (int b = 0; b < 2; ++b) {
-            if ((rc = alloc(&kx[b], 2 * sstride))) return rc;
-            if ((rc = alloc(&ky[b], 2 * sstride))) return rc;
             if ((rc = alloc(&kcnt[b], 2 * ntiles_max))) return rc;
             if ((rc = alloc(&kpre[b], 2 * ntiles_max))) return rc;
             if ((rc = alloc(&nk[b], 2 * nser_max))) return rc;
@@ -526,8 +518,6 @@ struct Plan : PlanBase {
     KnotsDev<real> knots(int b)
     {
         KnotsDev<real> k;
-        k.x = kx[b];
-        k.y = ky[b];
         k.cnt = kcnt[b];
         k.pre = kpre[b];
         k.n = nk[b];
@@ -569,7 +559,7 @@ struct Plan : PlanBase {
     int stage_fixed(int phase, int stage, const DevSeries& d, const InitArgs<real>& ia, int S,
                     real* out, hipStream_t st, Rec* record = nullptr, bool full = false)
     {
-        const SeriesDev sd = d.view(kstride, sstride, cap);
+        const SeriesDev sd = d.view(kstride, cap);
         int rc;
         if (epoch + (uint32_t)S + 8 >= kEpochs - 1 && (rc = reset_control(st))) return rc;
         const int64_t pass_bytes = 2 * (int64_t)sizeof(real) * d.samples;
@@ -577,13 +567,14 @@ struct Plan : PlanBase {
         if ((rc = prof_begin(0, pass_bytes, st))) return rc;
         ICM_TRY(launch_init<real>(sd, ia, hA, knots(0), ctl(e_emit), st));
         if ((rc = prof_end(st))) return rc;
-        ICM_TRY(launch_knot_prefix<real>(sd, knots(0), ctl(e_emit), nullptr, st));
+        ICM_TRY(launch_knot_prefix<real>(sd, knots(0), hA, ctl(e_emit), nullptr, st));
         if ((rc = trace_copy(phase, stage, 0, d, 0, st))) return rc;
         for (int j = 0; j < S; ++j) {
             const int b = j & 1;
             SolveArgs<real> sa{};
             sa.sd = sd;
             sa.k = knots(b);
+            sa.h = b ? hB : hA;
             sa.c = kc;
             sa.winlist = winlist;
             sa.wincount = wincount + e_emit;
@@ -625,7 +616,7 @@ struct Plan : PlanBase {
             ICM_TRY(launch_sift<real>(a, d.nsegs, st));
             if ((rc = prof_end(st))) return rc;
             if (j < S - 1) {
-                ICM_TRY(launch_knot_prefix<real>(sd, knots(b ^ 1), ctl(e_emit), nullptr, st));
+                ICM_TRY(launch_knot_prefix<real>(sd, knots(b ^ 1), a.h_out, ctl(e_emit), nullptr, st));
                 if ((rc = trace_copy(phase, stage, j + 1, d, b ^ 1, st))) return rc;
             }
         }
@@ -638,7 +629,7 @@ struct Plan : PlanBase {
     int stage_rilling(int phase, int stage, const DevSeries& d, const InitArgs<real>& ia, int max_pass,
                       hipStream_t st)
     {
-        const SeriesDev sd = d.view(kstride, sstride, cap);
+        const SeriesDev sd = d.view(kstride, cap);
         ICM_TRY(launch_fill_u32((uint32_t*)active, 1u, d.nseries, st));
         ICM_TRY(launch_fill_u32((uint32_t*)subs, 0u, d.nseries, st));
         ICM_TRY(launch_fill_u32((uint32_t*)stop_pass, 0u, d.nseries, st));
@@ -648,7 +639,7 @@ struct Plan : PlanBase {
         }
         uint32_t e_emit = next_epoch();
         ICM_TRY(launch_init<real>(sd, ia, hA, knots(0), ctl(e_emit), st));
-        ICM_TRY(launch_knot_prefix<real>(sd, knots(0), ctl(e_emit), nullptr, st));
+        ICM_TRY(launch_knot_prefix<real>(sd, knots(0), hA, ctl(e_emit), nullptr, st));
         int rc;
         if ((rc = trace_copy(phase, stage, 0, d, 0, st))) return rc;
         for (int j = 0; j < max_pass; ++j) {
@@ -656,6 +647,7 @@ struct Plan : PlanBase {
             SolveArgs<real> sa{};
             sa.sd = sd;
             sa.k = knots(b);
+            sa.h = b ? hB : hA;
             sa.c = kc;
             sa.winlist = winlist;
             sa.wincount = wincount + e_emit;
@@ -681,7 +673,7 @@ struct Plan : PlanBase {
             ICM_TRY(launch_fill_u32((uint32_t*)n_active, 0u, 1, st));
             ICM_TRY(launch_rilling_decide<real>(sd, nk[b], sx_cnt, sx_mark, active, stop_pass, subs,
                                                 j, max_pass, n_active, st));
-            ICM_TRY(launch_knot_prefix<real>(sd, knots(b ^ 1), ctl(e_emit), active, st));
+            ICM_TRY(launch_knot_prefix<real>(sd, knots(b ^ 1), a.h_out, ctl(e_emit), active, st));
             if ((rc = trace_copy(phase, stage, j + 1, d, b ^ 1, st))) return rc;
             // early exit: results are identical to running every pass to the
             // cap with finished series masked (they never change again)
@@ -716,7 +708,7 @@ struct Plan : PlanBase {
             } else {
                 if ((rc = stage_rilling(ICM_PHASE_NOISE, k, dn, ia, cfg.noise_max_sifts, st)))
                     return rc;
-                ICM_TRY(launch_select_copy<real>(dn.view(kstride, sstride, cap), dn.max_len, hA, hB, stop_pass,
+                ICM_TRY(launch_select_copy<real>(dn.view(kstride, cap), dn.max_len, hA, hB, stop_pass,
                                                  Ek, st));
             }
             ICM_TRY(launch_sub<real>(wres, Ek, hn.samples, st));
@@ -1109,7 +1101,7 @@ struct Plan : PlanBase {
                                           gr + chan_off[c], st));
         }
         if ((rc = reset_control(st))) return rc;
-        const SeriesDev sd = dsig.view(kstride, sstride, cap);
+        const SeriesDev sd = dsig.view(kstride, cap);
         for (int k = M - 2; k >= 0; --k) {
             const real* Ek = modes + (int64_t)k * hn.samples;
             const real* rk = k == 0 ? xt : r_hist + (int64_t)k * chan_total;
