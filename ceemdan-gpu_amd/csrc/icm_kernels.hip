@@ -561,37 +561,67 @@ __device__ void gather_knots(const SeriesDev& sd, const KnotsDev<real>& k, const
         const int last = 63 - __builtin_clzll(vm);
         const int pend = __shfl(p + c, last, kWave);
         const int gend = min(ghi, pend);
+        // chunk tiles holding the first and the last knot of [g, gend)
+        const uint64_t ua = wave_ballot(valid && p <= g), ub = wave_ballot(valid && p <= gend - 1);
+        const int xa = ua ? 63 - __builtin_clzll(ua) : 0;
+        const int xb = ub ? 63 - __builtin_clzll(ub) : 0;
         wave_sync();
-        for (int gb = g; gb < gend; gb += kWave * NIT) {
+        // positions into sp
+        if (xb - xa < 16) {
+            // few tiles (dense knots): every lane tests its own bit of each
+            // tile row; the set bits' ranks give the knot indices
+            for (int x = xa; x <= xb; ++x) {
+                int cb = __builtin_amdgcn_readfirstlane(tpre[x]);
+#pragma unroll
+                for (int r = 0; r < kWR; ++r) {
+                    const uint64_t mr = tmask[x][r];
+                    const uint64_t mu = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(mr >> 32)) << 32) |
+                                        (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)mr);
+                    if (__builtin_amdgcn_inverse_ballot_w64(mu)) {
+                        const int gi = cb + lanes_below(mu);
+                        if (gi >= g && gi < gend) sp[gi - lo] = (tile + x) * kWT + kWave * r + lane;
+                    }
+                    cb += __popcll(mu);
+                }
+            }
+        } else {
+            // many tiles: each lane finds the tile of its knots by binary
+            // search and the knot's bit by selection in the tile's masks
+            for (int gb = g; gb < gend; gb += kWave * NIT) {
+#pragma unroll
+                for (int j = 0; j < NIT; ++j) {
+                    const int gi = gb + lane + kWave * j;
+                    if (gi < gend) {
+                        int x = 0, y = last;
+                        while (x < y) {
+                            const int mid = (x + y + 1) >> 1;
+                            if (tpre[mid] <= gi) x = mid;
+                            else y = mid - 1;
+                        }
+                        uint64_t m[kWR];
+#pragma unroll
+                        for (int r = 0; r < kWR; ++r) m[r] = tmask[x][r];
+                        sp[gi - lo] = (tile + x) * kWT + mask_select(m, gi - tpre[x]);
+                    }
+                }
+            }
+        }
+        wave_sync();
+        // values: every sample load is issued before any store
+        for (int i0 = (int)(g - lo); i0 < (int)(gend - lo); i0 += kWave * NIT) {
             int pos[NIT];
 #pragma unroll
             for (int j = 0; j < NIT; ++j) {
-                const int gi = gb + lane + kWave * j;
-                pos[j] = -1;
-                if (gi < gend) {
-                    int x = 0, y = last;
-                    while (x < y) {
-                        const int mid = (x + y + 1) >> 1;
-                        if (tpre[mid] <= gi) x = mid;
-                        else y = mid - 1;
-                    }
-                    uint64_t m[kWR];
-#pragma unroll
-                    for (int r = 0; r < kWR; ++r) m[r] = tmask[x][r];
-                    pos[j] = min((tile + x) * kWT + mask_select(m, gi - tpre[x]), N - 1);
-                }
+                const int i = i0 + lane + kWave * j;
+                pos[j] = i < gend - lo ? min(max(sp[i], 0), N - 1) : -1;
             }
-            // all sample loads are issued before any store
             real yv[NIT];
 #pragma unroll
             for (int j = 0; j < NIT; ++j) yv[j] = pos[j] >= 0 ? hs[pos[j]] : (real)0;
 #pragma unroll
             for (int j = 0; j < NIT; ++j) {
-                const int gi = gb + lane + kWave * j;
-                if (pos[j] >= 0) {
-                    sp[gi - lo] = pos[j];
-                    sy[gi - lo] = yv[j];
-                }
+                const int i = i0 + lane + kWave * j;
+                if (pos[j] >= 0) sy[i] = yv[j];
             }
         }
         g = gend;
