@@ -462,10 +462,13 @@ struct alignas(2 * sizeof(real)) Coef {
     real a, b, c, d;
 };
 
+// Horner steps as fused multiply-adds (single rounding each), the way nvcc
+// contracts the reference's a + (b + (c + d t) t) t (cudaICEEMDAN.cu:436);
+// the oracle evaluates the same fma chain.
 template <typename real>
 __device__ __forceinline__ real horner(const Coef<real>& q, real t)
 {
-    return q.a + (q.b + (q.c + q.d * t) * t) * t;
+    return fma(fma(fma(q.d, t, q.c), t, q.b), t, q.a);
 }
 
 // ------------------------------------------------------- windowed spline solve

@@ -4,7 +4,7 @@
  * Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as
  * the checker.  The product library (ceemdan-gpu_amd/) never links it.
  *
- * Build: gcc -O2 -ffp-contract=off -fopenmp (see oracle/build_oracle.py).
+ * Build: gcc -O2 -ffp-contract=off -mfma -fopenmp (see oracle/build_oracle.py).
  * No FMA contraction and no fast-math anywhere: every expression below is
  * evaluated in IEEE double, in the written order, exactly as the HIP kernels
  * evaluate it, so FP64 results are bit-identical to the GPU path.
@@ -328,7 +328,9 @@ void ocm_envelope(int N, int n, const int* pos, const double* val, const double*
             interval_coeffs(pos, val, c, k, &b, &d);
         }
         double t = (double)(p - pos[k]);
-        env[p] = val[k] + (b + (c[k] + d * t) * t) * t;
+        /* fused multiply-adds, as nvcc contracts the reference's expression
+           (:436) and as the HIP kernels evaluate it */
+        env[p] = fma(fma(fma(d, t, c[k]), t, b), t, val[k]);
     }
 }
 
