@@ -1012,6 +1012,9 @@ __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs
     // that nothing loaded before the loop is outstanding inside it, so the
     // loop body never waits on the prefetch of the next tile
     wait_vmem();
+    // h_new of the sample just before the tile: the previous tile of the
+    // segment computed it (same interval, same coefficients, same bits)
+    real hcarry = (real)0;
 
     for (int i = 0; i < seg.count; ++i) {
         const int t = seg.t_first + i;
@@ -1145,8 +1148,8 @@ __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs
 
             if (a.emit) {
                 // h_new just outside the tile, evaluated by every lane (same LDS words)
-                real hnl = (real)0, hnr = (real)0;
-                if (FULL || t0 >= 1) {
+                real hnl = hcarry, hnr = (real)0;
+                if (i == 0 && (FULL || t0 >= 1)) {
                     const int p = t0 - 1;
                     const int ju = tp[0][1] == p ? 0 : 1;
                     const int jl = tp[1][1] == p ? 0 : 1;
@@ -1188,6 +1191,7 @@ __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs
                     a.kout.cnt[2 * gt + 1] = run[1];
                 }
             }
+            hcarry = lane_value(hn[kWR - 1], kWave - 1);
             return true;
         };
         const bool interior = t0 > 0 && t0 + kWT < N;
