@@ -44,7 +44,7 @@ WORKLOADS = {
     # one GPU's share is 16 channels (all 128 in one plan exceed the 2^32
     # work-items of one launch).  Per-pass knot records would take 1.26 TB,
     # so the backward re-sifts each stage.
-    "c5": dict(I=500, M=16, S=10, dtype="f32", records=False,
+    "c5": dict(I=500, M=16, S=10, dtype="f32", records=True,
                desc="one GPU's share of the 128-channel job: channels 0-15 of 128 EEG-like x "
                     "N=244032 (+50 Hz 5 uV, seeds 6000+c), I=500, M=16, 10 sifts, float32; "
                     "forward + adjoint of L = sum_k w_k |IMF_k|^2 (backward re-sifts)"),

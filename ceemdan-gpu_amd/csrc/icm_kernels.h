@@ -113,7 +113,6 @@ struct SolveArgs {
     const unsigned* wincount;
     const int* active;
     int* err;
-    int* px;                // optional [2][kstride]: knot positions, compact (for the adjoint)
     int* fixf;              // optional [nseries][2][2]: endpoint fix applied (for the adjoint)
     int nak;                // not-a-knot spline ends
 };
@@ -121,11 +120,12 @@ struct SolveArgs {
 // ---------------------------------------------------------------- adjoint
 // Recorded structure of one forward pass (the knots of its input samples).
 struct PassRec {
-    int* px;     // [2][kstride] knot positions by knot index
-    int* cnt;    // [ntiles][2]
-    int* pre;    // [ntiles][2]
-    int* n;      // [nseries][2]
-    int* fixf;   // [nseries][2][2]
+    const uint64_t* mask;   // [ntiles][2][4] knot bitmasks
+    const int* left;        // [ntiles][2] position of the last knot before the tile (-1: none)
+    const int* cnt;         // [ntiles][2]
+    const int* pre;         // [ntiles][2]
+    const int* n;           // [nseries][2]
+    const int* fixf;        // [nseries][2][2]
 };
 
 template <typename real>
@@ -215,6 +215,7 @@ hipError_t launch_copy(real* dst, const real* src, int64_t n, hipStream_t st);
 template <typename real>
 hipError_t launch_select_copy(const SeriesDev& sd, int max_len, const real* hA, const real* hB,
                               const int* parity, real* dst, hipStream_t st);
+hipError_t launch_rec_left(const int* nbx, int* left, int ntiles, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st);
 hipError_t launch_check_index(const int* d_x, int64_t N, int* err, hipStream_t st);
 

@@ -520,26 +520,27 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_wave_barrier();
 }
 
-// Positions and values of knots [glo, ghi) of (series s, type) into LDS
-// (sp/sy index = g - lo).  One wavefront; tpre (64 ints) and tmask (64 x 4
-// words) are LDS scratch.  Tiles are taken 64 at a time (their knot prefixes
-// and bitmasks in LDS); each lane finds the tile of its knots g + lane + 64 j
-// by binary search and the knot's bit in the tile's masks, then issues all
-// its sample loads (hs = the series' pass input) before storing any.
-// NIT >= the knots of one window divided by 64.
-template <typename real, int NIT>
-__device__ void gather_knots(const SeriesDev& sd, const KnotsDev<real>& k, const real* hs, int N,
-                             int s, int type, long lo, int glo, int ghi, int* sp, real* sy,
-                             int* tpre, uint64_t (*tmask)[kWR], int hint)
+// Positions of knots [glo, ghi) of (series s, type) into LDS (sp index =
+// g - lo), from the tile bitmasks (mask [tile][type][row]), knot prefixes and
+// counts.  One wavefront; tpre (64 ints) and tmask (64 x 4 words) are LDS
+// scratch.  Tiles are taken 64 at a time (their prefixes and masks in LDS).
+// When the knots of a chunk lie in few tiles (dense knots), every lane tests
+// its own bit of each tile row and the set bits' ranks give the knot
+// indices; otherwise each lane finds the tile of its knots g + lane + 64 j by
+// binary search and the knot's bit by selection.  hint = the tile holding
+// knot glo, or -1 (64-ary search).  NIT >= the knots of one window / 64.
+template <int NIT>
+__device__ void gather_positions(const SeriesDev& sd, const uint64_t* kmask, const int* kpre,
+                                 const int* kcnt, int s, int type, long lo, int glo, int ghi,
+                                 int* sp, int* tpre, uint64_t (*tmask)[kWR], int hint)
 {
     const int lane = threadIdx.x & 63;
     const int tb = sd.tile0[s], ntl = sd.ntl[s];
-    // the tile holding knot glo: given by the prefix kernel, else a 64-ary search
     int a = hint >= 0 ? hint : 0, b = hint >= 0 ? hint + 1 : ntl;
     while (b - a > 1) {
         const int step = (b - a + kWave - 1) / kWave;
         const int idx = a + lane * step;
-        const bool ok = idx < b && k.pre[2 * (tb + idx) + type] <= glo;
+        const bool ok = idx < b && kpre[2 * (tb + idx) + type] <= glo;
         const uint64_t m = wave_ballot(ok);
         const int L = 63 - __builtin_clzll(m);
         a = a + L * step;
@@ -550,14 +551,12 @@ __device__ void gather_knots(const SeriesDev& sd, const KnotsDev<real>& k, const
     while (g < ghi) {
         const int tl = tile + lane;
         const bool valid = tl < ntl;
-        const int p = valid ? k.pre[2 * (tb + tl) + type] : 0x7FFFFFFF;
-        const int c = valid ? k.cnt[2 * (tb + tl) + type] : 0;
+        const int p = valid ? kpre[2 * (tb + tl) + type] : 0x7FFFFFFF;
+        const int c = valid ? kcnt[2 * (tb + tl) + type] : 0;
         tpre[lane] = p;
         if (valid) {
-            uint64_t m[kWR];
-            load_masks(k, tb + tl, type, m);
 #pragma unroll
-            for (int r = 0; r < kWR; ++r) tmask[lane][r] = m[r];
+            for (int r = 0; r < kWR; ++r) tmask[lane][r] = kmask[8 * (int64_t)(tb + tl) + kWR * type + r];
         }
         const uint64_t vm = wave_ballot(valid);
         if (vm == 0ull) break;   // counts inconsistent with n: leave the rest
@@ -569,27 +568,23 @@ __device__ void gather_knots(const SeriesDev& sd, const KnotsDev<real>& k, const
         const int xa = ua ? 63 - __builtin_clzll(ua) : 0;
         const int xb = ub ? 63 - __builtin_clzll(ub) : 0;
         wave_sync();
-        // positions into sp
         if (xb - xa < 16) {
-            // few tiles (dense knots): every lane tests its own bit of each
-            // tile row; the set bits' ranks give the knot indices
             for (int x = xa; x <= xb; ++x) {
-                int cb = __builtin_amdgcn_readfirstlane(tpre[x]);
+                // the tile's prefix and four row masks in one round of LDS reads
+                int cb = tpre[x];
+                uint64_t mr[kWR];
+#pragma unroll
+                for (int r = 0; r < kWR; ++r) mr[r] = tmask[x][r];
 #pragma unroll
                 for (int r = 0; r < kWR; ++r) {
-                    const uint64_t mr = tmask[x][r];
-                    const uint64_t mu = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(mr >> 32)) << 32) |
-                                        (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)mr);
-                    if (__builtin_amdgcn_inverse_ballot_w64(mu)) {
-                        const int gi = cb + lanes_below(mu);
+                    if ((mr[r] >> lane) & 1ull) {
+                        const int gi = cb + lanes_below(mr[r]);
                         if (gi >= g && gi < gend) sp[gi - lo] = (tile + x) * kWT + kWave * r + lane;
                     }
-                    cb += __popcll(mu);
+                    cb += __popcll(mr[r]);
                 }
             }
         } else {
-            // many tiles: each lane finds the tile of its knots by binary
-            // search and the knot's bit by selection in the tile's masks
             for (int gb = g; gb < gend; gb += kWave * NIT) {
 #pragma unroll
                 for (int j = 0; j < NIT; ++j) {
@@ -609,37 +604,37 @@ __device__ void gather_knots(const SeriesDev& sd, const KnotsDev<real>& k, const
                 }
             }
         }
-        wave_sync();
-        // values: every sample load is issued before any store
-        for (int i0 = (int)(g - lo); i0 < (int)(gend - lo); i0 += kWave * NIT) {
-            int pos[NIT];
-#pragma unroll
-            for (int j = 0; j < NIT; ++j) {
-                const int i = i0 + lane + kWave * j;
-                pos[j] = i < gend - lo ? min(max(sp[i], 0), N - 1) : -1;
-            }
-            real yv[NIT];
-#pragma unroll
-            for (int j = 0; j < NIT; ++j) yv[j] = pos[j] >= 0 ? hs[pos[j]] : (real)0;
-#pragma unroll
-            for (int j = 0; j < NIT; ++j) {
-                const int i = i0 + lane + kWave * j;
-                if (pos[j] >= 0) sy[i] = yv[j];
-            }
-        }
         g = gend;
         tile += kWave;
         wave_sync();
     }
 }
 
-// Solve one window of W = 64*B rows of a diagonally dominant tridiagonal
-// system, one wavefront, lane l owning rows [l*B, l*B+B): per-lane
-// elimination of the B-1 interior rows (x_q = al_q + be_q z_{l-1} + ga_q z_l),
-// a reduced tridiagonal system in the lanes' last rows z_l solved by parallel
-// cyclic reduction across the wave, then back substitution.  Coupling beyond
-// the window's first and last rows is dropped.  row(r, dl, d, du, f) gives
-// row r of the window.  x[q] = solution of row l*B + q.
+// Values of knots [glo, ghi) (positions already in sp) from the series' pass
+// input hs (N samples) into sy; every load is issued before any store.
+template <typename real, int NIT>
+__device__ void gather_values(const real* hs, int N, long lo, int glo, int ghi, const int* sp,
+                              real* sy)
+{
+    const int lane = threadIdx.x & 63;
+    for (int i0 = (int)(glo - lo); i0 < (int)(ghi - lo); i0 += kWave * NIT) {
+        int pos[NIT];
+#pragma unroll
+        for (int j = 0; j < NIT; ++j) {
+            const int i = i0 + lane + kWave * j;
+            pos[j] = i < ghi - lo ? min(max(sp[i], 0), N - 1) : -1;
+        }
+        real yv[NIT];
+#pragma unroll
+        for (int j = 0; j < NIT; ++j) yv[j] = pos[j] >= 0 ? hs[pos[j]] : (real)0;
+#pragma unroll
+        for (int j = 0; j < NIT; ++j) {
+            const int i = i0 + lane + kWave * j;
+            if (pos[j] >= 0) sy[i] = yv[j];
+        }
+    }
+}
+
 template <typename real, int B, class RowFn>
 __device__ __forceinline__ void partition_pcr(int lane, RowFn row, real (&x)[B])
 {
@@ -742,9 +737,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
         const long lo = g0 - 1;
         const int glo = (int)max(lo, 0L);
         const int ghi = (int)min(lo + W + 2, (long)n);
-        gather_knots<real, (W + 2 + kWave - 1) / kWave>(a.sd, a.k, a.h + a.sd.off[s], a.sd.len[s], s,
-                                                         type, lo, glo, ghi, sp, sy, s_tpre[wv],
-                                                         s_tmask[wv], hint);
+        constexpr int NIT = (W + 2 + kWave - 1) / kWave;
+        gather_positions<NIT>(a.sd, a.k.mask, a.k.pre, a.k.cnt, s, type, lo, glo, ghi, sp,
+                              s_tpre[wv], s_tmask[wv], hint);
+        wave_sync();
+        gather_values<real, NIT>(a.h + a.sd.off[s], a.sd.len[s], lo, glo, ghi, sp, sy);
         wave_sync();
         // endpoint fix-up when n >= 6 (setBoundaryMax/Min, cudaICEEMDAN.cu:209-291);
         // idempotent, so every window that holds an end applies it
@@ -804,7 +801,6 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
             const long gg = g0 + r;
             if (r >= H && r < H + K && gg < n) {
                 if (!(nak && (gg == 0 || gg == n - 1))) cout[gg] = xs[q];
-                if (a.px) a.px[kb + gg] = sp[r + 1];
             }
         }
         if (nak) {
@@ -1309,7 +1305,6 @@ __global__ __launch_bounds__(kThreads) void k_adj_tile(AdjTileArgs<real> a)
     constexpr int NOKEY = 1 << 28;
     __shared__ real s_delta[NW][T];
     __shared__ int s_tp[NW][KC];
-    __shared__ unsigned long long s_bits[NW][T / 64];
     const int lane = threadIdx.x & 63, wv = wave_index();
     const int gt = blockIdx.x * NW + wv;
     const SeriesDev& sd = a.sd;
@@ -1322,7 +1317,6 @@ __global__ __launch_bounds__(kThreads) void k_adj_tile(AdjTileArgs<real> a)
     real* mu = a.mu + sd.off[s];
     real* delta = s_delta[wv];
     int* tp = s_tp[wv];
-    unsigned long long* bits = s_bits[wv];
     const int ub = 4 * lane;
 
     real v[4];
@@ -1336,20 +1330,26 @@ __global__ __launch_bounds__(kThreads) void k_adj_tile(AdjTileArgs<real> a)
 #pragma unroll
         for (int q = 0; q < 4; ++q) v[q] = ub + q < nt ? mu[t0 + ub + q] : (real)0;
     }
-    // ---- scatter of the knot gradients of pass sc
-    if (a.sc.px && a.sc.n[2 * s] > 3 && a.sc.n[2 * s + 1] > 3) {
+    // ---- scatter of the knot gradients of pass sc: the lane of each knot
+    // bit (sample 64 r + lane of the tile) adds the gradient of its knot
+    if (a.sc.mask && a.sc.n[2 * s] > 3 && a.sc.n[2 * s + 1] > 3) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) delta[ub + q] = (real)0;
         wave_sync();
         for (int ty = 0; ty < 2; ++ty) {
             const int64_t kb = sd.kbase[s] + (ty ? sd.kstride : 0);
-            const int cnt = a.sc.cnt[2 * gt + ty], pre = a.sc.pre[2 * gt + ty];
-            for (int i = lane; i < cnt; i += kWave) {
-                const int x = a.sc.px[kb + pre + i] - t0;
-                delta[x] = delta[x] + a.ga[kb + pre + i];
+            int cb = a.sc.pre[2 * gt + ty];
+#pragma unroll
+            for (int r = 0; r < kWR; ++r) {
+                const uint64_t m = load_scalar(a.sc.mask + 8 * (int64_t)gt + kWR * ty + r);
+                if (__builtin_amdgcn_inverse_ballot_w64(m)) {
+                    const int u = kWave * r + lane;
+                    delta[u] = delta[u] + a.ga[kb + cb + lanes_below(m)];
+                }
+                cb += __popcll(m);
             }
-            wave_sync();
         }
+        wave_sync();
 #pragma unroll
         for (int q = 0; q < 4; ++q) v[q] = v[q] - delta[ub + q] / (real)2;
         dirty = true;
@@ -1360,18 +1360,24 @@ __global__ __launch_bounds__(kThreads) void k_adj_tile(AdjTileArgs<real> a)
             if (ub + q < nt) mu[t0 + ub + q] = v[q];
     }
     // ---- moments of mu over the intervals of pass mo
-    if (!(a.mo.px && a.mo.n[2 * s] > 3 && a.mo.n[2 * s + 1] > 3)) return;
+    if (!(a.mo.mask && a.mo.n[2 * s] > 3 && a.mo.n[2 * s + 1] > 3)) return;
     for (int ty = 0; ty < 2; ++ty) {
         const int64_t kb = sd.kbase[s] + (ty ? sd.kstride : 0);
         const int cnt = a.mo.cnt[2 * gt + ty], pre = a.mo.pre[2 * gt + ty];
-        const int left = pre > 0 ? a.mo.px[kb + pre - 1] : 0;
+        const int left = pre > 0 ? a.mo.left[2 * gt + ty] : 0;
+        // the tile's knot bits (row words) and their positions in rank order
+        uint64_t bits[kWR];
+#pragma unroll
+        for (int r = 0; r < kWR; ++r) bits[r] = load_scalar(a.mo.mask + 8 * (int64_t)gt + kWR * ty + r);
         wave_sync();
-        if (lane < T / 64) bits[lane] = 0ull;
-        wave_sync();
-        for (int i = lane; i < cnt; i += kWave) {
-            const int x = a.mo.px[kb + pre + i];
-            tp[i] = x;
-            atomicOr(&bits[(x - t0) >> 6], 1ull << ((x - t0) & 63));
+        {
+            int cb = 0;
+#pragma unroll
+            for (int r = 0; r < kWR; ++r) {
+                if (__builtin_amdgcn_inverse_ballot_w64(bits[r]))
+                    tp[cb + lanes_below(bits[r])] = t0 + kWave * r + lane;
+                cb += __popcll(bits[r]);
+            }
         }
         wave_sync();
         real* e = a.edge + 4 * (4 * (int64_t)gt + 2 * ty);
@@ -1387,11 +1393,13 @@ __global__ __launch_bounds__(kThreads) void k_adj_tile(AdjTileArgs<real> a)
         // the four samples of a lane lie in knot word ub >> 6: its bits and
         // the knots of the words before it
         const int wd = ub >> 6;
-        const unsigned long long wbits = bits[wd];
+        uint64_t wbits = bits[0];
         int wbefore = 0;
 #pragma unroll
-        for (int w = 0; w < T / 64; ++w)
-            if (w < wd) wbefore += __popcll(bits[w]);
+        for (int w = 1; w < kWR; ++w) {
+            wbefore += wd >= w ? __popcll(bits[w - 1]) : 0;
+            wbits = wd == w ? bits[w] : wbits;
+        }
         auto key_of = [&](int u) -> int {
             if (u >= nt) return NOKEY;
             if (t0 == 0 && u == 0) return 1;
@@ -1515,20 +1523,26 @@ __global__ void k_adj_span(AdjArgs<real> a)
 // moments (transposed coefficient formulas, cudaICEEMDAN.cu:394-398), the
 // solve, the transposed right-hand side f(a) (:333-341) and endpoint fix
 // (:209-291).  One wavefront per window, the forward's window geometry.
+// (three waves per SIMD: the compiler is held to 168 registers)
 template <typename real, int B, int H>
-__global__ __launch_bounds__(kThreads) void k_adj_knot(AdjArgs<real> a)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_adj_knot(AdjArgs<real> a)
 {
     constexpr int W = kWave * B;
     constexpr int K = W - 2 * H;
     constexpr int NW = kThreads / kWave;
     __shared__ real s_f[NW][W];            // knot-value gradients of the window rows
+    __shared__ int s_pos[NW][W + 2];       // knot positions g0-1 .. g0+W
+    __shared__ int s_tpre[NW][kWave];
+    __shared__ uint64_t s_tmask[NW][kWave][kWR];
     const int wv = wave_index(), lane = threadIdx.x & 63;
     real* sf = s_f[wv];
+    int* sp = s_pos[wv];
     const real third = (real)1 / (real)3;
     const unsigned total = *a.wincount;
     for (unsigned item = blockIdx.x * NW + wv; item < total; item += gridDim.x * NW) {
         const int s = a.winlist[kWinItem * item];
         const int code = a.winlist[kWinItem * item + 1];
+        const int hint = a.winlist[kWinItem * item + 2];
         const int type = code & 1;
         const int w = code >> 1;
         const int n0 = a.ps.n[2 * s], n1 = a.ps.n[2 * s + 1];
@@ -1536,6 +1550,15 @@ __global__ __launch_bounds__(kThreads) void k_adj_knot(AdjArgs<real> a)
         const int n = type ? n1 : n0;
         const int64_t kb = a.sd.kbase[s] + (type ? a.sd.kstride : 0);
         const long g0 = (long)w * K - H;
+        // knot positions of the window (and one beyond each side) from the
+        // pass's bitmasks: sp index = g - lo
+        const long lo = g0 - 1;
+        const int glo = (int)max(lo, 0L);
+        const int ghi = (int)min(lo + W + 2, (long)n);
+        gather_positions<(W + 2 + kWave - 1) / kWave>(a.sd, a.ps.mask, a.ps.pre, a.ps.cnt, s, type,
+                                                      lo, glo, ghi, sp, s_tpre[wv], s_tmask[wv],
+                                                      hint);
+        wave_sync();
         // lane l owns window rows r = l*B + q (knot gb + q); everything a row
         // needs comes from its own lane's registers or a neighbour lane's
         const long gb = g0 + (long)lane * B;
@@ -1545,7 +1568,7 @@ __global__ __launch_bounds__(kThreads) void k_adj_knot(AdjArgs<real> a)
 #pragma unroll
         for (int j = 0; j < B + 2; ++j) {
             const long g = gb - 1 + j;
-            xk[j] = a.ps.px[kb + min(max(g, 0L), (long)n - 1)];
+            xk[j] = sp[min(max(g, 0L), (long)n - 1) - lo];
         }
         // interval gb-1+j (j = 0..B): its moments feed its left knot (row j-1)
         // and its right knot (row j) through the transposed coefficient
@@ -1617,7 +1640,7 @@ __global__ __launch_bounds__(kThreads) void k_adj_knot(AdjArgs<real> a)
         wave_sync();
         // endpoint fix: a_0 = a_1 (1 + rho) - a_2 rho when it replaced a_0
         if (lane == 0 && n >= 6) {
-            const int* xp = a.ps.px + kb;
+            const int* xp = sp - lo;   // xp[g] = position of knot g (g in the window)
             const long r0 = 0 - g0;
             if (r0 >= 1 && r0 + 2 <= W - 2 && a.ps.fixf[4 * s + 2 * type]) {
                 const real rho = (real)(xp[1] - xp[0]) / (real)(xp[2] - xp[1]);
@@ -1880,6 +1903,14 @@ __global__ void k_select_copy(SeriesDev sd, const real* hA, const real* hB, cons
     }
 }
 
+// left[2 t + ty] = position of the last knot before tile t (from the
+// prefix's neighbour knots), for the adjoint's pass records
+__global__ void k_rec_left(const int* nbx, int* left, int ntiles)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 2 * ntiles) left[i] = nbx[6 * (int64_t)(i >> 1) + 3 * (i & 1)];
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2079,6 +2110,13 @@ hipError_t launch_adj_input(int nchan, const int* chan_len, const int64_t* chan_
 {
     hipLaunchKernelGGL((k_adj_input<real>), dim3(nchan), dim3(1024), 0, st, chan_len, chan_off, y,
                        gxt, gy);
+    return hipGetLastError();
+}
+
+hipError_t launch_rec_left(const int* nbx, int* left, int ntiles, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_rec_left, dim3(blocks_for(2 * (int64_t)ntiles, 256)), dim3(256), 0, st, nbx,
+                       left, ntiles);
     return hipGetLastError();
 }
 

@@ -408,7 +408,7 @@ int64_t plan_bytes_estimate(const icm_config* c)
     b += (int64_t)(M - 1) * P.chan_total * s;     // residue history
     if (c->keep_records && c->stop_mode == ICM_STOP_FIXED)   // per-pass knot records
         b += (int64_t)(M - 1) * c->max_sifts *
-             (2 * P.kstride * 4 + 4 * (int64_t)P.hsig.ntiles * 4 + 6 * (int64_t)P.hsig.len.size() * 4 +
+             ((int64_t)P.hsig.ntiles * (64 + 8 + 16) + 6 * (int64_t)P.hsig.len.size() * 4 +
               (int64_t)kWinItem * P.hsig.maxwin * 4);
     return b;
 }
@@ -434,7 +434,8 @@ struct Plan : PlanBase {
     bool have_forward = false;
     // adjoint state (allocated by the first backward)
     struct Rec {
-        int *px = nullptr, *cnt = nullptr, *pre = nullptr, *n = nullptr, *fixf = nullptr;
+        uint64_t* mask = nullptr;   // [ntiles][2][4] knot bitmasks of the pass input
+        int *left = nullptr, *cnt = nullptr, *pre = nullptr, *n = nullptr, *fixf = nullptr;
         int* winlist = nullptr;
         unsigned* wincount = nullptr;
     };
@@ -584,8 +585,9 @@ struct Plan : PlanBase {
             if (record) {
                 // structure of pass j's input for the adjoint
                 const Rec& R = record[j];
-                sa.px = R.px;
                 sa.fixf = R.fixf;
+                ICM_TRY(launch_copy<float>((float*)R.mask, (const float*)kmask[b], 16 * (int64_t)d.ntiles, st));
+                ICM_TRY(launch_rec_left(knbx[b], R.left, d.ntiles, st));
                 ICM_TRY(launch_copy<float>((float*)R.cnt, (const float*)kcnt[b], 2 * (int64_t)d.ntiles, st));
                 ICM_TRY(launch_copy<float>((float*)R.pre, (const float*)kpre[b], 2 * (int64_t)d.ntiles, st));
                 ICM_TRY(launch_copy<float>((float*)R.n, (const float*)nk[b], 2 * (int64_t)d.nseries, st));
@@ -1050,10 +1052,9 @@ struct Plan : PlanBase {
         const int S = cfg.max_sifts;
         int rc;
         std::vector<Rec> rr(S);
-        int* px_all = nullptr;
-        if ((rc = alloc(&px_all, (int64_t)S * 2 * kstride))) return rc;
         for (int j = 0; j < S; ++j) {
-            rr[j].px = px_all + (int64_t)j * 2 * kstride;
+            if ((rc = alloc(&rr[j].mask, 8 * (int64_t)dsig.ntiles))) return rc;
+            if ((rc = alloc(&rr[j].left, 2 * (int64_t)dsig.ntiles))) return rc;
             if ((rc = alloc(&rr[j].cnt, 2 * (int64_t)dsig.ntiles))) return rc;
             if ((rc = alloc(&rr[j].pre, 2 * (int64_t)dsig.ntiles))) return rc;
             if ((rc = alloc(&rr[j].n, 2 * (int64_t)dsig.nseries))) return rc;
@@ -1132,7 +1133,8 @@ struct Plan : PlanBase {
             auto rec_of = [&](int j) {
                 PassRec r{};
                 if (j >= 0 && j < S) {
-                    r.px = R[j].px;
+                    r.mask = R[j].mask;
+                    r.left = R[j].left;
                     r.cnt = R[j].cnt;
                     r.pre = R[j].pre;
                     r.n = R[j].n;
