@@ -151,9 +151,9 @@ struct AdjTileArgs {
     real* mu;
     const real* bcast;          // non-null: mu = bcast[chan_off[chan[s]] + p] first
     const int64_t* chan_off;
-    PassRec sc;                 // scatter pass (sc.px == nullptr: none)
+    PassRec sc;                 // scatter pass (sc.mask == nullptr: none)
     const real* ga;             // knot gradients of pass sc
-    PassRec mo;                 // moments pass (mo.px == nullptr: none)
+    PassRec mo;                 // moments pass (mo.mask == nullptr: none)
     real* mom;
     real* edge;
 };

@@ -1,14 +1,19 @@
 // HIP kernels of the ICEEMDAN hot path (gfx950, wave64).
 //
-// One sift pass of every series is two launches:
-//   k_solve  one wavefront per window of 64*B knots: natural-spline system
-//            rows, per-lane elimination of B rows, parallel cyclic reduction
-//            across the 64 lanes, back substitution -> c of the kept rows
-//   k_sift   one workgroup per tile of T samples: interval of every sample
-//            from the tile's extrema flags, interval coefficients in LDS,
-//            upper/lower envelope, mean, subtract; then the extrema of the new
-//            samples are compacted into the next pass's knot arrays with a
-//            single-pass decoupled look-back over the tiles of the series.
+// A knot set is stored as bitmasks: 64-bit words, one per 64 samples of a
+// 256-sample tile and knot type, written by the kernel that wrote the
+// samples; a knot's position is its bit, its value the sample there.  One
+// sift pass of every series is three launches:
+//   k_solve         one wavefront per window of 64*B knots: positions from
+//                   the bitmasks, values from the samples, natural-spline
+//                   rows, per-lane elimination of B rows, parallel cyclic
+//                   reduction across the 64 lanes, back substitution -> c
+//   k_sift_w        one wavefront per segment of tiles: interval of every
+//                   sample from the tile's bitmasks, interval coefficients in
+//                   LDS, upper/lower envelope, mean, subtract; then the
+//                   extrema of the new samples as the next pass's bitmasks
+//   k_knot_prefix_* per series: tile knot prefixes, neighbour knots of every
+//                   tile, the solve's window list
 // Reference counterparts: find_extrema_shfl_* :92-148, CUB ExclusiveSum
 // :859-883, select_extrema_* :150-207, setBoundary* :209-291,
 // tridiagonal_setup :310-343, gtsv2_nopivot :899-921, spline_coefficients
@@ -1208,7 +1213,7 @@ __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs
 }
 
 // Stage start for 256-sample tiles, one wavefront per tile: forms x_i
-// (the same arithmetic as init_value) and writes its knots to the tile slots.
+// (the same arithmetic as init_value) and stores its knots as bitmasks.
 template <typename real>
 __global__ __launch_bounds__(kThreads) void k_init_w(SeriesDev sd, InitArgs<real> ia, real* h_out,
                                                      KnotsDev<real> kout)

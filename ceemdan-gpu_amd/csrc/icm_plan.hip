@@ -185,7 +185,7 @@ struct PlanBase {
     HostSeries hn, hsig;
     DevSeries dn, dsig;
     int64_t kstride = 0, maxwin = 0, ntiles_max = 0, nser_max = 0, samples_max = 0;
-    int cap = 0;           // knot slots per tile and type
+    int cap = 0;           // most knots of one type in a tile (T/2 + 2)
     int64_t chan_total = 0;
     std::vector<void*> allocs;
     int64_t bytes = 0;
