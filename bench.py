@@ -244,6 +244,20 @@ def main():
                             "backward_vs_plain_forward": bwd_ms / ms_per_step,
                             "loss": "sum_k w_k |IMF_k|^2, w_k ~ U(0,1) seed 5",
                             "grad_finite": all(bool(torch.isfinite(g).all()) for g in gx)}
+    if W["dtype"] == "f32":
+        # per-mode error of the FP32 decomposition against FP64 (channel 0,
+        # same settings; the FP64 GPU path equals the FP64 oracle bit for bit)
+        n0 = lengths[0]
+        p64 = icm.Plan([n0], I, M, dtype=torch.float64, max_sifts=S, noise_strength=0.2, seed=1234,
+                       use_graph=False, device=dev)
+        ref = p64.forward([torch.tensor(xs[0], dtype=torch.float64, device=dev)])[0]
+        p64.close()
+        got = out[0].to(torch.float64)
+        rel = [float((got[k] - ref[k]).abs().max() / ref[k].abs().max().clamp_min(1e-300))
+               for k in range(M)]
+        rms = [float((got[k] - ref[k]).norm() / ref[k].norm().clamp_min(1e-300)) for k in range(M)]
+        line["fp32_vs_fp64"] = {"channel": 0, "samples": n0, "per_mode_max_rel": rel,
+                                "per_mode_rms_rel": rms}
     if not args.no_batched:
         line["batched_vs_per_signal"] = batched_vs_per_signal(icm, sig, torch, dev)
     if not args.no_cpu:
