@@ -122,6 +122,7 @@ struct SolveArgs {
 struct PassRec {
     const uint64_t* mask;   // [ntiles][2][4] knot bitmasks
     const int* left;        // [ntiles][2] position of the last knot before the tile (-1: none)
+    const int* right;       // [ntiles][2] position of the first knot after the tile (-1: none)
     const int* cnt;         // [ntiles][2]
     const int* pre;         // [ntiles][2]
     const int* n;           // [nseries][2]
@@ -215,7 +216,7 @@ hipError_t launch_copy(real* dst, const real* src, int64_t n, hipStream_t st);
 template <typename real>
 hipError_t launch_select_copy(const SeriesDev& sd, int max_len, const real* hA, const real* hB,
                               const int* parity, real* dst, hipStream_t st);
-hipError_t launch_rec_left(const int* nbx, int* left, int ntiles, hipStream_t st);
+hipError_t launch_rec_left(const int* nbx, int* left, int* right, int ntiles, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st);
 hipError_t launch_check_index(const int* d_x, int64_t N, int* err, hipStream_t st);
 

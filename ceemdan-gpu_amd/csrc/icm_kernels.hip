@@ -1507,14 +1507,33 @@ __global__ void k_adj_span(AdjArgs<real> a)
         if (g >= (ty ? n1 : n0) - 1) continue;
         const real* e = a.edge + 4 * (4 * (int64_t)gt + 2 * ty + 1);
         real m0 = e[0], m1 = e[1], m2 = e[2], m3 = e[3];
-        const int last = sd.tile0[s] + sd.ntl[s] - 1;
-        for (int tau = gt + 1; tau <= last; ++tau) {
+        // up to the tile holding the next knot (known from the record, so
+        // the loads do not wait on each other)
+        const int last = sd.tile0[s] + a.ps.right[2 * gt + ty] / kWT;
+        int tau = gt + 1;
+        // eight tiles at a time: all loads first, then the adds in order
+        for (; tau + 7 <= last; tau += 8) {
+            real f[8][4];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const real* fp = a.edge + 4 * (4 * (int64_t)(tau + u) + 2 * ty);
+#pragma unroll
+                for (int m = 0; m < 4; ++m) f[u][m] = fp[m];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                m0 += f[u][0];
+                m1 += f[u][1];
+                m2 += f[u][2];
+                m3 += f[u][3];
+            }
+        }
+        for (; tau <= last; ++tau) {
             const real* f = a.edge + 4 * (4 * (int64_t)tau + 2 * ty);
             m0 += f[0];
             m1 += f[1];
             m2 += f[2];
             m3 += f[3];
-            if (a.ps.cnt[2 * tau + ty] > 0) break;
         }
         real* dst = a.mom + 4 * (sd.kbase[s] + (ty ? sd.kstride : 0) + g);
         dst[0] = m0;
@@ -1908,12 +1927,16 @@ __global__ void k_select_copy(SeriesDev sd, const real* hA, const real* hB, cons
     }
 }
 
-// left[2 t + ty] = position of the last knot before tile t (from the
-// prefix's neighbour knots), for the adjoint's pass records
-__global__ void k_rec_left(const int* nbx, int* left, int ntiles)
+// left/right[2 t + ty] = positions of the last knot before tile t and of the
+// first knot after it (from the prefix's neighbour knots), for the
+// adjoint's pass records
+__global__ void k_rec_left(const int* nbx, int* left, int* right, int ntiles)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < 2 * ntiles) left[i] = nbx[6 * (int64_t)(i >> 1) + 3 * (i & 1)];
+    if (i < 2 * ntiles) {
+        left[i] = nbx[6 * (int64_t)(i >> 1) + 3 * (i & 1)];
+        right[i] = nbx[6 * (int64_t)(i >> 1) + 3 * (i & 1) + 2];
+    }
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n)
@@ -2118,10 +2141,10 @@ hipError_t launch_adj_input(int nchan, const int* chan_len, const int64_t* chan_
     return hipGetLastError();
 }
 
-hipError_t launch_rec_left(const int* nbx, int* left, int ntiles, hipStream_t st)
+hipError_t launch_rec_left(const int* nbx, int* left, int* right, int ntiles, hipStream_t st)
 {
     hipLaunchKernelGGL(k_rec_left, dim3(blocks_for(2 * (int64_t)ntiles, 256)), dim3(256), 0, st, nbx,
-                       left, ntiles);
+                       left, right, ntiles);
     return hipGetLastError();
 }
 

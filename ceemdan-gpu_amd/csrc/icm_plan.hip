@@ -408,7 +408,7 @@ int64_t plan_bytes_estimate(const icm_config* c)
     b += (int64_t)(M - 1) * P.chan_total * s;     // residue history
     if (c->keep_records && c->stop_mode == ICM_STOP_FIXED)   // per-pass knot records
         b += (int64_t)(M - 1) * c->max_sifts *
-             ((int64_t)P.hsig.ntiles * (64 + 8 + 16) + 6 * (int64_t)P.hsig.len.size() * 4 +
+             ((int64_t)P.hsig.ntiles * (64 + 16 + 16) + 6 * (int64_t)P.hsig.len.size() * 4 +
               (int64_t)kWinItem * P.hsig.maxwin * 4);
     return b;
 }
@@ -435,7 +435,8 @@ struct Plan : PlanBase {
     // adjoint state (allocated by the first backward)
     struct Rec {
         uint64_t* mask = nullptr;   // [ntiles][2][4] knot bitmasks of the pass input
-        int *left = nullptr, *cnt = nullptr, *pre = nullptr, *n = nullptr, *fixf = nullptr;
+        int *left = nullptr, *right = nullptr, *cnt = nullptr, *pre = nullptr, *n = nullptr,
+            *fixf = nullptr;
         int* winlist = nullptr;
         unsigned* wincount = nullptr;
     };
@@ -587,7 +588,7 @@ struct Plan : PlanBase {
                 const Rec& R = record[j];
                 sa.fixf = R.fixf;
                 ICM_TRY(launch_copy<float>((float*)R.mask, (const float*)kmask[b], 16 * (int64_t)d.ntiles, st));
-                ICM_TRY(launch_rec_left(knbx[b], R.left, d.ntiles, st));
+                ICM_TRY(launch_rec_left(knbx[b], R.left, R.right, d.ntiles, st));
                 ICM_TRY(launch_copy<float>((float*)R.cnt, (const float*)kcnt[b], 2 * (int64_t)d.ntiles, st));
                 ICM_TRY(launch_copy<float>((float*)R.pre, (const float*)kpre[b], 2 * (int64_t)d.ntiles, st));
                 ICM_TRY(launch_copy<float>((float*)R.n, (const float*)nk[b], 2 * (int64_t)d.nseries, st));
@@ -1055,6 +1056,7 @@ struct Plan : PlanBase {
         for (int j = 0; j < S; ++j) {
             if ((rc = alloc(&rr[j].mask, 8 * (int64_t)dsig.ntiles))) return rc;
             if ((rc = alloc(&rr[j].left, 2 * (int64_t)dsig.ntiles))) return rc;
+            if ((rc = alloc(&rr[j].right, 2 * (int64_t)dsig.ntiles))) return rc;
             if ((rc = alloc(&rr[j].cnt, 2 * (int64_t)dsig.ntiles))) return rc;
             if ((rc = alloc(&rr[j].pre, 2 * (int64_t)dsig.ntiles))) return rc;
             if ((rc = alloc(&rr[j].n, 2 * (int64_t)dsig.nseries))) return rc;
@@ -1135,6 +1137,7 @@ struct Plan : PlanBase {
                 if (j >= 0 && j < S) {
                     r.mask = R[j].mask;
                     r.left = R[j].left;
+                    r.right = R[j].right;
                     r.cnt = R[j].cnt;
                     r.pre = R[j].pre;
                     r.n = R[j].n;
