@@ -101,6 +101,12 @@ struct SiftArgs {
     const int* active;      // Rilling: per-series active flag (nullptr = all)
     int* sx_cnt;            // Rilling: per-tile count of sx > 0.05 (nullptr = off)
     int* sx_mark;           // Rilling: per-tile any(sx > 0.5)
+    // sparse pass (non-null): sift with small knot tables, list the tiles
+    // that do not fit in ovf_segs[*ovf_count] (zero before the launch) and
+    // sift those in a second launch with full tables
+    Segment* ovf_segs;      // [ntiles]
+    unsigned* ovf_count;
+    const unsigned* nsegs_dev;  // set by the launcher for the second launch
 };
 
 template <typename real>

@@ -36,8 +36,8 @@ template <typename real> struct Promote { using type = double; };
 
 constexpr int kWT = 256;               // samples per tile
 constexpr int kWR = kWT / kWave;       // 64-sample chunks per tile
-constexpr int kWKC = kWT / 2 + 8;      // table entries per type (>= cnt + 3)
-constexpr int kWNC = (kWKC + kWave - 1) / kWave;
+constexpr int kWKC = kWT / 2 + 8;      // table entries per type (>= cnt + 3 for any tile)
+constexpr int kWKS = kWave;             // small table (sparse passes): tiles with cnt + 3 <= 64
 
 // ------------------------------------------------------------- knot bitmasks
 // A tile's knots of one type are the set bits of four 64-bit row masks
@@ -927,32 +927,31 @@ __device__ __forceinline__ void store_masks(uint64_t* dst, const uint64_t (&m)[2
     }
 }
 
-template <typename real>
+template <typename real, int KC>
 struct WTile {
+    static constexpr int NC = (KC + kWave - 1) / kWave;
     real h[kWR];
     real hl, hr;          // h[t0 - 1], h[t0 + kWT] (0 outside the series)
-    real c[2][kWNC];      // c of knots pre-2+j, j = lane + 64 k
+    real c[2][NC];        // c of knots pre-2+j, j = lane + 64 k
     int nbx;              // lanes 0..5: positions of knots pre-1, pre-2, pre+cnt per type
     real nby;
     uint64_t mk;          // lanes 0..7: knot bitmasks [type][row] of h (other lanes: copies)
 };
 
 
-template <typename real, bool RILL>
-__global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs)
+// One segment of tiles, sifted by one wavefront with a knot table of KC
+// entries per type.  A tile whose knots do not fit (cnt + 3 > KC) is an error
+// with the full table (KC = kWKC fits every tile); with the small table it is
+// appended to a.ovf_segs for a second launch with the full table, and the
+// tile after it takes h_new of its left neighbour from its own knots.
+template <typename real, bool RILL, int KC>
+__device__ __forceinline__ void sift_segment(const SiftArgs<real>& a, int sid, int lane,
+                                             int (*tp)[KC], Coef<real> (*tc)[KC])
 {
-    constexpr int NW = kThreads / kWave;
-    __shared__ int s_tp[NW][2][kWKC];
-    __shared__ Coef<real> s_tc[NW][2][kWKC];
-    const int lane = threadIdx.x & 63, wv = wave_index();
-    const int sid = blockIdx.x * NW + wv;
-    if (sid >= nsegs) return;
     const SeriesDev& sd = a.sd;
     const Segment seg = a.segs[sid];
     const int s = seg.s;
     if (a.active && !a.active[s]) return;
-    int(*tp)[kWKC] = s_tp[wv];
-    Coef<real>(*tc)[kWKC] = s_tc[wv];
     const int N = sd.len[s];
     const int tb = sd.tile0[s];
     const int64_t kb0 = sd.kbase[s], kb1 = sd.kstride + sd.kbase[s];
@@ -963,7 +962,7 @@ __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs
                            {a.kin.fix[4 * s + 2], a.kin.fix[4 * s + 3]}};
     const bool use_mean = (nn[0] > 3) && (nn[1] > 3);
 
-    auto load = [&](int t, const int* pre, const int* cnt, WTile<real>& w) {
+    auto load = [&](int t, const int* pre, const int* cnt, WTile<real, KC>& w) {
         const int t0 = t * kWT;
 #pragma unroll
         for (int r = 0; r < kWR; ++r) {
@@ -976,7 +975,7 @@ __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs
         for (int ty = 0; ty < 2; ++ty) {
             const real* cp = a.c + (ty ? kb1 : kb0);
 #pragma unroll
-            for (int k = 0; k < kWNC; ++k) {
+            for (int k = 0; k < WTile<real, KC>::NC; ++k) {
                 const int j = lane + k * kWave;
                 const int g = pre[ty] - 2 + j;
                 w.c[ty][k] = (j < cnt[ty] + 3 && g >= 0 && g < nn[ty]) ? cp[g] : (real)0;
@@ -1007,7 +1006,7 @@ __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs
         ncnt[0] = load_scalar(a.kin.cnt + 2 * gt);
         ncnt[1] = load_scalar(a.kin.cnt + 2 * gt + 1);
     }
-    WTile<real> cur, nxt;
+    WTile<real, KC> cur, nxt;
     load(seg.t_first, mpre, mcnt, cur);
     // the first tile is needed at once; waiting here also tells the compiler
     // that nothing loaded before the loop is outstanding inside it, so the
@@ -1016,6 +1015,7 @@ __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs
     // h_new of the sample just before the tile: the previous tile of the
     // segment computed it (same interval, same coefficients, same bits)
     real hcarry = (real)0;
+    bool carry = false;   // hcarry is valid (the previous tile was sifted here)
 
     for (int i = 0; i < seg.count; ++i) {
         const int t = seg.t_first + i;
@@ -1035,29 +1035,30 @@ __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs
             }
         }
 
+        // ---- knot flags of h (find_extrema_shfl_*, cudaICEEMDAN.cu:92-148):
+        // the previous pass (or the stage start) found them when it wrote h,
+        // from the same neighbour values, and stored them as bitmasks
+        uint64_t bm[2][kWR];
+        int cb[2][kWR];
+        int tot[2] = {0, 0};
+#pragma unroll
+        for (int ty = 0; ty < 2; ++ty)
+#pragma unroll
+            for (int r = 0; r < kWR; ++r) {
+                bm[ty][r] = lane_value(cur.mk, ty * kWR + r);
+                cb[ty][r] = tot[ty];
+                tot[ty] += __popcll(bm[ty][r]);
+            }
+        const bool over = cnt[0] + 3 > KC || cnt[1] + 3 > KC;
+        if (tot[0] != cnt[0] || tot[1] != cnt[1] || (over && !a.ovf_segs)) {
+            if (lane == 0) atomicOr(a.pc.err, ICM_DEVERR_KNOTCAP);
+            return;
+        }
+
         // interior tiles (neither series end inside, all 256 samples valid)
         // take a copy of the body without the per-lane end tests
-        auto tile = [&](auto full_c) -> bool {
+        auto tile = [&](auto full_c) {
             constexpr bool FULL = decltype(full_c)::value;
-            // ---- knot flags of h (find_extrema_shfl_*, cudaICEEMDAN.cu:92-148):
-            // the previous pass (or the stage start) found them when it wrote
-            // h, from the same neighbour values, and stored them as bitmasks
-            uint64_t bm[2][kWR];
-            int cb[2][kWR];
-            int tot[2] = {0, 0};
-#pragma unroll
-            for (int ty = 0; ty < 2; ++ty)
-#pragma unroll
-                for (int r = 0; r < kWR; ++r) {
-                    bm[ty][r] = lane_value(cur.mk, ty * kWR + r);
-                    cb[ty][r] = tot[ty];
-                    tot[ty] += __popcll(bm[ty][r]);
-                }
-            if (tot[0] != cnt[0] || tot[1] != cnt[1] || cnt[0] + 3 > kWKC || cnt[1] + 3 > kWKC) {
-                if (lane == 0) atomicOr(a.pc.err, ICM_DEVERR_KNOTCAP);
-                return false;
-            }
-
             // ---- knot table: index j <-> knot pre - 2 + j
 #pragma unroll
             for (int r = 0; r < kWR; ++r) {
@@ -1092,7 +1093,7 @@ __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs
 #pragma unroll
             for (int ty = 0; ty < 2; ++ty)
 #pragma unroll
-                for (int k = 0; k < kWNC; ++k) {
+                for (int k = 0; k < WTile<real, KC>::NC; ++k) {
                     const int j = lane + k * kWave;
                     if (j < cnt[ty] + 3) tc[ty][j].c = cur.c[ty][k];
                 }
@@ -1150,7 +1151,7 @@ __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs
             if (a.emit) {
                 // h_new just outside the tile, evaluated by every lane (same LDS words)
                 real hnl = hcarry, hnr = (real)0;
-                if (i == 0 && (FULL || t0 >= 1)) {
+                if (!carry && (FULL || t0 >= 1)) {
                     const int p = t0 - 1;
                     const int ju = tp[0][1] == p ? 0 : 1;
                     const int jl = tp[1][1] == p ? 0 : 1;
@@ -1193,11 +1194,25 @@ __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs
                 }
             }
             hcarry = lane_value(hn[kWR - 1], kWave - 1);
-            return true;
+            carry = true;
         };
-        const bool interior = t0 > 0 && t0 + kWT < N;
-        const bool ok = interior ? tile(std::true_type{}) : tile(std::false_type{});
-        if (!ok) return;
+        if (over) {
+            // left to the launch with the full table
+            if (lane == 0) {
+                const unsigned q = atomicAdd(a.ovf_count, 1u);
+                Segment o;
+                o.s = s;
+                o.t_first = t;
+                o.count = 1;
+                o.pad = 0;
+                a.ovf_segs[q] = o;
+            }
+            carry = false;
+        } else if (t0 > 0 && t0 + kWT < N) {
+            tile(std::true_type{});
+        } else {
+            tile(std::false_type{});
+        }
         // the table is rewritten by the next tile
         wave_sync();
         cur = nxt;
@@ -1210,6 +1225,34 @@ __global__ __launch_bounds__(kThreads) void k_sift_w(SiftArgs<real> a, int nsegs
         ncnt[0] = n2cnt[0];
         ncnt[1] = n2cnt[1];
     }
+}
+
+// One wavefront per segment.  The small-table instance is held to 96
+// registers, so that five of its waves fit on a SIMD (its tables allow eight).
+template <typename real, bool RILL, int KC>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(KC == kWKS ? 5 : 1)))
+void k_sift_w(SiftArgs<real> a, int nsegs)
+{
+    constexpr int NW = kThreads / kWave;
+    __shared__ int s_tp[NW][2][KC];
+    __shared__ Coef<real> s_tc[NW][2][KC];
+    const int lane = threadIdx.x & 63, wv = wave_index();
+    const int sid = blockIdx.x * NW + wv;
+    if (sid < nsegs) sift_segment<real, RILL, KC>(a, sid, lane, s_tp[wv], s_tc[wv]);
+}
+
+// The tiles a small-table pass listed (a.nsegs_dev of them, one-tile
+// segments), with full tables: a grid-stride loop over the list.
+template <typename real, bool RILL>
+__global__ __launch_bounds__(kThreads) void k_sift_list(SiftArgs<real> a)
+{
+    constexpr int NW = kThreads / kWave;
+    __shared__ int s_tp[NW][2][kWKC];
+    __shared__ Coef<real> s_tc[NW][2][kWKC];
+    const int lane = threadIdx.x & 63, wv = wave_index();
+    const int ns = (int)load_scalar(a.nsegs_dev);
+    for (int sid = blockIdx.x * NW + wv; sid < ns; sid += gridDim.x * NW)
+        sift_segment<real, RILL, kWKC>(a, sid, lane, s_tp[wv], s_tc[wv]);
 }
 
 // Stage start for 256-sample tiles, one wavefront per tile: forms x_i
@@ -1985,12 +2028,29 @@ hipError_t launch_knot_prefix(const SeriesDev& sd, const KnotsDev<real>& k, cons
 template <typename real>
 hipError_t launch_sift(const SiftArgs<real>& a, int nsegs, hipStream_t st)
 {
-    if (a.sx_cnt)
-        hipLaunchKernelGGL((k_sift_w<real, true>), dim3(blocks_for(nsegs, kThreads / kWave)),
-                           dim3(kThreads), 0, st, a, nsegs);
-    else
-        hipLaunchKernelGGL((k_sift_w<real, false>), dim3(blocks_for(nsegs, kThreads / kWave)),
-                           dim3(kThreads), 0, st, a, nsegs);
+    const dim3 grid(blocks_for(nsegs, kThreads / kWave));
+    if (!a.ovf_segs) {
+        if (a.sx_cnt)
+            hipLaunchKernelGGL((k_sift_w<real, true, kWKC>), grid, dim3(kThreads), 0, st, a, nsegs);
+        else
+            hipLaunchKernelGGL((k_sift_w<real, false, kWKC>), grid, dim3(kThreads), 0, st, a, nsegs);
+        return hipGetLastError();
+    }
+    // sparse pass: small knot tables (more waves per SIMD), then the tiles
+    // they could not hold, listed on the device, with full tables
+    SiftArgs<real> f = a;
+    f.segs = a.ovf_segs;
+    f.nsegs_dev = a.ovf_count;
+    f.ovf_segs = nullptr;
+    f.ovf_count = nullptr;
+    constexpr int kFixGrid = 512;
+    if (a.sx_cnt) {
+        hipLaunchKernelGGL((k_sift_w<real, true, kWKS>), grid, dim3(kThreads), 0, st, a, nsegs);
+        hipLaunchKernelGGL((k_sift_list<real, true>), dim3(kFixGrid), dim3(kThreads), 0, st, f);
+    } else {
+        hipLaunchKernelGGL((k_sift_w<real, false, kWKS>), grid, dim3(kThreads), 0, st, a, nsegs);
+        hipLaunchKernelGGL((k_sift_list<real, false>), dim3(kFixGrid), dim3(kThreads), 0, st, f);
+    }
     return hipGetLastError();
 }
 

@@ -404,6 +404,7 @@ int64_t plan_bytes_estimate(const icm_config* c)
     b += 2 * 2 * 2 * P.ntiles_max * 4 + 2 * 2 * P.nser_max * (4 + 2 * s);
     b += 2 * P.ntiles_max * (6 * (4 + s) + 8 * 8);  // neighbour knots, knot bitmasks
     b += (int64_t)kEpochs * 4 + (int64_t)kWinItem * P.maxwin * 4;
+    b += (int64_t)kEpochs * 4 + P.ntiles_max * (int64_t)sizeof(Segment);  // sparse-pass tile lists
     b += 5 * P.chan_total * s + P.chan_total * M * s;
     b += (int64_t)(M - 1) * P.chan_total * s;     // residue history
     if (c->keep_records && c->stop_mode == ICM_STOP_FIXED)   // per-pass knot records
@@ -440,6 +441,9 @@ struct Plan : PlanBase {
         int* winlist = nullptr;
         unsigned* wincount = nullptr;
     };
+    Segment* ovfsegs = nullptr;        // [ntiles] tiles a sparse sift pass left to the full tables
+    unsigned* ovfcount = nullptr;      // [kEpochs] their number per pass
+    int sparse_stage = 1;              // first stage of a phase sifted with small knot tables
     std::vector<Rec> rec;                  // re-sift records of one stage (backward)
     std::vector<std::vector<Rec>> krec;    // per stage, kept by the forward (keep_records)
     std::vector<char> krec_valid;
@@ -467,6 +471,13 @@ struct Plan : PlanBase {
         }
         if ((rc = alloc(&kc, 2 * kstride))) return rc;
         if ((rc = alloc(&wincount, kEpochs))) return rc;
+        if ((rc = alloc(&ovfcount, kEpochs))) return rc;
+        // ICM_SPARSE_STAGE: first stage sifted with small knot tables (a
+        // tuning knob; 0 also sends the dense first stage through them, and
+        // most of its tiles through the full-table launch)
+        if (const char* env = std::getenv("ICM_SPARSE_STAGE"))
+            if (std::atoi(env) >= 0) sparse_stage = std::atoi(env);
+        if ((rc = alloc(&ovfsegs, ntiles_max))) return rc;
         if ((rc = alloc(&winlist, (int64_t)kWinItem * maxwin))) return rc;
         if ((rc = alloc(&err, 1))) return rc;
         if ((rc = alloc(&n_active, 1))) return rc;
@@ -541,10 +552,22 @@ struct Plan : PlanBase {
     int reset_control(hipStream_t st)
     {
         ICM_TRY(launch_fill_u32((uint32_t*)wincount, 0u, kEpochs, st));
+        ICM_TRY(launch_fill_u32((uint32_t*)ovfcount, 0u, kEpochs, st));
         epoch = 0;
         return ICM_OK;
     }
     uint32_t next_epoch() { return ++epoch; }
+
+    // Sift passes after the first stage of a phase meet a few tens of knots
+    // per 256-sample tile: they run with small knot tables (more waves per
+    // SIMD), and the rare tile with more knots with full tables in a second
+    // launch.  The result is the same either way.
+    void sparse_pass(SiftArgs<real>& a, int stage, uint32_t e)
+    {
+        if (stage < sparse_stage) return;
+        a.ovf_segs = ovfsegs;
+        a.ovf_count = ovfcount + e;
+    }
 
     int trace_copy(int phase, int stage, int pass, const DevSeries& d, int b, hipStream_t st)
     {
@@ -615,6 +638,7 @@ struct Plan : PlanBase {
             a.active = nullptr;
             a.sx_cnt = nullptr;
             a.sx_mark = nullptr;
+            sparse_pass(a, stage, e_emit);
             if ((rc = prof_begin(2, pass_bytes, st))) return rc;
             ICM_TRY(launch_sift<real>(a, d.nsegs, st));
             if ((rc = prof_end(st))) return rc;
@@ -672,6 +696,7 @@ struct Plan : PlanBase {
             a.active = active;
             a.sx_cnt = sx_cnt;
             a.sx_mark = sx_mark;
+            sparse_pass(a, stage, e_emit);
             ICM_TRY(launch_sift<real>(a, d.nsegs, st));
             ICM_TRY(launch_fill_u32((uint32_t*)n_active, 0u, 1, st));
             ICM_TRY(launch_rilling_decide<real>(sd, nk[b], sx_cnt, sx_mark, active, stop_pass, subs,

@@ -179,3 +179,27 @@ def test_long_series_bitexact(icm, orc, dev):
     got = icm.iceemdan(torch.tensor(x, device=dev), I, M, max_sifts=S, noise=w).cpu().numpy()
     ref = orc.iceemdan(x, w.cpu().numpy(), M, S_or_cap=S)["imfs"]
     assert np.array_equal(got, ref), relerr(got, ref)
+
+
+@pytest.mark.parametrize("sparse_stage,stop_mode", [(0, 0), (0, 1), (1, 0)])
+def test_small_knot_tables_bitexact(icm, orc, dev, monkeypatch, sparse_stage, stop_mode):
+    """Sift passes with small knot tables: tiles with more knots than they
+    hold go to a second launch with full tables, and the next tile of their
+    segment evaluates h_new left of itself from its own knots.  With small
+    tables from stage 0 on (white-noise first stages: most tiles overflow)
+    and segments of 8 tiles, FP64 modes equal the oracle bit for bit."""
+    import torch
+    monkeypatch.setenv("ICM_SPARSE_STAGE", str(sparse_stage))
+    monkeypatch.setenv("ICM_SEGMENT_TILES", "8")
+    N, I, M = 20000, 6, 5
+    x = _sig().eeg_like(N, seed=3)
+    print(f"case: small knot tables sparse_stage={sparse_stage} stop_mode={stop_mode}", flush=True)
+    w = icm.noise(I, N, seed=21, dtype=torch.float64, device=dev)
+    S = 10 if stop_mode == 0 else 40
+    plan = icm.Plan([N], I, M, dtype=torch.float64, stop_mode=stop_mode, max_sifts=S,
+                    noise_max_sifts=S if stop_mode == 0 else 200, use_graph=False, device=dev)
+    got = plan.forward([torch.tensor(x, device=dev)], noise=w)[0].cpu().numpy()
+    plan.close()
+    ref = orc.iceemdan(x, w.cpu().numpy(), M, stop_mode=stop_mode, S_or_cap=S,
+                       noise_cap=S if stop_mode == 0 else 200)["imfs"]
+    assert np.array_equal(got, ref), relerr(got, ref)
