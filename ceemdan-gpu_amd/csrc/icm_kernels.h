@@ -68,6 +68,7 @@ struct PassCtl {
     int K;                 // rows kept per window of the solve
     int H;                 // halo rows on each side of a window
     int* err;              // device error word
+    unsigned* dense;       // [epochs] tiles whose knots overflow the small sift table
 };
 
 enum InitMode : int {
@@ -106,7 +107,10 @@ struct SiftArgs {
     // sift those in a second launch with full tables
     Segment* ovf_segs;      // [ntiles]
     unsigned* ovf_count;
-    const unsigned* nsegs_dev;  // set by the launcher for the second launch
+    // small tables only if at most dense_max tiles of the pass input overflow
+    // them (*dense, counted by the knot prefix); else full tables throughout
+    const unsigned* dense;
+    unsigned dense_max;
 };
 
 template <typename real>

@@ -160,6 +160,16 @@ __global__ __launch_bounds__(kPT) void k_knot_prefix_b(SeriesDev sd, KnotsDev<re
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int tb = sd.tile0[s], nt = sd.ntl[s];
     const real* hs = h + sd.off[s];
+    {
+        // tiles the small sift table cannot hold
+        unsigned d = 0;
+        for (int t = tid; t < nt; t += kPT) {
+            const int c0 = k.cnt[2 * (tb + t)], c1 = k.cnt[2 * (tb + t) + 1];
+            d += (c0 + 3 > kWKS || c1 + 3 > kWKS) ? 1u : 0u;
+        }
+        for (int o = 32; o >= 1; o >>= 1) d += __shfl_xor(d, o, kWave);
+        if (lane == 0 && d) atomicAdd(&pc.dense[pc.epoch], d);
+    }
     int ntot[2] = {0, 0};
     for (int ty = 0; ty < 2; ++ty) {
         // forward: exclusive count and the last two knots before each tile
@@ -303,6 +313,7 @@ __global__ __launch_bounds__(kPT) void k_knot_prefix_1(SeriesDev sd, KnotsDev<re
     __shared__ int s_pre[2][kPT];
     __shared__ int s_first[2][kPT];
     __shared__ unsigned s_wbase;
+    __shared__ unsigned s_dense[NWV];
     const int s = blockIdx.x;
     if (active && !active[s]) return;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -330,6 +341,11 @@ __global__ __launch_bounds__(kPT) void k_knot_prefix_1(SeriesDev sd, KnotsDev<re
         }
         inc[ty] = c[ty];
         mn[ty] = c[ty] > 0 ? t : INF;
+    }
+    {
+        // tiles the small sift table cannot hold
+        const unsigned d = (unsigned)__popcll(wave_ballot(c[0] + 3 > kWKS || c[1] + 3 > kWKS));
+        if (lane == 0) s_dense[wv] = d;
     }
     for (int o = 1; o < kWave; o <<= 1) {
 #pragma unroll
@@ -419,6 +435,9 @@ __global__ __launch_bounds__(kPT) void k_knot_prefix_1(SeriesDev sd, KnotsDev<re
         k.n[2 * s] = n0;
         k.n[2 * s + 1] = n1;
         s_wbase = atomicAdd(&pc.wincount[pc.epoch], (unsigned)(wmx + wmn));
+        unsigned d = 0;
+        for (int w = 0; w < NWV; ++w) d += s_dense[w];
+        if (d) atomicAdd(&pc.dense[pc.epoch], d);
     }
     __syncthreads();
     const unsigned wbase = s_wbase;
@@ -945,11 +964,15 @@ struct WTile {
 // appended to a.ovf_segs for a second launch with the full table, and the
 // tile after it takes h_new of its left neighbour from its own knots.
 template <typename real, bool RILL, int KC>
-__device__ __forceinline__ void sift_segment(const SiftArgs<real>& a, int sid, int lane,
-                                             int (*tp)[KC], Coef<real> (*tc)[KC])
+__device__ __forceinline__ void sift_segment(const SiftArgs<real>& a, const Segment* segs, int sid,
+                                             int lane, int (*tp)[KC], Coef<real> (*tc)[KC])
 {
     const SeriesDev& sd = a.sd;
-    const Segment seg = a.segs[sid];
+    const Segment seg = segs[sid];
+    // a small-table pass whose input has too many over-full tiles is left
+    // to k_sift_list.  The test also reads the segment record (its pad word
+    // is 0), so that the two loads go out together and share one wait.
+    if (KC == kWKS && load_scalar(a.dense) > a.dense_max + (unsigned)seg.pad) return;
     const int s = seg.s;
     if (a.active && !a.active[s]) return;
     const int N = sd.len[s];
@@ -1238,21 +1261,27 @@ void k_sift_w(SiftArgs<real> a, int nsegs)
     __shared__ Coef<real> s_tc[NW][2][KC];
     const int lane = threadIdx.x & 63, wv = wave_index();
     const int sid = blockIdx.x * NW + wv;
-    if (sid < nsegs) sift_segment<real, RILL, KC>(a, sid, lane, s_tp[wv], s_tc[wv]);
+    if (sid < nsegs) sift_segment<real, RILL, KC>(a, a.segs, sid, lane, s_tp[wv], s_tc[wv]);
 }
 
-// The tiles a small-table pass listed (a.nsegs_dev of them, one-tile
-// segments), with full tables: a grid-stride loop over the list.
+// Second launch of a small-table pass, with full tables, as a grid-stride
+// loop: the tiles the small-table launch listed (*a.ovf_count one-tile
+// segments in a.ovf_segs) or, when the pass input had too many over-full
+// tiles for it to run, every segment of the pass.
 template <typename real, bool RILL>
-__global__ __launch_bounds__(kThreads) void k_sift_list(SiftArgs<real> a)
+__global__ __launch_bounds__(kThreads) void k_sift_list(SiftArgs<real> a, int nsegs)
 {
     constexpr int NW = kThreads / kWave;
     __shared__ int s_tp[NW][2][kWKC];
     __shared__ Coef<real> s_tc[NW][2][kWKC];
     const int lane = threadIdx.x & 63, wv = wave_index();
-    const int ns = (int)load_scalar(a.nsegs_dev);
+    const bool all = load_scalar(a.dense) > a.dense_max;
+    const Segment* segs = all ? a.segs : a.ovf_segs;
+    const int ns = all ? nsegs : (int)load_scalar(a.ovf_count);
+    SiftArgs<real> b = a;
+    b.ovf_segs = nullptr;   // full tables hold every tile
     for (int sid = blockIdx.x * NW + wv; sid < ns; sid += gridDim.x * NW)
-        sift_segment<real, RILL, kWKC>(a, sid, lane, s_tp[wv], s_tc[wv]);
+        sift_segment<real, RILL, kWKC>(b, segs, sid, lane, s_tp[wv], s_tc[wv]);
 }
 
 // Stage start for 256-sample tiles, one wavefront per tile: forms x_i
@@ -2036,20 +2065,19 @@ hipError_t launch_sift(const SiftArgs<real>& a, int nsegs, hipStream_t st)
             hipLaunchKernelGGL((k_sift_w<real, false, kWKC>), grid, dim3(kThreads), 0, st, a, nsegs);
         return hipGetLastError();
     }
-    // sparse pass: small knot tables (more waves per SIMD), then the tiles
-    // they could not hold, listed on the device, with full tables
-    SiftArgs<real> f = a;
-    f.segs = a.ovf_segs;
-    f.nsegs_dev = a.ovf_count;
-    f.ovf_segs = nullptr;
-    f.ovf_count = nullptr;
-    constexpr int kFixGrid = 512;
+    // sparse pass: if few tiles of the input overflow the small knot tables
+    // (the knot prefix counted them), small tables (more waves per SIMD) and
+    // then the listed overflow tiles with full tables; else the small-table
+    // launch returns at once and the second launch sifts every segment.
+    // Its grid is what the full-table kernel keeps resident (four
+    // workgroups per CU), so it costs little when it has nothing to do.
+    const dim3 lgrid(std::min<int>(grid.x, 1024));
     if (a.sx_cnt) {
         hipLaunchKernelGGL((k_sift_w<real, true, kWKS>), grid, dim3(kThreads), 0, st, a, nsegs);
-        hipLaunchKernelGGL((k_sift_list<real, true>), dim3(kFixGrid), dim3(kThreads), 0, st, f);
+        hipLaunchKernelGGL((k_sift_list<real, true>), lgrid, dim3(kThreads), 0, st, a, nsegs);
     } else {
         hipLaunchKernelGGL((k_sift_w<real, false, kWKS>), grid, dim3(kThreads), 0, st, a, nsegs);
-        hipLaunchKernelGGL((k_sift_list<real, false>), dim3(kFixGrid), dim3(kThreads), 0, st, f);
+        hipLaunchKernelGGL((k_sift_list<real, false>), lgrid, dim3(kThreads), 0, st, a, nsegs);
     }
     return hipGetLastError();
 }

@@ -404,7 +404,7 @@ int64_t plan_bytes_estimate(const icm_config* c)
     b += 2 * 2 * 2 * P.ntiles_max * 4 + 2 * 2 * P.nser_max * (4 + 2 * s);
     b += 2 * P.ntiles_max * (6 * (4 + s) + 8 * 8);  // neighbour knots, knot bitmasks
     b += (int64_t)kEpochs * 4 + (int64_t)kWinItem * P.maxwin * 4;
-    b += (int64_t)kEpochs * 4 + P.ntiles_max * (int64_t)sizeof(Segment);  // sparse-pass tile lists
+    b += 2 * (int64_t)kEpochs * 4 + P.ntiles_max * (int64_t)sizeof(Segment);  // sparse-pass tile lists
     b += 5 * P.chan_total * s + P.chan_total * M * s;
     b += (int64_t)(M - 1) * P.chan_total * s;     // residue history
     if (c->keep_records && c->stop_mode == ICM_STOP_FIXED)   // per-pass knot records
@@ -443,7 +443,9 @@ struct Plan : PlanBase {
     };
     Segment* ovfsegs = nullptr;        // [ntiles] tiles a sparse sift pass left to the full tables
     unsigned* ovfcount = nullptr;      // [kEpochs] their number per pass
+    unsigned* densecnt = nullptr;      // [kEpochs] tiles of a pass input that overflow small tables
     int sparse_stage = 1;              // first stage of a phase sifted with small knot tables
+    int64_t sparse_dense_max = -1;     // over-full tiles tolerated by them (-1: tiles / 64)
     std::vector<Rec> rec;                  // re-sift records of one stage (backward)
     std::vector<std::vector<Rec>> krec;    // per stage, kept by the forward (keep_records)
     std::vector<char> krec_valid;
@@ -472,11 +474,16 @@ struct Plan : PlanBase {
         if ((rc = alloc(&kc, 2 * kstride))) return rc;
         if ((rc = alloc(&wincount, kEpochs))) return rc;
         if ((rc = alloc(&ovfcount, kEpochs))) return rc;
+        if ((rc = alloc(&densecnt, kEpochs))) return rc;
         // ICM_SPARSE_STAGE: first stage sifted with small knot tables (a
         // tuning knob; 0 also sends the dense first stage through them, and
         // most of its tiles through the full-table launch)
         if (const char* env = std::getenv("ICM_SPARSE_STAGE"))
             if (std::atoi(env) >= 0) sparse_stage = std::atoi(env);
+        // ICM_SPARSE_DENSE_MAX: over-full tiles a small-table pass tolerates
+        // before it runs with full tables throughout (default tiles / 64)
+        if (const char* env = std::getenv("ICM_SPARSE_DENSE_MAX"))
+            if (std::atoll(env) >= 0) sparse_dense_max = std::atoll(env);
         if ((rc = alloc(&ovfsegs, ntiles_max))) return rc;
         if ((rc = alloc(&winlist, (int64_t)kWinItem * maxwin))) return rc;
         if ((rc = alloc(&err, 1))) return rc;
@@ -526,6 +533,7 @@ struct Plan : PlanBase {
         pc.K = solve_K(cfg.dtype);
         pc.H = solve_H(cfg.dtype);
         pc.err = err;
+        pc.dense = densecnt;
         return pc;
     }
     KnotsDev<real> knots(int b)
@@ -553,6 +561,7 @@ struct Plan : PlanBase {
     {
         ICM_TRY(launch_fill_u32((uint32_t*)wincount, 0u, kEpochs, st));
         ICM_TRY(launch_fill_u32((uint32_t*)ovfcount, 0u, kEpochs, st));
+        ICM_TRY(launch_fill_u32((uint32_t*)densecnt, 0u, kEpochs, st));
         epoch = 0;
         return ICM_OK;
     }
@@ -561,12 +570,18 @@ struct Plan : PlanBase {
     // Sift passes after the first stage of a phase meet a few tens of knots
     // per 256-sample tile: they run with small knot tables (more waves per
     // SIMD), and the rare tile with more knots with full tables in a second
-    // launch.  The result is the same either way.
-    void sparse_pass(SiftArgs<real>& a, int stage, uint32_t e)
+    // launch, unless more than 1/64 of the input's tiles (counted by the knot
+    // prefix of epoch e_in) have more knots: then with full tables.  The
+    // result is the same either way.
+    void sparse_pass(SiftArgs<real>& a, int stage, const DevSeries& d, uint32_t e_in, uint32_t e)
     {
         if (stage < sparse_stage) return;
         a.ovf_segs = ovfsegs;
         a.ovf_count = ovfcount + e;
+        a.dense = densecnt + e_in;
+        a.dense_max = sparse_dense_max >= 0
+                          ? (unsigned)std::min<int64_t>(sparse_dense_max, 0xFFFFFFFFll)
+                          : (unsigned)(d.ntiles / 64);
     }
 
     int trace_copy(int phase, int stage, int pass, const DevSeries& d, int b, hipStream_t st)
@@ -633,12 +648,13 @@ struct Plan : PlanBase {
             a.c = kc;
             a.kout = knots(b ^ 1);
             a.emit = (j < S - 1) ? 1 : 0;
+            const uint32_t e_in = e_emit;
             e_emit = next_epoch();
             a.pc = ctl(e_emit);
             a.active = nullptr;
             a.sx_cnt = nullptr;
             a.sx_mark = nullptr;
-            sparse_pass(a, stage, e_emit);
+            sparse_pass(a, stage, d, e_in, e_emit);
             if ((rc = prof_begin(2, pass_bytes, st))) return rc;
             ICM_TRY(launch_sift<real>(a, d.nsegs, st));
             if ((rc = prof_end(st))) return rc;
@@ -691,12 +707,13 @@ struct Plan : PlanBase {
             a.c = kc;
             a.kout = knots(b ^ 1);
             a.emit = 1;
+            const uint32_t e_in = e_emit;
             e_emit = next_epoch();
             a.pc = ctl(e_emit);
             a.active = active;
             a.sx_cnt = sx_cnt;
             a.sx_mark = sx_mark;
-            sparse_pass(a, stage, e_emit);
+            sparse_pass(a, stage, d, e_in, e_emit);
             ICM_TRY(launch_sift<real>(a, d.nsegs, st));
             ICM_TRY(launch_fill_u32((uint32_t*)n_active, 0u, 1, st));
             ICM_TRY(launch_rilling_decide<real>(sd, nk[b], sx_cnt, sx_mark, active, stop_pass, subs,

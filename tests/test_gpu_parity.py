@@ -181,19 +181,26 @@ def test_long_series_bitexact(icm, orc, dev):
     assert np.array_equal(got, ref), relerr(got, ref)
 
 
-@pytest.mark.parametrize("sparse_stage,stop_mode", [(0, 0), (0, 1), (1, 0)])
-def test_small_knot_tables_bitexact(icm, orc, dev, monkeypatch, sparse_stage, stop_mode):
+@pytest.mark.parametrize("sparse_stage,dense_max,stop_mode",
+                         [(0, 1 << 30, 0), (0, 1 << 30, 1), (0, 0, 0), (1, -1, 0)])
+def test_small_knot_tables_bitexact(icm, orc, dev, monkeypatch, sparse_stage, dense_max,
+                                    stop_mode):
     """Sift passes with small knot tables: tiles with more knots than they
     hold go to a second launch with full tables, and the next tile of their
-    segment evaluates h_new left of itself from its own knots.  With small
-    tables from stage 0 on (white-noise first stages: most tiles overflow)
-    and segments of 8 tiles, FP64 modes equal the oracle bit for bit."""
+    segment evaluates h_new left of itself from its own knots; a pass whose
+    input has more such tiles than tolerated runs with full tables in the
+    second launch.  Small tables from stage 0 on (white-noise first stages:
+    most tiles overflow) with every overflow tolerated, with none tolerated,
+    and the default; segments of 8 tiles.  FP64 modes equal the oracle bit
+    for bit."""
     import torch
     monkeypatch.setenv("ICM_SPARSE_STAGE", str(sparse_stage))
+    monkeypatch.setenv("ICM_SPARSE_DENSE_MAX", str(dense_max))
     monkeypatch.setenv("ICM_SEGMENT_TILES", "8")
     N, I, M = 20000, 6, 5
     x = _sig().eeg_like(N, seed=3)
-    print(f"case: small knot tables sparse_stage={sparse_stage} stop_mode={stop_mode}", flush=True)
+    print(f"case: small knot tables sparse_stage={sparse_stage} dense_max={dense_max} "
+          f"stop_mode={stop_mode}", flush=True)
     w = icm.noise(I, N, seed=21, dtype=torch.float64, device=dev)
     S = 10 if stop_mode == 0 else 40
     plan = icm.Plan([N], I, M, dtype=torch.float64, stop_mode=stop_mode, max_sifts=S,
