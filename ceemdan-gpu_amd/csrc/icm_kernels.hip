@@ -559,7 +559,7 @@ __device__ void gather_positions(const SeriesDev& sd, const uint64_t* kmask, con
                                  int* sp, int* tpre, uint64_t (*tmask)[kWR], int hint)
 {
     const int lane = threadIdx.x & 63;
-    const int tb = sd.tile0[s], ntl = sd.ntl[s];
+    const int tb = load_scalar(sd.tile0 + s), ntl = load_scalar(sd.ntl + s);
     int a = hint >= 0 ? hint : 0, b = hint >= 0 ? hint + 1 : ntl;
     while (b - a > 1) {
         const int step = (b - a + kWave - 1) / kWave;
@@ -747,16 +747,18 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     const int wv = wave_index(), lane = threadIdx.x & 63;
     int* sp = s_pos[wv];
     real* sy = s_y[wv];
-    const unsigned total = *a.wincount;
+    // window items and series metadata through the scalar cache (written
+    // by earlier launches): one round trip for the item, one for its series
+    const unsigned total = load_scalar(a.wincount);
     for (unsigned item = blockIdx.x * NW + wv; item < total; item += gridDim.x * NW) {
-        const int s = a.winlist[kWinItem * item];
-        const int code = a.winlist[kWinItem * item + 1];
-        const int hint = a.winlist[kWinItem * item + 2];
+        const int s = load_scalar(a.winlist + kWinItem * item);
+        const int code = load_scalar(a.winlist + kWinItem * item + 1);
+        const int hint = load_scalar(a.winlist + kWinItem * item + 2);
         const int type = code & 1;
         const int w = code >> 1;
-        if (a.active && !a.active[s]) continue;
-        const int n = a.k.n[2 * s + type];
-        const int64_t kb = a.sd.kbase[s] + (type ? a.sd.kstride : 0);
+        if (a.active && !load_scalar(a.active + s)) continue;
+        const int n = load_scalar(a.k.n + 2 * s + type);
+        const int64_t kb = load_scalar(a.sd.kbase + s) + (type ? a.sd.kstride : 0);
         const long g0 = (long)w * K - H;
         const long lo = g0 - 1;
         const int glo = (int)max(lo, 0L);
@@ -765,7 +767,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
         gather_positions<NIT>(a.sd, a.k.mask, a.k.pre, a.k.cnt, s, type, lo, glo, ghi, sp,
                               s_tpre[wv], s_tmask[wv], hint);
         wave_sync();
-        gather_values<real, NIT>(a.h + a.sd.off[s], a.sd.len[s], lo, glo, ghi, sp, sy);
+        gather_values<real, NIT>(a.h + load_scalar(a.sd.off + s), load_scalar(a.sd.len + s), lo, glo,
+                                 ghi, sp, sy);
         wave_sync();
         // endpoint fix-up when n >= 6 (setBoundaryMax/Min, cudaICEEMDAN.cu:209-291);
         // idempotent, so every window that holds an end applies it
@@ -1634,17 +1637,18 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) v
     real* sf = s_f[wv];
     int* sp = s_pos[wv];
     const real third = (real)1 / (real)3;
-    const unsigned total = *a.wincount;
+    // window items and series metadata through the scalar cache (read-only here)
+    const unsigned total = load_scalar(a.wincount);
     for (unsigned item = blockIdx.x * NW + wv; item < total; item += gridDim.x * NW) {
-        const int s = a.winlist[kWinItem * item];
-        const int code = a.winlist[kWinItem * item + 1];
-        const int hint = a.winlist[kWinItem * item + 2];
+        const int s = load_scalar(a.winlist + kWinItem * item);
+        const int code = load_scalar(a.winlist + kWinItem * item + 1);
+        const int hint = load_scalar(a.winlist + kWinItem * item + 2);
         const int type = code & 1;
         const int w = code >> 1;
-        const int n0 = a.ps.n[2 * s], n1 = a.ps.n[2 * s + 1];
+        const int n0 = load_scalar(a.ps.n + 2 * s), n1 = load_scalar(a.ps.n + 2 * s + 1);
         if (!(n0 > 3 && n1 > 3)) continue;
         const int n = type ? n1 : n0;
-        const int64_t kb = a.sd.kbase[s] + (type ? a.sd.kstride : 0);
+        const int64_t kb = load_scalar(a.sd.kbase + s) + (type ? a.sd.kstride : 0);
         const long g0 = (long)w * K - H;
         // knot positions of the window (and one beyond each side) from the
         // pass's bitmasks: sp index = g - lo
