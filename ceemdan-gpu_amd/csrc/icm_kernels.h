@@ -73,6 +73,8 @@ struct PassCtl {
 
 enum InitMode : int {
     INIT_COPY = 0,     // h0 = src[off + p]                       (noise residual)
+    INIT_RESID = 3,    // h0 = src[off + p] - E[off + p], also stored in resid[off + p]
+                       //  (the next noise residual, formed where it is first read)
     INIT_STAGE0 = 1,   // h0 = xt[c][p] + (eps0 * E[p]) * inv[s]  (cudaICEEMDAN.cu:68-78)
     INIT_STAGEK = 2,   // h0 = r[c][p] + (eps0 * E[p]) * sq[c]    (cudaICEEMDAN.cu:80-90)
 };
@@ -80,7 +82,8 @@ enum InitMode : int {
 template <typename real>
 struct InitArgs {
     int mode;
-    const real* src;        // INIT_COPY: samples buffer (series offsets)
+    const real* src;        // INIT_COPY / INIT_RESID: samples buffer (series offsets)
+    real* resid;            // INIT_RESID: h0 also stored here (another buffer than src)
     const real* chan_sig;   // INIT_STAGE0/K: per-channel signal (xt or r)
     const int64_t* chan_off;  // offset of channel c in chan_sig
     const real* E;          // noise modes of this stage (eoff offsets)

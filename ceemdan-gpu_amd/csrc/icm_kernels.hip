@@ -467,6 +467,7 @@ __device__ __forceinline__ real init_value(const InitArgs<real>& ia, const Serie
 {
     using P = typename Promote<real>::type;
     if (ia.mode == INIT_COPY) return ia.src[sd.off[s] + p];
+    if (ia.mode == INIT_RESID) return ia.src[sd.off[s] + p] - ia.E[sd.off[s] + p];
     const int c = sd.chan[s];
     const real base = ia.chan_sig[ia.chan_off[c] + p];
     const real e = ia.E[sd.eoff[s] + p];
@@ -1309,6 +1310,9 @@ __global__ __launch_bounds__(kThreads) void k_init_w(SeriesDev sd, InitArgs<real
     real sclk = (real)0;
     if (ia.mode == INIT_COPY) {
         src = ia.src + sd.off[s];
+    } else if (ia.mode == INIT_RESID) {
+        src = ia.src + sd.off[s];
+        E = ia.E + sd.off[s];
     } else {
         const int c = sd.chan[s];
         base = ia.chan_sig + ia.chan_off[c];
@@ -1318,10 +1322,12 @@ __global__ __launch_bounds__(kThreads) void k_init_w(SeriesDev sd, InitArgs<real
     }
     auto val = [&](int p) -> real {
         if (ia.mode == INIT_COPY) return src[p];
+        if (ia.mode == INIT_RESID) return src[p] - E[p];
         const real e = E[p];
         if (ia.mode == INIT_STAGE0) return (real)((P)base[p] + (P)(ia.eps0 * e) * scl0);
         return base[p] + (ia.eps0 * e) * sclk;
     };
+    real* resid = ia.mode == INIT_RESID ? ia.resid + sd.off[s] : nullptr;
     real x[kWR];
 #pragma unroll
     for (int r = 0; r < kWR; ++r) {
@@ -1330,6 +1336,7 @@ __global__ __launch_bounds__(kThreads) void k_init_w(SeriesDev sd, InitArgs<real
         if (u < nt) {
             x[r] = val(t0 + u);
             out[t0 + u] = x[r];
+            if (resid) resid[t0 + u] = x[r];
         }
     }
     const real xl = t0 >= 1 ? val(t0 - 1) : (real)0;

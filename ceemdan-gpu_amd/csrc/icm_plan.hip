@@ -399,7 +399,7 @@ int64_t plan_bytes_estimate(const icm_config* c)
     int64_t b = 0;
     b += 2 * P.samples_max * s;                    // h ping-pong
     b += (int64_t)(M - 1) * P.hn.samples * s;      // noise modes
-    b += P.hn.samples * s;                         // noise residual
+    b += 2 * P.hn.samples * s;                     // noise residuals (two buffers)
     b += 2 * P.kstride * s;                        // c
     b += 2 * 2 * 2 * P.ntiles_max * 4 + 2 * 2 * P.nser_max * (4 + 2 * s);
     b += 2 * P.ntiles_max * (6 * (4 + s) + 8 * 8);  // neighbour knots, knot bitmasks
@@ -418,7 +418,7 @@ int64_t plan_bytes_estimate(const icm_config* c)
 
 template <typename real>
 struct Plan : PlanBase {
-    real *hA = nullptr, *hB = nullptr, *modes = nullptr, *wres = nullptr;
+    real *hA = nullptr, *hB = nullptr, *modes = nullptr, *wres = nullptr, *wres2 = nullptr;
     int* kcnt[2] = {nullptr, nullptr};
     int* kpre[2] = {nullptr, nullptr};
     int* nk[2] = {nullptr, nullptr};
@@ -462,6 +462,7 @@ struct Plan : PlanBase {
         if ((rc = alloc(&hB, samples_max))) return rc;
         if ((rc = alloc(&modes, (int64_t)(M - 1) * hn.samples))) return rc;
         if ((rc = alloc(&wres, hn.samples))) return rc;
+        if ((rc = alloc(&wres2, hn.samples))) return rc;
         for (int b = 0; b < 2; ++b) {
             if ((rc = alloc(&kcnt[b], 2 * ntiles_max))) return rc;
             if ((rc = alloc(&kpre[b], 2 * ntiles_max))) return rc;
@@ -742,10 +743,21 @@ struct Plan : PlanBase {
     {
         const int M = cfg.num_imfs;
         int rc;
-        InitArgs<real> ia{};
-        ia.mode = INIT_COPY;
-        ia.src = wres;
+        // stage k starts from the residual w - E_0 - ... - E_{k-1}; stage
+        // k >= 1 forms it from the previous residual and E_{k-1} while it
+        // reads it (k_init_w, INIT_RESID), alternating two buffers
+        real* res[2] = {wres, wres2};
         for (int k = 0; k < M - 1; ++k) {
+            InitArgs<real> ia{};
+            if (k == 0) {
+                ia.mode = INIT_COPY;
+                ia.src = wres;
+            } else {
+                ia.mode = INIT_RESID;
+                ia.src = res[(k - 1) & 1];
+                ia.E = modes + (int64_t)(k - 1) * hn.samples;
+                ia.resid = res[k & 1];
+            }
             real* Ek = modes + (int64_t)k * hn.samples;
             if (cfg.stop_mode == ICM_STOP_FIXED) {
                 if ((rc = stage_fixed(ICM_PHASE_NOISE, k, dn, ia, cfg.noise_max_sifts, Ek, st)))
@@ -756,7 +768,6 @@ struct Plan : PlanBase {
                 ICM_TRY(launch_select_copy<real>(dn.view(kstride, cap), dn.max_len, hA, hB, stop_pass,
                                                  Ek, st));
             }
-            ICM_TRY(launch_sub<real>(wres, Ek, hn.samples, st));
         }
         return ICM_OK;
     }
