@@ -86,7 +86,8 @@ struct InitArgs {
     real* resid;            // INIT_RESID: h0 also stored here (another buffer than src)
     const real* chan_sig;   // INIT_STAGE0/K: per-channel signal (xt or r)
     const int64_t* chan_off;  // offset of channel c in chan_sig
-    const real* E;          // noise modes of this stage (eoff offsets)
+    const real* E;          // noise modes of this stage (eoff offsets); INIT_RESID: the
+                            //  previous noise mode (series offsets)
     const double* scale;    // INIT_STAGE0: inv[s] per series; INIT_STAGEK: sq[c] per channel
     real eps0;
 };
