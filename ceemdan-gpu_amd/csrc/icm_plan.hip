@@ -133,7 +133,7 @@ static void build_series(HostSeries& hs, const std::vector<int64_t>& lens, int I
     int seg = 1;
     const char* env = std::getenv("ICM_SEGMENT_TILES");
     if (env && std::atoi(env) > 0) seg = std::atoi(env);
-    else seg = std::max(1, std::min(32, tiles / (T == 256 ? 32768 : 6144)));
+    else seg = std::max(1, std::min(32, tiles / (T == 256 ? 56000 : 6144)));
     for (size_t s = 0; s < hs.ntl.size(); ++s)
         for (int t = 0; t < hs.ntl[s]; t += seg) {
             Segment g;
