@@ -550,7 +550,7 @@ struct Plan : PlanBase {
         return k;
     }
 
-    int solve_grid() const { return 2048; }
+    int solve_grid() const { return 4096; }
 
 #define ICM_TRY(x)                                     \
     do {                                               \
