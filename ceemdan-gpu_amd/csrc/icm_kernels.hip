@@ -1290,7 +1290,9 @@ __global__ __launch_bounds__(kThreads) void k_sift_list(SiftArgs<real> a, int ns
 
 // Stage start for 256-sample tiles, one wavefront per tile: forms x_i
 // (the same arithmetic as init_value) and stores its knots as bitmasks.
-template <typename real>
+// MODE is ia.mode as a compile-time constant: one straight-line body per
+// mode, so that every load of the tile goes out before the first wait.
+template <typename real, int MODE>
 __global__ __launch_bounds__(kThreads) void k_init_w(SeriesDev sd, InitArgs<real> ia, real* h_out,
                                                      KnotsDev<real> kout)
 {
@@ -1308,39 +1310,49 @@ __global__ __launch_bounds__(kThreads) void k_init_w(SeriesDev sd, InitArgs<real
     const real* E = nullptr;
     P scl0 = (P)0;
     real sclk = (real)0;
-    if (ia.mode == INIT_COPY) {
+    if (MODE == INIT_COPY) {
         src = ia.src + sd.off[s];
-    } else if (ia.mode == INIT_RESID) {
+    } else if (MODE == INIT_RESID) {
         src = ia.src + sd.off[s];
         E = ia.E + sd.off[s];
     } else {
         const int c = sd.chan[s];
         base = ia.chan_sig + ia.chan_off[c];
         E = ia.E + sd.eoff[s];
-        if (ia.mode == INIT_STAGE0) scl0 = (P)ia.scale[sd.nidx[s]];
+        if (MODE == INIT_STAGE0) scl0 = (P)ia.scale[sd.nidx[s]];
         else sclk = (real)ia.scale[c];
     }
     auto val = [&](int p) -> real {
-        if (ia.mode == INIT_COPY) return src[p];
-        if (ia.mode == INIT_RESID) return src[p] - E[p];
+        if (MODE == INIT_COPY) return src[p];
+        if (MODE == INIT_RESID) return src[p] - E[p];
         const real e = E[p];
-        if (ia.mode == INIT_STAGE0) return (real)((P)base[p] + (P)(ia.eps0 * e) * scl0);
+        if (MODE == INIT_STAGE0) return (real)((P)base[p] + (P)(ia.eps0 * e) * scl0);
         return base[p] + (ia.eps0 * e) * sclk;
     };
-    real* resid = ia.mode == INIT_RESID ? ia.resid + sd.off[s] : nullptr;
+    real* resid = MODE == INIT_RESID ? ia.resid + sd.off[s] : nullptr;
+    // every load of the tile before any store (the compiler may not move a
+    // load across a store to another pointer, so interleaving them would
+    // make each row wait for the previous one)
+    // (unconditional loads at clamped indices, then selects: a load under a
+    // lane condition would sit in its own block, waited for at its end)
     real x[kWR];
 #pragma unroll
     for (int r = 0; r < kWR; ++r) {
         const int u = r * kWave + lane;
-        x[r] = (real)0;
+        const real v = val(min(t0 + u, N - 1));
+        x[r] = u < nt ? v : (real)0;
+    }
+    const real xlv = val(max(t0 - 1, 0)), xrv = val(min(t0 + kWT, N - 1));
+    const real xl = t0 >= 1 ? xlv : (real)0;
+    const real xr = t0 + kWT < N ? xrv : (real)0;
+#pragma unroll
+    for (int r = 0; r < kWR; ++r) {
+        const int u = r * kWave + lane;
         if (u < nt) {
-            x[r] = val(t0 + u);
             out[t0 + u] = x[r];
             if (resid) resid[t0 + u] = x[r];
         }
     }
-    const real xl = t0 >= 1 ? val(t0 - 1) : (real)0;
-    const real xr = t0 + kWT < N ? val(t0 + kWT) : (real)0;
     int run[2] = {0, 0};
     uint64_t em[2][kWR];
 #pragma unroll
@@ -2047,8 +2059,21 @@ template <typename real>
 hipError_t launch_init(const SeriesDev& sd, const InitArgs<real>& ia, real* h_out,
                        const KnotsDev<real>& kout, const PassCtl&, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_init_w<real>), dim3(blocks_for(sd.ntiles, kThreads / kWave)),
-                       dim3(kThreads), 0, st, sd, ia, h_out, kout);
+    const dim3 grid(blocks_for(sd.ntiles, kThreads / kWave));
+    switch (ia.mode) {
+    case INIT_COPY:
+        hipLaunchKernelGGL((k_init_w<real, INIT_COPY>), grid, dim3(kThreads), 0, st, sd, ia, h_out, kout);
+        break;
+    case INIT_RESID:
+        hipLaunchKernelGGL((k_init_w<real, INIT_RESID>), grid, dim3(kThreads), 0, st, sd, ia, h_out, kout);
+        break;
+    case INIT_STAGE0:
+        hipLaunchKernelGGL((k_init_w<real, INIT_STAGE0>), grid, dim3(kThreads), 0, st, sd, ia, h_out, kout);
+        break;
+    default:
+        hipLaunchKernelGGL((k_init_w<real, INIT_STAGEK>), grid, dim3(kThreads), 0, st, sd, ia, h_out, kout);
+        break;
+    }
     return hipGetLastError();
 }
 
