@@ -324,14 +324,22 @@ __global__ __launch_bounds__(kPT) void k_knot_prefix_1(SeriesDev sd, KnotsDev<re
     LastTwo sc[2];
     int inc[2], mn[2];
     int first[2];         // tile offset of the tile's first knot per type
+    // both types' counts, then both types' bitmasks, in one round trip each
+    c[0] = t < nt ? k.cnt[2 * (tb + t)] : 0;
+    c[1] = t < nt ? k.cnt[2 * (tb + t) + 1] : 0;
+    uint64_t mm[2][kWR];
+    if (c[0] > 0 || c[1] > 0) {
+#pragma unroll
+        for (int q = 0; q < 2 * kWR; ++q) mm[q / kWR][q % kWR] = k.mask[8 * (int64_t)(tb + t) + q];
+    }
 #pragma unroll
     for (int ty = 0; ty < 2; ++ty) {
-        c[ty] = t < nt ? k.cnt[2 * (tb + t) + ty] : 0;
         sc[ty] = lt_none();
         first[ty] = -1;
         if (c[ty] > 0) {
             uint64_t m[kWR];
-            load_masks(k, tb + t, ty, m);
+#pragma unroll
+            for (int r = 0; r < kWR; ++r) m[r] = mm[ty][r];
             first[ty] = mask_first(m);
             const int b1 = mask_last(m);
             m[b1 / kWave] ^= 1ull << (b1 % kWave);
@@ -406,26 +414,34 @@ __global__ __launch_bounds__(kPT) void k_knot_prefix_1(SeriesDev sd, KnotsDev<re
     }
     __syncthreads();
     if (t < nt) {
+        // positions of both types' neighbour knots, then all six value loads
+        // (at clamped positions, so that none sits under a condition), then
+        // the stores
+        int ex[2], xs[2][3];
 #pragma unroll
         for (int ty = 0; ty < 2; ++ty) {
-            const int ex = s_ex_tot[ty][wv] + (lane > 0 ? xi[ty] : 0);
+            ex[ty] = s_ex_tot[ty][wv] + (lane > 0 ? xi[ty] : 0);
             const LastTwo before = lt_combine(s_ex_lt[ty][wv], lane > 0 ? xl[ty] : lt_none());
             const int after = min(s_ex_min[ty][wv], lane < kWave - 1 ? xm[ty] : INF);
-            const int x1 = before.l1 >= 0 ? (int)before.l1 : -1;
-            const real y1 = before.l1 >= 0 ? hs[before.l1] : (real)0;
-            const int x2 = before.l2 >= 0 ? (int)before.l2 : -1;
-            const real y2 = before.l2 >= 0 ? hs[before.l2] : (real)0;
-            const int x3 = after == INF ? -1 : after * kWT + s_first[ty][after];
-            const real y3 = after == INF ? (real)0 : hs[x3];
-            k.pre[2 * (tb + t) + ty] = ex;
-            s_pre[ty][t] = ex;
+            xs[ty][0] = before.l1 >= 0 ? (int)before.l1 : -1;
+            xs[ty][1] = before.l2 >= 0 ? (int)before.l2 : -1;
+            xs[ty][2] = after == INF ? -1 : after * kWT + s_first[ty][after];
+        }
+        real ys[2][3];
+#pragma unroll
+        for (int ty = 0; ty < 2; ++ty)
+#pragma unroll
+            for (int w = 0; w < 3; ++w) ys[ty][w] = hs[max(xs[ty][w], 0)];
+#pragma unroll
+        for (int ty = 0; ty < 2; ++ty) {
+            k.pre[2 * (tb + t) + ty] = ex[ty];
+            s_pre[ty][t] = ex[ty];
             const int64_t o = 6 * (int64_t)(tb + t) + 3 * ty;
-            k.nbx[o] = x1;
-            k.nby[o] = y1;
-            k.nbx[o + 1] = x2;
-            k.nby[o + 1] = y2;
-            k.nbx[o + 2] = x3;
-            k.nby[o + 2] = y3;
+#pragma unroll
+            for (int w = 0; w < 3; ++w) {
+                k.nbx[o + w] = xs[ty][w];
+                k.nby[o + w] = xs[ty][w] >= 0 ? ys[ty][w] : (real)0;
+            }
         }
     }
     const int n0 = s_all[0], n1 = s_all[1];
