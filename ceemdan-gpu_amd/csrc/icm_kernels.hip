@@ -523,28 +523,32 @@ template <typename real>
 __device__ __forceinline__ void spline_row(int n, long g, int i, const int* sp, const real* sy,
                                            bool nak, real& dl, real& d, real& du, real& f)
 {
-    if (g <= 0 || g >= n - 1) {
-        dl = (real)0; d = (real)1; du = (real)0; f = (real)0;
-        return;
-    }
+    // computed on every lane and selected at the end (an early return here
+    // kept the four outputs in scratch memory); rows outside (0, n-1) read
+    // neighbours that exist in the window's LDS but take the identity row
+    const bool inner = g > 0 && g < n - 1;
     const int hl = sp[i] - sp[i - 1];
     const int hr = sp[i + 1] - sp[i];
-    dl = (real)hl;
-    du = (real)hr;
-    d = (real)(2 * (hl + hr));
-    f = (real)3 * ((sy[i + 1] - sy[i]) / (real)hr - (sy[i] - sy[i - 1]) / (real)hl);
+    real tdl = (real)hl;
+    real tdu = (real)hr;
+    real td = (real)(2 * (hl + hr));
+    const real tf = (real)3 * ((sy[i + 1] - sy[i]) / (real)hr - (sy[i] - sy[i - 1]) / (real)hl);
     if (nak && n >= 4) {
         if (g == 1) {
-            dl = (real)0;
-            d = (real)(3 * hl + 2 * hr) + (real)hl * (real)hl / (real)hr;
-            du = (real)hr - (real)hl * (real)hl / (real)hr;
+            tdl = (real)0;
+            td = (real)(3 * hl + 2 * hr) + (real)hl * (real)hl / (real)hr;
+            tdu = (real)hr - (real)hl * (real)hl / (real)hr;
         }
         if (g == n - 2) {
-            dl = (real)hl - (real)hr * (real)hr / (real)hl;
-            d = (real)(2 * hl + 3 * hr) + (real)hr * (real)hr / (real)hl;
-            du = (real)0;
+            tdl = (real)hl - (real)hr * (real)hr / (real)hl;
+            td = (real)(2 * hl + 3 * hr) + (real)hr * (real)hr / (real)hl;
+            tdu = (real)0;
         }
     }
+    dl = inner ? tdl : (real)0;
+    d = inner ? td : (real)1;
+    du = inner ? tdu : (real)0;
+    f = inner ? tf : (real)0;
 }
 
 // not-a-knot end value: c_e = c_1 (1 + r) - c_2 r, r = h_e / h_next
@@ -571,9 +575,10 @@ __device__ __forceinline__ void wave_sync()
 // binary search and the knot's bit by selection.  hint = the tile holding
 // knot glo, or -1 (64-ary search).  NIT >= the knots of one window / 64.
 template <int NIT>
-__device__ void gather_positions(const SeriesDev& sd, const uint64_t* kmask, const int* kpre,
-                                 const int* kcnt, int s, int type, long lo, int glo, int ghi,
-                                 int* sp, int* tpre, uint64_t (*tmask)[kWR], int hint)
+__device__ __forceinline__ void gather_positions_body(const SeriesDev& sd, const uint64_t* kmask,
+                                                      const int* kpre, const int* kcnt, int s,
+                                                      int type, long lo, int glo, int ghi, int* sp,
+                                                      int* tpre, uint64_t (*tmask)[kWR], int hint)
 {
     const int lane = threadIdx.x & 63;
     const int tb = load_scalar(sd.tile0 + s), ntl = load_scalar(sd.ntl + s);
@@ -649,6 +654,16 @@ __device__ void gather_positions(const SeriesDev& sd, const uint64_t* kmask, con
         tile += kWave;
         wave_sync();
     }
+}
+
+// The same as a called function (for kernels whose registers are tight).
+template <int NIT>
+__device__ __noinline__ void gather_positions(const SeriesDev& sd, const uint64_t* kmask,
+                                              const int* kpre, const int* kcnt, int s, int type,
+                                              long lo, int glo, int ghi, int* sp, int* tpre,
+                                              uint64_t (*tmask)[kWR], int hint)
+{
+    gather_positions_body<NIT>(sd, kmask, kpre, kcnt, s, type, lo, glo, ghi, sp, tpre, tmask, hint);
 }
 
 // Values of knots [glo, ghi) (positions already in sp) from the series' pass
@@ -750,9 +765,10 @@ __device__ __forceinline__ void partition_pcr(int lane, RowFn row, real (&x)[B])
     for (int q = 0; q < B; ++q) x[q] = (q == B - 1) ? z : (al[q] + be[q] * zl) + ga[q] * z;
 }
 
-// (four waves per SIMD: the compiler is held to 128 registers)
+// (three waves per SIMD in FP64, ≈ 146 registers: held to 128 for four
+// waves, the partition + PCR spills to scratch and the solve is 7 % slower)
 template <typename real, int B, int H>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_solve(SolveArgs<real> a)
+__global__ __launch_bounds__(kThreads) void k_solve(SolveArgs<real> a)
 {
     constexpr int W = kWave * B;
     constexpr int K = W - 2 * H;
