@@ -574,11 +574,13 @@ __device__ __forceinline__ void wave_sync()
 // indices; otherwise each lane finds the tile of its knots g + lane + 64 j by
 // binary search and the knot's bit by selection.  hint = the tile holding
 // knot glo, or -1 (64-ary search).  NIT >= the knots of one window / 64.
+// (inlined: as a called function it cost the solve 8 % and the adjoint
+// window solve 1 %)
 template <int NIT>
-__device__ __forceinline__ void gather_positions_body(const SeriesDev& sd, const uint64_t* kmask,
-                                                      const int* kpre, const int* kcnt, int s,
-                                                      int type, long lo, int glo, int ghi, int* sp,
-                                                      int* tpre, uint64_t (*tmask)[kWR], int hint)
+__device__ __forceinline__ void gather_positions(const SeriesDev& sd, const uint64_t* kmask,
+                                                 const int* kpre, const int* kcnt, int s, int type,
+                                                 long lo, int glo, int ghi, int* sp, int* tpre,
+                                                 uint64_t (*tmask)[kWR], int hint)
 {
     const int lane = threadIdx.x & 63;
     const int tb = load_scalar(sd.tile0 + s), ntl = load_scalar(sd.ntl + s);
@@ -654,16 +656,6 @@ __device__ __forceinline__ void gather_positions_body(const SeriesDev& sd, const
         tile += kWave;
         wave_sync();
     }
-}
-
-// The same as a called function (for kernels whose registers are tight).
-template <int NIT>
-__device__ __noinline__ void gather_positions(const SeriesDev& sd, const uint64_t* kmask,
-                                              const int* kpre, const int* kcnt, int s, int type,
-                                              long lo, int glo, int ghi, int* sp, int* tpre,
-                                              uint64_t (*tmask)[kWR], int hint)
-{
-    gather_positions_body<NIT>(sd, kmask, kpre, kcnt, s, type, lo, glo, ghi, sp, tpre, tmask, hint);
 }
 
 // Values of knots [glo, ghi) (positions already in sp) from the series' pass
@@ -765,7 +757,7 @@ __device__ __forceinline__ void partition_pcr(int lane, RowFn row, real (&x)[B])
     for (int q = 0; q < B; ++q) x[q] = (q == B - 1) ? z : (al[q] + be[q] * zl) + ga[q] * z;
 }
 
-// (three waves per SIMD in FP64, ≈ 146 registers: held to 128 for four
+// (three waves per SIMD in FP64, ≈ 147 registers: held to 128 for four
 // waves, the partition + PCR spills to scratch and the solve is 7 % slower)
 template <typename real, int B, int H>
 __global__ __launch_bounds__(kThreads) void k_solve(SolveArgs<real> a)
