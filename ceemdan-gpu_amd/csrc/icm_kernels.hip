@@ -13,7 +13,11 @@
 //                   LDS, upper/lower envelope, mean, subtract; then the
 //                   extrema of the new samples as the next pass's bitmasks
 //   k_knot_prefix_* per series: tile knot prefixes, neighbour knots of every
-//                   tile, the solve's window list
+//                   tile, the solve's window list, the count of tiles too
+//                   full for the small knot table
+// From the second stage of a phase on, k_sift_w runs with a small knot table
+// (more waves per SIMD) and k_sift_list sifts, with the full table, the tiles
+// it could not hold (or every segment, when the count says they are many).
 // Reference counterparts: find_extrema_shfl_* :92-148, CUB ExclusiveSum
 // :859-883, select_extrema_* :150-207, setBoundary* :209-291,
 // tridiagonal_setup :310-343, gtsv2_nopivot :899-921, spline_coefficients
