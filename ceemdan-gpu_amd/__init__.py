@@ -149,6 +149,32 @@ def _require_cuda(*ts):
             raise ValueError("tensors must be contiguous")
 
 
+def _check_rows(bufs, lengths, rows, dtype, what):
+    """Caller-supplied per-channel buffers: one per channel, the plan's dtype,
+    rows * N_c elements each, on the GPU and contiguous (the library writes
+    exactly that many elements through each pointer)."""
+    if len(bufs) != len(lengths):
+        raise ValueError(f"{what}: {len(bufs)} buffers for {len(lengths)} channels")
+    for b, n in zip(bufs, lengths):
+        _require_cuda(b)
+        if b.dtype != dtype:
+            raise ValueError(f"{what}: dtype {b.dtype} differs from the plan's {dtype}")
+        if b.numel() != rows * n:
+            raise ValueError(f"{what}: {b.numel()} elements where {rows} x {n} are needed")
+
+
+def _check_noise(noise, num_noise, lengths, dtype):
+    """Caller noise: [I, N] of the plan's dtype; all channels must share N."""
+    _require_cuda(noise)
+    if noise.dtype != dtype:
+        raise ValueError("noise dtype differs from the plan")
+    if len(set(lengths)) != 1:
+        raise ValueError("caller noise needs channels of one length")
+    if noise.numel() != num_noise * lengths[0]:
+        raise ValueError(f"noise has {noise.numel()} elements where "
+                         f"{num_noise} x {lengths[0]} are needed")
+
+
 def make_config(lengths, num_noise, num_imfs, dtype_code=DTYPE_F64, stop_mode=STOP_FIXED,
                 max_sifts=10, noise_max_sifts=None, noise_strength=0.2, seed=1234, use_graph=1,
                 boundary=BOUNDARY_NATURAL, keep_records=False):
@@ -270,12 +296,9 @@ class Plan:
         if out is None:
             out = [torch.empty((self.num_imfs, n), dtype=self.dtype, device=signals[0].device)
                    for n in self.lengths]
-        for o in out:
-            _require_cuda(o)
+        _check_rows(out, self.lengths, self.num_imfs, self.dtype, "forward out")
         if noise is not None:
-            _require_cuda(noise)
-            if noise.dtype != self.dtype:
-                raise ValueError("noise dtype differs from the plan")
+            _check_noise(noise, self.num_noise, self.lengths, self.dtype)
         sp = (C.c_void_p * len(signals))(*[s.data_ptr() for s in signals])
         op = (C.c_void_p * len(out))(*[o.data_ptr() for o in out])
         st = _stream_ptr(signals[0].device)
@@ -302,7 +325,7 @@ class Plan:
         """Inputs, noise modes and stage-0 statistics; returns stage 0."""
         signals = self._signals(signals)
         if noise is not None:
-            _require_cuda(noise)
+            _check_noise(noise, self.num_noise, self.lengths, self.dtype)
         sp = (C.c_void_p * len(signals))(*[s.data_ptr() for s in signals])
         _check(lib().icm_forward_begin(self._h, sp,
                                        C.c_void_p(noise.data_ptr()) if noise is not None else None,
@@ -321,8 +344,7 @@ class Plan:
         if out is None:
             out = [torch.empty((self.num_imfs, n), dtype=self.dtype, device=self.device)
                    for n in self.lengths]
-        for o in out:
-            _require_cuda(o)
+        _check_rows(out, self.lengths, self.num_imfs, self.dtype, "forward_finish out")
         op = (C.c_void_p * len(out))(*[o.data_ptr() for o in out])
         _check(lib().icm_forward_finish(self._h, op, _stream_ptr(self.device)), "icm_forward_finish")
         self.forward_count = getattr(self, "forward_count", 0) + 1
@@ -343,7 +365,7 @@ class Plan:
         import numpy as np
         buf = np.ascontiguousarray(buf, dtype=np.uint8)
         if noise is not None:
-            _require_cuda(noise)
+            _check_noise(noise, self.num_noise, self.lengths, self.dtype)
         _check(lib().icm_state_load(self._h, C.c_void_p(buf.ctypes.data), buf.size,
                                     C.c_void_p(noise.data_ptr()) if noise is not None else None,
                                     _stream_ptr(self.device)), "icm_state_load")
@@ -367,8 +389,7 @@ class Plan:
             gs.append(g)
         if out is None:
             out = [torch.empty(n, dtype=self.dtype, device=gs[0].device) for n in self.lengths]
-        for o in out:
-            _require_cuda(o)
+        _check_rows(out, self.lengths, 1, self.dtype, "backward out")
         gp = (C.c_void_p * len(gs))(*[g.data_ptr() for g in gs])
         op = (C.c_void_p * len(out))(*[o.data_ptr() for o in out])
         _check(lib().icm_backward(self._h, gp, op, _stream_ptr(gs[0].device)), "icm_backward")

@@ -23,7 +23,7 @@ double run(int dtype, size_t I, size_t N, size_t M, size_t max_iter, int* d_x, r
         return st < 0 ? (double)st : -1.0;
     };
     if (!d_x || !d_y || !d_imfs) return fail(ICM_ERR_INVALID_ARG);
-    if (I < 1 || N < 4 || M < 2 || max_iter < 1 || I > 65535 || M > 64 || max_iter > 100000)
+    if (I < 1 || N < 4 || M < 2 || max_iter < 1 || I > 65535 || M > 64 || max_iter > 32000)
         return fail(ICM_ERR_INVALID_ARG);
     // d_x must be the sample numbers (cudaICEEMDAN.cu:423 compares them with p)
     int* d_err = nullptr;

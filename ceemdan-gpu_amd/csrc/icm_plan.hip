@@ -42,6 +42,12 @@ int hip_fail(hipError_t e, int line)
 
 const int64_t kMaxLen = 16000000;
 const uint32_t kEpochs = 65536;
+// Passes per stage: one epoch per pass (FIXED uses S + 1 epochs per stage,
+// RILLING at most cap + 1, and the control words are reset when fewer than
+// 2 cap + 8 remain), so a cap must leave a stage's epochs below kEpochs.
+const int kMaxSifts = 32000;
+// Channels: the per-channel launches put the channel on gridDim.y.
+const int kMaxChannels = 65535;
 
 // --------------------------------------------------------------- geometry
 
@@ -58,14 +64,15 @@ int validate(const icm_config* c)
 {
     if (!c) return ICM_ERR_INVALID_ARG;
     if (c->dtype != ICM_DTYPE_F32 && c->dtype != ICM_DTYPE_F64) return ICM_ERR_INVALID_ARG;
-    if (c->num_channels < 1 || !c->lengths) return ICM_ERR_INVALID_ARG;
+    if (c->num_channels < 1 || c->num_channels > kMaxChannels || !c->lengths)
+        return ICM_ERR_INVALID_ARG;
     for (int i = 0; i < c->num_channels; ++i)
         if (c->lengths[i] < 4 || c->lengths[i] > kMaxLen) return ICM_ERR_INVALID_ARG;
     if (c->num_noise < 1 || c->num_noise > 65535) return ICM_ERR_INVALID_ARG;
     if (c->num_imfs < 2 || c->num_imfs > 64) return ICM_ERR_INVALID_ARG;
     if (c->stop_mode != ICM_STOP_FIXED && c->stop_mode != ICM_STOP_RILLING) return ICM_ERR_INVALID_ARG;
-    if (c->max_sifts < 1 || c->max_sifts > 100000) return ICM_ERR_INVALID_ARG;
-    if (c->noise_max_sifts < 1 || c->noise_max_sifts > 100000) return ICM_ERR_INVALID_ARG;
+    if (c->max_sifts < 1 || c->max_sifts > kMaxSifts) return ICM_ERR_INVALID_ARG;
+    if (c->noise_max_sifts < 1 || c->noise_max_sifts > kMaxSifts) return ICM_ERR_INVALID_ARG;
     if (!std::isfinite(c->noise_strength)) return ICM_ERR_INVALID_ARG;
     if (c->boundary != ICM_BOUNDARY_NATURAL && c->boundary != ICM_BOUNDARY_NOT_A_KNOT)
         return ICM_ERR_INVALID_ARG;
@@ -896,6 +903,10 @@ struct Plan : PlanBase {
         const int C = cfg.num_channels, M = cfg.num_imfs;
         *h_err = 0;
         stage_ = -1;
+        // the saved state is overwritten from here on: a backward is valid
+        // again only after this forward has completed
+        have_forward = false;
+        std::fill(krec_valid.begin(), krec_valid.end(), (char)0);
         ICM_TRY(hipEventRecord(ev[0], st));
         ICM_TRY(launch_fill_u32((uint32_t*)err, 0u, 1, st));
         for (int c = 0; c < C; ++c)

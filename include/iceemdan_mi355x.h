@@ -50,13 +50,14 @@ extern "C" {
 
 typedef struct icm_config {
     int dtype;                  /* ICM_DTYPE_F32 / ICM_DTYPE_F64 */
-    int num_channels;           /* C >= 1 */
+    int num_channels;           /* 1 <= C <= 65535 */
     const int64_t* lengths;     /* host array [C]: samples per channel (4 .. 16,000,000) */
     int num_noise;              /* I >= 1: noise realizations per channel */
     int num_imfs;               /* M >= 2: output rows (M-1 sifted stages + residue) */
     int stop_mode;              /* ICM_STOP_FIXED / ICM_STOP_RILLING */
-    int max_sifts;              /* FIXED: sifts per stage; RILLING: cap (reference max_iter) */
-    int noise_max_sifts;        /* the same for the noise-mode stages (reference: 2000) */
+    int max_sifts;              /* FIXED: sifts per stage; RILLING: cap (reference max_iter);
+                                   1 .. 32000 */
+    int noise_max_sifts;        /* the same for the noise-mode stages (reference: 2000), 1 .. 32000 */
     double noise_strength;      /* eps0 */
     uint64_t seed;              /* Philox key of the white noise */
     int use_graph;              /* FIXED only: replay the forward as a captured hipGraph */

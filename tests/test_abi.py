@@ -82,9 +82,18 @@ def test_host_validation(icm):
     bad = [dict(lengths=[3]), dict(num_noise=0), dict(num_imfs=1),
            dict(max_sifts=0), dict(stop_mode=7), dict(dtype_code=5),
            dict(noise_strength=float("nan")), dict(lengths=[20_000_000]), dict(boundary=2),
-           dict(num_noise=70000)]
+           dict(num_noise=70000),
+           # one epoch per pass: a stage's passes must stay below the 65536
+           # control words, so caps above 32000 are refused
+           dict(max_sifts=32001), dict(noise_max_sifts=32001),
+           dict(stop_mode=1, max_sifts=65535),
+           # per-channel launches put the channel on gridDim.y
+           dict(lengths=[16] * 65536)]
     for b in bad:
         assert _cfg(icm, **b) == -1, b
+    for ok in (dict(max_sifts=32000), dict(stop_mode=1, noise_max_sifts=32000),
+               dict(lengths=[16] * 65535, num_noise=1)):
+        assert _cfg(icm, **ok) == 0, ok
 
 
 def test_plan_bytes_estimate(icm):

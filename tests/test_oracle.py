@@ -313,6 +313,121 @@ def test_iceemdan_rilling_stops(orc):
     assert np.max(np.abs(imfs.sum(0) - xt)) < 1e-12
 
 
+# ------------------------------------------------------ stage algebra pins
+# Each check below fails on one plausible misreading of the stage algebra
+# (cudaICEEMDAN.cu:41-90 kernels, :845-994 noise modes, :1003-1298 stages):
+#   - a missing 1/std(E_1^i) at stage 0 (:76), or one std shared by all i
+#   - a normalised E_{k+1} at stages k >= 1 (:88)
+#   - the wrong noise-mode index at stage 0 or at stages k >= 1
+#   - exporting h - m instead of h when the Rilling rule stops (:535-542)
+
+
+def _alt_ramp(I, N, slopes, amps):
+    """Realization i: slopes[i] * (p - N/2) + amps[i] * (-1)^(p+i).  With
+    |amps| > |slopes| / 2 every interior sample is a strict extremum, and all
+    values, differences and slopes are exact in binary floating point."""
+    p = np.arange(N, dtype=np.float64)
+    return np.stack([slopes[i] * (p - N // 2) + amps[i] * (-1.0) ** ((np.arange(N) + i) % 2)
+                     for i in range(I)])
+
+
+def test_noise_modes_of_exact_alternation_and_ramp(orc):
+    """Noise = ramp + alternation: the first sift pass meets two straight
+    envelopes (the endpoint fix puts the end knots on the lines), so
+    E_1 = the alternation, E_2 = the ramp (no interior extrema: nothing is
+    subtracted) and E_k = 0 for k >= 3, all exactly."""
+    N, I, M = 512, 3, 5
+    slopes, amps = [1.0, -0.5, 2.0], [1.0, 0.75, 1.5]
+    w = _alt_ramp(I, N, slopes, amps)
+    r = orc.iceemdan(three_tone(N), w, M, S_or_cap=10, want_noise_modes=True)
+    E = r["noise_modes"]
+    ramp = _alt_ramp(I, N, slopes, [0.0] * I)
+    assert np.array_equal(E[0], w - ramp)
+    assert np.array_equal(E[1], ramp)
+    assert np.all(E[2:] == 0.0)
+
+
+def test_stage_noise_normalisation_and_index(orc):
+    """Stage 0 adds eps0 E_1^i / std(E_1^i), stage k >= 1 adds the raw
+    eps0 std(r_k) E_{k+1}^i:
+      (a) scaling one realization's E_1 by 2 (its alternation) changes
+          nothing: stage 0 normalises per realization, later stages do not
+          see E_1;
+      (b) changing E_2 only (the ramps' slopes) keeps row 0 and changes
+          row 1: stage 0 uses E_1, stage 1 uses E_2;
+      (c) scaling the whole noise by 2 (exact in floating point) keeps row 0
+          bit for bit and changes rows >= 1: E_{k+1} is not normalised at
+          k >= 1."""
+    N, I, M = 512, 3, 5
+    x = three_tone(N, seed=4)
+    base = orc.iceemdan(x, _alt_ramp(I, N, [1.0, -0.5, 2.0], [1.0, 0.75, 1.5]), M,
+                        S_or_cap=10)["imfs"]
+    amp2 = orc.iceemdan(x, _alt_ramp(I, N, [1.0, -0.5, 2.0], [2.0, 0.75, 1.5]), M,
+                        S_or_cap=10)["imfs"]
+    assert np.array_equal(amp2, base)
+    slope = orc.iceemdan(x, _alt_ramp(I, N, [0.5, -0.25, 1.0], [1.0, 0.75, 1.5]), M,
+                         S_or_cap=10)["imfs"]
+    assert np.array_equal(slope[0], base[0])
+    assert not np.array_equal(slope[1], base[1])
+    w = orc.noise(3, 6, 1500)
+    x = three_tone(1500, seed=5)
+    one = orc.iceemdan(x, w, M, S_or_cap=10)["imfs"]
+    two = orc.iceemdan(x, 2.0 * w, M, S_or_cap=10)["imfs"]
+    assert np.array_equal(one[0], two[0])
+    for k in range(1, M):
+        assert not np.array_equal(one[k], two[k]), k
+    # and the noise does enter stage 0: eps0 = 0 gives another row 0
+    zero = orc.iceemdan(x, w, M, S_or_cap=10, eps0=0.0)["imfs"]
+    assert np.max(np.abs(zero[0] - one[0])) > 1e-3
+
+
+@pytest.mark.parametrize("noise_cap", [1, 2000])
+def test_rilling_cap_one_exports_unsifted(orc, noise_cap):
+    """RILLING with a cap of one pass exports each realization's input
+    unsubtracted (cudaICEEMDAN.cu:535-542): r_1 = mean(x_i - x_i) = 0, so
+    the rows are exactly [x~, 0, ..., 0]."""
+    N, I, M = 800, 5, 4
+    x = three_tone(N, seed=6)
+    r = orc.iceemdan(x, orc.noise(4, I, N), M, stop_mode=orc.STOP_RILLING, S_or_cap=1,
+                     noise_cap=noise_cap)
+    imfs = r["imfs"]
+    xt = x / np.sqrt(np.var(x, ddof=1))
+    assert np.max(np.abs(imfs[0] - xt)) <= 1e-14 * np.max(np.abs(xt))
+    assert np.all(imfs[1:] == 0.0)
+    assert np.all(r["stops"][1] == 0)
+
+
+def test_box_muller_pairs(orc):
+    """The Gaussian stream is Box-Muller on numpy's Philox4x64-10 words:
+    z0 = sqrt(-2 ln u0) cos(2 pi u1), z1 = sqrt(-2 ln u0) sin(2 pi u1) (and
+    the same on u2, u3), u = ((bits >> 11) + 1/2) 2^-53; the pairs are
+    uncorrelated and each component passes a Kolmogorov-Smirnov test."""
+    from scipy import stats
+    I, N, seed = 3, 40000, 9
+    w = orc.noise(seed, I, N)
+    for i in range(I):
+        # numpy increments its counter before each block: start one below
+        # (0, i, 0, 0), borrowing through the words
+        start = [2**64 - 1, i - 1, 0, 0] if i else [2**64 - 1] * 4
+        bg = np.random.Philox(key=np.array([seed, 0], dtype=np.uint64),
+                              counter=np.array(start, dtype=np.uint64))
+        bits = bg.random_raw(N).reshape(-1, 4)
+        u = ((bits >> np.uint64(11)).astype(np.float64) + 0.5) * 2.0 ** -53
+        r0 = np.sqrt(-2.0 * np.log(u[:, 0]))
+        r1 = np.sqrt(-2.0 * np.log(u[:, 2]))
+        ref = np.stack([r0 * np.cos(2 * np.pi * u[:, 1]), r0 * np.sin(2 * np.pi * u[:, 1]),
+                        r1 * np.cos(2 * np.pi * u[:, 3]), r1 * np.sin(2 * np.pi * u[:, 3])],
+                       axis=1).reshape(-1)
+        assert np.max(np.abs(w[i] - ref)) < 1e-13
+    z = w.reshape(-1, 4)
+    n = z.shape[0]
+    for a, b in ((0, 1), (2, 3), (0, 2)):
+        assert abs(np.corrcoef(z[:, a], z[:, b])[0, 1]) < 4.0 / np.sqrt(n)
+    for j in range(4):
+        assert stats.kstest(z[:, j], "norm").pvalue > 1e-3, j
+    assert not np.array_equal(z[:, 0], z[:, 1])
+
+
 # ---------------------------------------------------------- not-a-knot
 
 @pytest.mark.parametrize("n", [4, 5, 9, 40, 3000])
