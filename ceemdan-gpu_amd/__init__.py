@@ -77,6 +77,7 @@ _SIGS = {
     "icm_get_kernel_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double),
                                        C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "icm_get_trace": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int32), C.c_int64]),
+    "icm_get_trace_hash": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint64), C.c_int64]),
     "icm_get_stops": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int32), C.c_int64]),
     "icm_get_noise_modes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     "icm_noise": (C.c_int, [C.c_int, C.c_uint64, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p]),
@@ -266,6 +267,18 @@ class Plan:
         out = np.zeros((self.num_imfs - 1, P, nser, 2), np.int32)
         _check(lib().icm_get_trace(self._h, phase, out.ctypes.data_as(C.POINTER(C.c_int32)),
                                    out.size), "icm_get_trace")
+        return out
+
+    def trace_hash(self, phase):
+        """[M-1, passes, series] uint64 digests of the knot sets of every
+        traced pass input (equal digests: equal extrema index sets)."""
+        import numpy as np
+        inf = self.info()
+        nser = inf["noise_series"] if phase == PHASE_NOISE else inf["signal_series"]
+        out = np.zeros((self.num_imfs - 1, self._trace_passes, nser), np.uint64)
+        _check(lib().icm_get_trace_hash(self._h, phase,
+                                        out.ctypes.data_as(C.POINTER(C.c_uint64)), out.size),
+               "icm_get_trace_hash")
         return out
 
     def stops(self, phase):

@@ -232,6 +232,8 @@ hipError_t launch_select_copy(const SeriesDev& sd, int max_len, const real* hA, 
                               const int* parity, real* dst, hipStream_t st);
 hipError_t launch_rec_left(const int* nbx, int* left, int* right, int ntiles, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st);
+hipError_t launch_knot_digest(const int* tile_series, const int* tile0, const uint64_t* mask,
+                              int ntiles, uint64_t* out, int nseries, hipStream_t st);
 hipError_t launch_check_index(const int* d_x, int64_t N, int* err, hipStream_t st);
 
 }  // namespace icm

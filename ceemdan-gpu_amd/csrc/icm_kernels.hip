@@ -2068,6 +2068,31 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n)
     if (i < n) p[i] = v;
 }
 
+// Order-independent 64-bit digest of the knot sets of a pass input, per
+// series: the sum (mod 2^64) over the series' tiles of a mix of each of the
+// tile's eight bitmask words with its (tile, type, row) index.  Two passes
+// with equal digests met the same knot sets (up to a hash collision).
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    // splitmix64 finalizer (Steele, Lea, Flood 2014)
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ void k_knot_digest(const int* tile_series, const int* tile0, const uint64_t* mask,
+                              int ntiles, unsigned long long* out)
+{
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gt >= ntiles) return;
+    const int s = tile_series[gt];
+    const uint64_t t = (uint64_t)(gt - tile0[s]);
+    uint64_t h = 0;
+#pragma unroll
+    for (int q = 0; q < 2 * kWR; ++q) h += mix64(mask[8 * (int64_t)gt + q] ^ mix64(8 * t + q + 1));
+    atomicAdd(out + s, (unsigned long long)h);
+}
+
 __global__ void k_check_index(const int* x, int64_t N, int* err)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2303,6 +2328,18 @@ hipError_t launch_rec_left(const int* nbx, int* left, int* right, int ntiles, hi
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st)
 {
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_knot_digest(const int* tile_series, const int* tile0, const uint64_t* mask,
+                              int ntiles, uint64_t* out, int nseries, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(2 * (int64_t)nseries, 256)), dim3(256), 0, st,
+                       (uint32_t*)out, 0u, 2 * (int64_t)nseries);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_knot_digest, dim3(blocks_for(ntiles, 256)), dim3(256), 0, st, tile_series,
+                       tile0, mask, ntiles, (unsigned long long*)out);
     return hipGetLastError();
 }
 

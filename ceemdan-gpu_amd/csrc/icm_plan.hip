@@ -210,6 +210,7 @@ struct PlanBase {
     // trace
     int trace_passes = 0;
     int32_t* trace[2] = {nullptr, nullptr};
+    uint64_t* trace_hash[2] = {nullptr, nullptr};   // [M-1][passes][series] knot-set digests
     int32_t* stops[2] = {nullptr, nullptr};
     // timing
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -598,6 +599,8 @@ struct Plan : PlanBase {
         const int64_t n = 2 * (int64_t)d.nseries;
         int32_t* dst = trace[phase] + ((int64_t)stage * trace_passes + pass) * n;
         ICM_TRY(launch_copy<float>((float*)dst, (const float*)nk[b], n, st));
+        uint64_t* dh = trace_hash[phase] + ((int64_t)stage * trace_passes + pass) * d.nseries;
+        ICM_TRY(launch_knot_digest(d.tile_series, d.tile0, kmask[b], d.ntiles, dh, d.nseries, st));
         return ICM_OK;
     }
 
@@ -1434,8 +1437,12 @@ int icm_set_trace(icm_plan plan, int passes)
         const int64_t nn = (int64_t)(M - 1) * passes * 2 * (int64_t)P.hn.len.size();
         const int64_t ns = (int64_t)(M - 1) * passes * 2 * (int64_t)P.hsig.len.size();
         if (P.alloc(&P.trace[0], nn) || P.alloc(&P.trace[1], ns)) return ICM_ERR_OUT_OF_MEMORY;
+        if (P.alloc(&P.trace_hash[0], nn / 2) || P.alloc(&P.trace_hash[1], ns / 2))
+            return ICM_ERR_OUT_OF_MEMORY;
         (void)hipMemset(P.trace[0], 0, nn * 4);
         (void)hipMemset(P.trace[1], 0, ns * 4);
+        (void)hipMemset(P.trace_hash[0], 0, nn * 4);
+        (void)hipMemset(P.trace_hash[1], 0, ns * 4);
     }
     if (P.cfg.stop_mode == ICM_STOP_RILLING && !P.stops[0]) {
         if (P.alloc(&P.stops[0], (int64_t)(M - 1) * P.hn.len.size()) ||
@@ -1454,6 +1461,19 @@ int icm_get_trace(icm_plan plan, int phase, int32_t* host_out, int64_t count)
     const int64_t n = (int64_t)(P.cfg.num_imfs - 1) * P.trace_passes * 2 * nser;
     if (count < n) return ICM_ERR_INVALID_ARG;
     if (hipMemcpy(host_out, P.trace[phase], n * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        return ICM_ERR_HIP;
+    return ICM_OK;
+}
+
+int icm_get_trace_hash(icm_plan plan, int phase, uint64_t* host_out, int64_t count)
+{
+    if (!plan || !host_out || (phase != 0 && phase != 1)) return ICM_ERR_INVALID_ARG;
+    PlanBase& P = *plan->p;
+    if (!P.trace_hash[phase]) return ICM_ERR_STATE;
+    const int64_t nser = phase == 0 ? (int64_t)P.hn.len.size() : (int64_t)P.hsig.len.size();
+    const int64_t n = (int64_t)(P.cfg.num_imfs - 1) * P.trace_passes * nser;
+    if (count < n) return ICM_ERR_INVALID_ARG;
+    if (hipMemcpy(host_out, P.trace_hash[phase], n * 8, hipMemcpyDeviceToHost) != hipSuccess)
         return ICM_ERR_HIP;
     return ICM_OK;
 }

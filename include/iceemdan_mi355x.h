@@ -133,6 +133,10 @@ int icm_get_kernel_stats(icm_plan plan, int kind, double* total_ms, int64_t* lau
  * (n_max, n_min of each pass input) of a phase. */
 int icm_set_trace(icm_plan plan, int passes);
 int icm_get_trace(icm_plan plan, int phase, int32_t* host_out, int64_t count);
+/* Knot-set digests of the same traced passes: [M-1][passes][series] 64-bit
+ * order-independent hashes of both knot sets of each pass input (equal
+ * digests: the same extrema index sets, up to a hash collision). */
+int icm_get_trace_hash(icm_plan plan, int phase, uint64_t* host_out, int64_t count);
 /* RILLING: subtractions performed per [M-1][series] of a phase. */
 int icm_get_stops(icm_plan plan, int phase, int32_t* host_out, int64_t count);
 /* noise modes E_1..E_{M-1} of the last forward: [M-1][noise series][N] */

@@ -1,0 +1,350 @@
+"""The benchmark configurations on the GPU, held to the FP64 oracle.
+
+C3  batched channels of uneven length (lengths drawn like the 64-channel job,
+    seed 3; reduced I and M): FP64 per channel bit for bit against the oracle
+    (modes, noise modes, knot counts of every pass), FP32 per-mode error
+    against the oracle fed the same float32 input and noise.
+C4  forward + adjoint on 16384-sample EEG-like channels: the FP64 gradient
+    against the oracle's; at the full 32 x 16384, I = 100, M = 12 size a
+    central difference (h = 1e-7) in directions whose perturbed forwards
+    meet the same extrema index sets (knot-set digests of every pass).
+C5  one GPU's share of the 128-channel FP32 job at reduced I: completeness
+    and a finite gradient.
+Also the paths earlier compared only with themselves: graph replay, resumed
+staged forward, multi-channel RILLING, batched backward and the command-line
+driver, each against the oracle channel by channel.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def _sig():
+    import importlib
+    return importlib.import_module("ceemdan-gpu_amd.signals")
+
+
+def relerr(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _gpu_noise(icm, dev, I, N, dtype=None, seed=1234):
+    """The plan's own white noise for one length (same kernel, same bits),
+    on the host as float64 for the oracle."""
+    import torch
+    w = icm.noise(I, N, seed=seed, dtype=dtype or torch.float64, device=dev)
+    return w.cpu().numpy().astype(np.float64)
+
+
+def _c3_channels(C):
+    sig = _sig()
+    lens = sig.batch_lengths(64, 8192, 262144, seed=3)[:C]
+    return lens, [sig.eeg_like(n, seed=1000 + c) for c, n in enumerate(lens)]
+
+
+def _noise_mode_rows(flat, lens, I, M):
+    """Split the plan's noise modes [M-1][noise series][N] (one group of I
+    series per distinct length, in first-appearance order) per length."""
+    groups, off, seen = {}, 0, []
+    for n in lens:
+        if n not in groups:
+            groups[n] = off
+            seen.append(n)
+            off += I * n
+    total = off
+    flat = flat.reshape(M - 1, total)
+    return {n: flat[:, o:o + I * n].reshape(M - 1, I, n) for n, o in groups.items()}
+
+
+# -------------------------------------------------------------------- C3
+
+def test_c3_channels_fp64_bitexact(icm, orc, dev):
+    """Six channels with the 64-channel job's lengths (seed 3) in one plan:
+    modes, noise modes and the knot counts of every pass equal the oracle's
+    per channel, bit for bit."""
+    import torch
+    C, I, M, S = 6, 4, 5, 10
+    lens, xs = _c3_channels(C)
+    print(f"case: c3 fp64 lens={lens} I={I} M={M}", flush=True)
+    plan = icm.Plan(lens, I, M, dtype=torch.float64, max_sifts=S, use_graph=False, device=dev)
+    plan.enable_trace(S)
+    out = plan.forward([torch.tensor(x, device=dev) for x in xs])
+    tr = plan.trace(icm.PHASE_SIGNAL)            # [M-1][S][C*I][2]
+    modes = _noise_mode_rows(plan.noise_modes().cpu().numpy(), lens, I, M)
+    plan.close()
+    for c, (n, x) in enumerate(zip(lens, xs)):
+        ref = orc.iceemdan(x, _gpu_noise(icm, dev, I, n), M, S_or_cap=S, trace_passes=S,
+                           want_noise_modes=True)
+        got = out[c].cpu().numpy()
+        assert np.array_equal(tr[:, :, c * I:(c + 1) * I], ref["trace"][1]), c
+        assert np.array_equal(modes[n], ref["noise_modes"]), c
+        assert np.array_equal(got, ref["imfs"]), (c, relerr(got, ref["imfs"]))
+
+
+def test_c3_channels_fp32_vs_oracle(icm, orc, dev):
+    """The same channels in FP32.  The oracle is fed the same float32 input
+    and the plan's float32 noise: the first pass of every noise mode sees
+    identical samples, so its knot counts are equal; the per-mode max
+    relative error is reported and held to what rounding-induced extrema
+    flips allow at this size (measured, with margin)."""
+    import torch
+    C, I, M, S = 6, 4, 5, 10
+    lens, xs = _c3_channels(C)
+    print(f"case: c3 fp32 lens={lens} I={I} M={M}", flush=True)
+    plan = icm.Plan(lens, I, M, dtype=torch.float32, max_sifts=S, use_graph=False, device=dev)
+    plan.enable_trace(1)
+    out = plan.forward([torch.tensor(x, dtype=torch.float32, device=dev) for x in xs])
+    trn = plan.trace(icm.PHASE_NOISE)            # [M-1][1][noise series][2]
+    plan.close()
+    worst = np.zeros(M)
+    off = 0
+    for c, (n, x) in enumerate(zip(lens, xs)):
+        x32 = x.astype(np.float32).astype(np.float64)
+        ref = orc.iceemdan(x32, _gpu_noise(icm, dev, I, n, dtype=torch.float32), M, S_or_cap=S,
+                           trace_passes=1)
+        got = out[c].cpu().numpy().astype(np.float64)
+        assert np.all(np.isfinite(got))
+        assert np.array_equal(trn[0, 0, off:off + I], ref["trace"][0][0, 0]), c
+        off += I
+        errs = [relerr(got[k], ref["imfs"][k]) for k in range(M)]
+        print(f"c3 fp32 channel {c} N={n} per-mode max rel err:",
+              " ".join(f"{e:.2e}" for e in errs), flush=True)
+        worst = np.maximum(worst, errs)
+        # completeness in float32: the rows telescope to x~ up to rounding
+        xt = x32 / np.sqrt(np.var(x32, ddof=1))
+        assert np.max(np.abs(got.sum(0) - xt)) < 2e-5 * np.max(np.abs(xt))
+    print("c3 fp32 worst per-mode max rel err:", " ".join(f"{e:.2e}" for e in worst))
+    assert worst[0] < 5e-3, worst
+
+
+def test_rilling_channels_vs_oracle(icm, orc, dev):
+    """Uneven channels under the RILLING rule in one plan (per-series device
+    masks): every channel's stop counts and modes equal the oracle's."""
+    import torch
+    sig = _sig()
+    lens, I, M, cap, ncap = [3000, 4500, 2000, 4500], 5, 4, 50, 300
+    xs = [sig.eeg_like(n, seed=60 + c) for c, n in enumerate(lens)]
+    print(f"case: rilling channels lens={lens}", flush=True)
+    plan = icm.Plan(lens, I, M, dtype=torch.float64, stop_mode=icm.STOP_RILLING, max_sifts=cap,
+                    noise_max_sifts=ncap, use_graph=False, device=dev)
+    plan.enable_trace(1)
+    out = plan.forward([torch.tensor(x, device=dev) for x in xs])
+    stops = plan.stops(icm.PHASE_SIGNAL)          # [M-1][C*I]
+    plan.close()
+    for c, (n, x) in enumerate(zip(lens, xs)):
+        ref = orc.iceemdan(x, _gpu_noise(icm, dev, I, n), M, stop_mode=orc.STOP_RILLING,
+                           S_or_cap=cap, noise_cap=ncap)
+        assert np.array_equal(stops[:, c * I:(c + 1) * I], ref["stops"][1]), c
+        got = out[c].cpu().numpy()
+        assert np.array_equal(got, ref["imfs"]), (c, relerr(got, ref["imfs"]))
+
+
+def test_graph_replay_matches_oracle(icm, orc, dev):
+    """The captured hipGraph, replayed three times with unrelated eager work
+    in between, reproduces the oracle bit for bit on every replay."""
+    import torch
+    sig = _sig()
+    lens, I, M, S = [5000, 3100], 6, 5, 10
+    xs = [sig.eeg_like(n, seed=80 + c) for c, n in enumerate(lens)]
+    refs = [orc.iceemdan(x, _gpu_noise(icm, dev, I, n), M, S_or_cap=S)["imfs"]
+            for n, x in zip(lens, xs)]
+    print("case: graph replay vs oracle", flush=True)
+    plan = icm.Plan(lens, I, M, dtype=torch.float64, max_sifts=S, use_graph=True, device=dev)
+    xd = [torch.tensor(x, device=dev) for x in xs]
+    for k in range(3):
+        out = plan.forward(xd)
+        for c in range(len(lens)):
+            assert np.array_equal(out[c].cpu().numpy(), refs[c]), (k, c)
+        junk = torch.randn(4096, device=dev).sum()
+        torch.cuda.synchronize()
+        assert torch.isfinite(junk)
+    plan.close()
+
+
+def test_resumed_run_matches_oracle(icm, orc, dev):
+    """A staged forward saved after two stages and resumed in a new plan
+    ends with the oracle's decomposition of every channel."""
+    import torch
+    sig = _sig()
+    lens, I, M, S = [2500, 1800], 5, 6, 10
+    xs = [sig.eeg_like(n, seed=90 + c) for c, n in enumerate(lens)]
+    print("case: resume vs oracle", flush=True)
+    kw = dict(dtype=torch.float64, max_sifts=S, use_graph=False, device=dev)
+    plan = icm.Plan(lens, I, M, **kw)
+    plan.forward_begin([torch.tensor(x, device=dev) for x in xs])
+    plan.forward_stages(2)
+    state = plan.save_state()
+    plan.close()
+    fresh = icm.Plan(lens, I, M, **kw)
+    assert fresh.load_state(state) == 2
+    fresh.forward_stages(100)
+    got = fresh.forward_finish()
+    fresh.close()
+    for c, (n, x) in enumerate(zip(lens, xs)):
+        ref = orc.iceemdan(x, _gpu_noise(icm, dev, I, n), M, S_or_cap=S)["imfs"]
+        assert np.array_equal(got[c].cpu().numpy(), ref), c
+
+
+def test_driver_matches_oracle(orc, icm, dev, tmp_path):
+    """The command-line driver (raw float32 files in, [M][N] float32 modes
+    out) on two channels in float64: each output file is the oracle's
+    decomposition rounded to float32."""
+    exe = os.path.join(ROOT, "ceemdan-gpu_amd", "bin", "iceemdan_file")
+    assert os.path.exists(exe)
+    x = np.fromfile(os.path.join(GOLD, "eegSampleDataCH4.f32"), dtype="<f4")
+    chans = [x[:3000].copy(), x[9000:13100].copy()]
+    ins = []
+    for c, a in enumerate(chans):
+        p = tmp_path / f"in{c}.bin"
+        a.tofile(p)
+        ins.append(str(p))
+    print("case: driver vs oracle", flush=True)
+    I, M, S = 6, 4, 10
+    p = subprocess.run([exe, "--noise", str(I), "--imfs", str(M), "--fixed", str(S), "--double",
+                        str(tmp_path / "m")] + ins, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    for c, a in enumerate(chans):
+        n = a.size
+        got = np.fromfile(tmp_path / f"m.{c}.bin", dtype="<f4").reshape(M, n)
+        ref = orc.iceemdan(a.astype(np.float64), _gpu_noise(icm, dev, I, n), M, S_or_cap=S)["imfs"]
+        assert np.array_equal(got, ref.astype(np.float32)), c
+
+
+# -------------------------------------------------------------------- C4
+
+def _c4_loss_grads(imfs, wk):
+    """dL/dIMF for L = sum_k w_k |IMF_k|^2."""
+    return [2.0 * wk[:, None] * im for im in imfs]
+
+
+def test_c4_adjoint_channels_vs_oracle(icm, orc, dev):
+    """FP64 adjoint on three 16384-sample EEG-like channels (M = 12, 10
+    sifts) in one plan: every channel's gradient of L = sum_k w_k |IMF_k|^2
+    (w_k ~ U(0,1), seed 5) is within 1e-9 of the oracle's."""
+    import torch
+    C, N, I, M, S = 3, 16384, 6, 12, 10
+    sig = _sig()
+    xs = [sig.eeg_like(N, seed=4000 + c) for c in range(C)]
+    wk = np.random.default_rng(5).uniform(0, 1, M)
+    print(f"case: c4 adjoint C={C} N={N} I={I} M={M}", flush=True)
+    plan = icm.Plan([N] * C, I, M, dtype=torch.float64, max_sifts=S, use_graph=False, device=dev)
+    out = plan.forward([torch.tensor(x, device=dev) for x in xs])
+    wkt = torch.tensor(wk, device=dev)
+    g = plan.backward(_c4_loss_grads(out, wkt))
+    plan.close()
+    w = _gpu_noise(icm, dev, I, N)
+    for c, x in enumerate(xs):
+        G = 2.0 * wk[:, None] * out[c].cpu().numpy()
+        ref_g, ref_imfs = orc.iceemdan_grad(x, w, M, G, S=S)
+        assert np.array_equal(out[c].cpu().numpy(), ref_imfs), c
+        e = relerr(g[c].cpu().numpy(), ref_g)
+        print(f"c4 channel {c} gradient rel err vs oracle: {e:.2e}", flush=True)
+        assert e <= 1e-9, (c, e)
+
+
+def test_c4_full_size_central_differences(icm, dev):
+    """32 x 16384, I = 100, M = 12, 10 sifts, FP64: the gradient is finite and
+    matches central differences with h = 1e-7 along unit directions on one
+    channel, for every direction whose two perturbed forwards meet the
+    forward's extrema index sets in every pass of every stage (equal knot-set
+    digests); at least one of the tried directions must qualify."""
+    import torch
+    C, N, I, M, S = 32, 16384, 100, 12, 10
+    sig = _sig()
+    xs = [sig.eeg_like(N, seed=4000 + c) for c in range(C)]
+    wk = torch.tensor(np.random.default_rng(5).uniform(0, 1, M), device=dev)
+    print(f"case: c4 full C={C} N={N} I={I} M={M}", flush=True)
+    plan = icm.Plan([N] * C, I, M, dtype=torch.float64, max_sifts=S, use_graph=False, device=dev)
+    plan.enable_trace(S)
+    xd = [torch.tensor(x, device=dev) for x in xs]
+    out = [o.clone() for o in plan.forward(xd)]
+    base = plan.trace_hash(icm.PHASE_SIGNAL)
+    g = plan.backward(_c4_loss_grads(out, wk))
+    assert all(bool(torch.isfinite(t).all()) for t in g)
+    h = 1e-7
+    rng = np.random.default_rng(6)
+    checked, flipped = 0, []
+    for trial in range(6):
+        c = trial % C
+        v = rng.standard_normal(N)
+        v /= np.linalg.norm(v)
+        vd = torch.tensor(v, device=dev)
+        runs, same = [], True
+        for sgn in (1.0, -1.0):
+            xp = list(xd)
+            xp[c] = xd[c] + sgn * h * vd
+            o = plan.forward(xp)[c].clone()
+            runs.append(o)
+            dh = plan.trace_hash(icm.PHASE_SIGNAL)
+            if not np.array_equal(dh, base):
+                same = False
+                flipped.append((trial, int(np.sum(dh != base))))
+        if not same:
+            continue
+        # L(+) - L(-) summed from per-sample differences: the loss itself is
+        # ~1e4 and would lose the 1e-7 change to rounding
+        a, b = runs
+        dl = float((wk[:, None] * (a - b) * (a + b)).sum())
+        fd = dl / (2 * h)
+        an = float(g[c] @ vd)
+        print(f"c4 fd direction {trial} (channel {c}): fd={fd:.10e} adjoint={an:.10e}", flush=True)
+        assert abs(fd - an) <= 1e-6 * max(abs(fd), abs(an)), (trial, fd, an)
+        checked += 1
+    print(f"c4 directions checked {checked}, with a changed knot set: {flipped}", flush=True)
+    plan.close()
+    assert checked >= 1, flipped
+
+
+# -------------------------------------------------------------------- C5
+
+def test_c5_share_fp32_reduced_I(icm, dev):
+    """One GPU's share of the 128-channel FP32 job (16 x 244032 EEG-like with
+    a 50 Hz line, M = 16, 10 sifts) at I = 40: every row finite, the rows
+    telescope to x~ within float32 rounding, and the adjoint of
+    L = sum_k w_k |IMF_k|^2 is finite and non-zero for every channel."""
+    import torch
+    C, N, I, M, S = 16, 244032, 40, 16, 10
+    sig = _sig()
+    xs = [sig.eeg_like(N, seed=6000 + c, line_uv=5.0) for c in range(C)]
+    print(f"case: c5 share C={C} N={N} I={I} M={M} fp32", flush=True)
+    plan = icm.Plan([N] * C, I, M, dtype=torch.float32, max_sifts=S, use_graph=True, device=dev)
+    out = plan.forward([torch.tensor(x, dtype=torch.float32, device=dev) for x in xs])
+    for c, x in enumerate(xs):
+        o = out[c].double().cpu().numpy()
+        assert np.all(np.isfinite(o)), c
+        x32 = x.astype(np.float32).astype(np.float64)
+        xt = x32 / np.sqrt(np.var(x32, ddof=1))
+        assert np.max(np.abs(o.sum(0) - xt)) < 2e-5 * np.max(np.abs(xt)), c
+    wk = torch.tensor(np.random.default_rng(5).uniform(0, 1, M), dtype=torch.float32, device=dev)
+    g = plan.backward(_c4_loss_grads(out, wk))
+    plan.close()
+    for c in range(C):
+        assert bool(torch.isfinite(g[c]).all()), c
+        assert float(g[c].abs().max()) > 0.0, c
+
+
+def test_batched_backward_matches_oracle(icm, orc, dev):
+    """Uneven channels in one plan: each channel's FP64 gradient is within
+    1e-9 of the oracle's."""
+    import torch
+    sig = _sig()
+    lens, I, M, S = [1500, 2200, 1500], 6, 4, 6
+    xs = [sig.eeg_like(n, seed=20 + c) for c, n in enumerate(lens)]
+    rng = np.random.default_rng(9)
+    Gs = [rng.standard_normal((M, n)) for n in lens]
+    print(f"case: batched backward vs oracle lens={lens}", flush=True)
+    plan = icm.Plan(lens, I, M, dtype=torch.float64, max_sifts=S, use_graph=False, device=dev)
+    plan.forward([torch.tensor(x, device=dev) for x in xs])
+    gb = plan.backward([torch.tensor(G, device=dev) for G in Gs])
+    plan.close()
+    for c, (n, x) in enumerate(zip(lens, xs)):
+        ref, _ = orc.iceemdan_grad(x, _gpu_noise(icm, dev, I, n), M, Gs[c], S=S)
+        e = relerr(gb[c].cpu().numpy(), ref)
+        assert e <= 1e-9, (c, e)
