@@ -251,10 +251,20 @@ def test_c4_adjoint_channels_vs_oracle(icm, orc, dev):
 
 def test_c4_full_size_central_differences(icm, dev):
     """32 x 16384, I = 100, M = 12, 10 sifts, FP64: the gradient is finite and
-    matches central differences with h = 1e-7 along unit directions on one
-    channel, for every direction whose two perturbed forwards meet the
-    forward's extrema index sets in every pass of every stage (equal knot-set
-    digests); at least one of the tried directions must qualify."""
+    matches central differences along unit directions on one channel, for
+    every direction whose two perturbed forwards meet the forward's extrema
+    index sets in every pass of every stage (equal knot-set digests); at
+    least one of the tried directions must qualify at each step.
+
+    The bound is the difference quotient's own rounding noise: each output
+    sample carries ~1e-14 relative rounding from the ~1e3 operations of 11
+    stages x 10 sifts, which changes chaotically with the perturbation, so
+    the quotient of 2e5 output samples has noise ~C / h with C ~ 1e-11.  The
+    FP64 oracle itself, differenced the same way on one 16384-sample channel
+    (I = 6), misses its own exact gradient by 8.7e-5, 6.1e-5, 4.1e-6 at
+    h = 1e-7 and by 1.4e-6, 2.5e-6, 1.6e-6 at h = 1e-5.  So h = 1e-7 is held
+    to 5e-4 and h = 1e-5 to 2e-5; the exact check of the gradient is the
+    comparison with the oracle's gradient (<= 1e-9) above."""
     import torch
     C, N, I, M, S = 32, 16384, 100, 12, 10
     sig = _sig()
@@ -268,10 +278,10 @@ def test_c4_full_size_central_differences(icm, dev):
     base = plan.trace_hash(icm.PHASE_SIGNAL)
     g = plan.backward(_c4_loss_grads(out, wk))
     assert all(bool(torch.isfinite(t).all()) for t in g)
-    h = 1e-7
     rng = np.random.default_rng(6)
-    checked, flipped = 0, []
-    for trial in range(6):
+    checked, flipped = {}, []
+    for trial in range(8):
+        h, tol = (1e-7, 5e-4) if trial % 2 == 0 else (1e-5, 2e-5)
         c = trial % C
         v = rng.standard_normal(N)
         v /= np.linalg.norm(v)
@@ -294,12 +304,14 @@ def test_c4_full_size_central_differences(icm, dev):
         dl = float((wk[:, None] * (a - b) * (a + b)).sum())
         fd = dl / (2 * h)
         an = float(g[c] @ vd)
-        print(f"c4 fd direction {trial} (channel {c}): fd={fd:.10e} adjoint={an:.10e}", flush=True)
-        assert abs(fd - an) <= 1e-6 * max(abs(fd), abs(an)), (trial, fd, an)
-        checked += 1
+        rel = abs(fd - an) / max(abs(fd), abs(an))
+        print(f"c4 fd direction {trial} (channel {c}, h={h:g}): fd={fd:.10e} adjoint={an:.10e} "
+              f"rel {rel:.2e}", flush=True)
+        assert rel <= tol, (trial, h, fd, an)
+        checked[h] = checked.get(h, 0) + 1
     print(f"c4 directions checked {checked}, with a changed knot set: {flipped}", flush=True)
     plan.close()
-    assert checked >= 1, flipped
+    assert checked.get(1e-7, 0) >= 1 and checked.get(1e-5, 0) >= 1, (checked, flipped)
 
 
 # -------------------------------------------------------------------- C5
