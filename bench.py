@@ -1,24 +1,41 @@
 """Benchmark of the ICEEMDAN hot path on one MI355X.
 
-Default workload (one EEG-like channel, the plain per-signal FP64 case):
-N = 244032 samples at 500 Hz (1/f pink noise at 30 uV RMS + 10 Hz alpha
-bursts, seed 2), I = 500 noise realizations, eps0 = 0.2, M = 16 rows
-(15 sifted stages), exactly 10 sifts per stage, float64.
+Default workload (the batched headline): 64 EEG-like channels with lengths
+drawn uniformly from 8192..262144 samples (seed 3), each 1/f pink noise at
+30 uV RMS plus 10 Hz alpha bursts (seeds 1000 + c), I = 200 noise
+realizations per channel, eps0 = 0.2, M = 14 rows (13 sifted stages),
+exactly 10 sifts per stage, float32, all channels and realizations in one
+plan.
 
 A "step" is one full forward: white-noise generation, the noise-mode stages
-E_1..E_15 (also 10 sifts each) and the 15 CEEMDAN stages, with the input
-already resident in HBM.  value = CEEMDAN sample-sifts of the step
-(sum over stages and realizations of N * 10 = 1.83e10) / step wall time.
+and the CEEMDAN stages, with the inputs already resident in HBM.
+value = CEEMDAN sample-sifts of the step (sum over channels, stages and
+realizations of N_c * 10) / step wall time.
 
-Prints one JSON line (rank 0).  `roofline` is the sift-pass kernel measured
-with HIP events on its stream over the timed steps; `cpu_baseline` is the
-FP64 OpenMP oracle on a bounded sample of the same workload (a subset of the
-realizations) on this host's cores.
+Also in the one JSON line (rank 0):
+  batched_vs_per_signal  the same 64 channels, one plan per channel run back
+                         to back (the plain per-signal path), same settings
+  roofline               the sift-pass kernel, HIP events on its stream over
+                         the timed steps; traffic from the committed
+                         rocprofv3 FETCH_SIZE / WRITE_SIZE summary (its file
+                         is named)
+  fp32_vs_fp64           per-mode error of channel 0 against the FP64
+                         decomposition (which equals the FP64 oracle bit for
+                         bit)
+  c2                     one EEG-like channel N = 244032, I = 500, M = 16,
+                         float64 (the per-signal FP64 case)
+  c4                     forward + adjoint, 32 x 16384, I = 100, M = 12,
+                         float64: backward / forward time
+  c5_share               (--c5) one GPU's share of the 128-channel FP32 job,
+                         forward + adjoint
+  cpu_baseline           the FP64 OpenMP oracle on a bounded sample of the
+                         workload, best of 3, on this host's cores
 """
 import argparse
 import importlib
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -29,25 +46,23 @@ METRIC = "ensemble sample-sifts/s batched vs per-signal (1 GPU); backward/forwar
 PEAK_HBM_GBS = 8000.0
 
 WORKLOADS = {
-    # name: (channels lengths builder, I, M, S, dtype, description)
+    "c3": dict(I=200, M=14, S=10, dtype="f32",
+               desc="64 EEG-like channels, N ~ U[8192, 262144] (seed 3), I=200, eps0=0.2, M=14, "
+                    "10 sifts per stage, float32, one plan"),
     "c2": dict(I=500, M=16, S=10, dtype="f64",
                desc="one EEG-like channel N=244032 fs=500 Hz (pink 30 uV + 10 Hz alpha, seed 2), "
                     "I=500, eps0=0.2, M=16, 10 sifts per stage, float64"),
     "c1": dict(I=50, M=10, S=10, dtype="f64",
                desc="three-tone N=4096 fs=1 kHz + N(0,0.1^2) (seed 0), I=50, M=10, 10 sifts, float64"),
-    "c3": dict(I=200, M=14, S=10, dtype="f32",
-               desc="64 EEG-like channels, N ~ U[8192, 262144] (seed 3), I=200, M=14, 10 sifts, float32"),
     "c4": dict(I=100, M=12, S=10, dtype="f64",
                desc="32 EEG-like channels x N=16384 (seed 4), I=100, M=12, 10 sifts, float64; "
                     "forward + adjoint of L = sum_k w_k |IMF_k|^2, w_k ~ U(0,1) (seed 5)"),
-    # the 128-channel job shards channels over 8 GPUs with no communication;
-    # one GPU's share is 16 channels (all 128 in one plan exceed the 2^32
-    # work-items of one launch).  Per-pass knot records would take 1.26 TB,
-    # so the backward re-sifts each stage.
+    # the 128-channel job shards channels over GPUs with no communication;
+    # one GPU's share of an 8-way split is 16 channels
     "c5": dict(I=500, M=16, S=10, dtype="f32", records=True,
                desc="one GPU's share of the 128-channel job: channels 0-15 of 128 EEG-like x "
                     "N=244032 (+50 Hz 5 uV, seeds 6000+c), I=500, M=16, 10 sifts, float32; "
-                    "forward + adjoint of L = sum_k w_k |IMF_k|^2 (backward re-sifts)"),
+                    "forward + adjoint of L = sum_k w_k |IMF_k|^2"),
 }
 
 
@@ -66,10 +81,14 @@ def signals_for(name, sig):
     raise ValueError(name)
 
 
+def traffic_file(workload, dtype):
+    return os.path.join("profiles", f"k_sift_{workload}_fp{64 if dtype == 'f64' else 32}_traffic.md")
+
+
 def measured_traffic(workload, dtype):
-    """HBM bytes per k_sift launch from the committed rocprofv3 FETCH_SIZE /
+    """HBM bytes per sift launch from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this workload (None when it was not profiled)."""
-    path = os.path.join(ROOT, "profiles", f"k_sift_{workload}_fp{64 if dtype == 'f64' else 32}_traffic.md")
+    path = os.path.join(ROOT, traffic_file(workload, dtype))
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -78,46 +97,238 @@ def measured_traffic(workload, dtype):
     return None
 
 
-def batched_vs_per_signal(icm, sig, torch, dev, channels=8, N=16384, I=100, M=12, S=10):
-    """Sample-sifts/s of one plan over `channels` channels against one plan
-    per channel run back to back (same FIXED settings, float64)."""
-    xs = [torch.tensor(sig.eeg_like(N, seed=4000 + c), device=dev) for c in range(channels)]
-    sifts = (M - 1) * I * N * S * channels
-    plan = icm.Plan([N] * channels, I, M, dtype=torch.float64, max_sifts=S, use_graph=True,
-                    device=dev)
-    plan.forward(xs)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    plan.forward(xs)
-    torch.cuda.synchronize()
-    tb = time.perf_counter() - t0
-    plan.close()
-    single = icm.Plan([N], I, M, dtype=torch.float64, max_sifts=S, use_graph=True, device=dev)
-    single.forward([xs[0]])
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for x in xs:
-        single.forward([x])
-    torch.cuda.synchronize()
-    ts = time.perf_counter() - t0
-    single.close()
-    return {"workload": f"{channels} EEG-like channels x N={N}, I={I}, M={M}, {S} sifts, float64",
-            "batched": sifts / tb, "per_signal": sifts / ts, "ratio": ts / tb}
+def _torch_dtype(torch, code):
+    return torch.float64 if code == "f64" else torch.float32
 
 
-def cpu_baseline(x, I_full, M, S, sample_I, threads):
-    """FP64 oracle (OpenMP over realizations) on sample_I of the realizations."""
+def tune_forwards(plan, xd, out):
+    """The plan's autotuning forwards (untimed, like plan creation): the sift
+    candidates are timed in rotation over the passes of these forwards.  A
+    plan whose shape this process has tuned already starts tuned."""
+    t0 = time.perf_counter()
+    n = 0
+    while plan.tuning_left() > 0 and n < 8:
+        plan.forward(xd, out=out)
+        n += 1
+    return {"forwards": n, "ms": 1e3 * (time.perf_counter() - t0)}
+
+
+def tuning_of(icm, plan, tune_ms):
+    """The autotuner's choice per stage: (tiles per segment, small tables)."""
+    desc = {}
+    for ph, name in ((icm.PHASE_NOISE, "noise_modes"), (icm.PHASE_SIGNAL, "ceemdan")):
+        desc[name] = [list(icm.tuning_candidate(int(c))) for c in plan.tuning(ph)]
+    desc["tuning_forwards"] = tune_ms
+    return desc
+
+
+def forward_workload(icm, torch, dev, name, xs, steps, warmup, graph=False):
+    """Times `steps` forwards of one plan over all channels (after `warmup`
+    untimed ones); the sift kernel's launches are timed with HIP events."""
     import numpy as np
+    W = WORKLOADS[name]
+    dtype = _torch_dtype(torch, W["dtype"])
+    lengths = [x.size for x in xs]
+    I, M, S = W["I"], W["M"], W["S"]
+    plan = icm.Plan(lengths, I, M, dtype=dtype, max_sifts=S, noise_strength=0.2, seed=1234,
+                    use_graph=graph, device=dev)
+    xd = [torch.tensor(x, dtype=dtype, device=dev) for x in xs]
+    out = [torch.empty((M, n), dtype=dtype, device=dev) for n in lengths]
+    tune_ms = tune_forwards(plan, xd, out)
+    plan.set_profiling(not graph)
+    for _ in range(warmup):
+        plan.forward(xd, out=out)
+    torch.cuda.synchronize()
+    sift_ms = sift_n = sift_bytes = 0.0
+    totals = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.forward(xd, out=out)
+        totals.append(plan.timing())
+        if not graph:
+            ks = plan.kernel_stats()
+            sift_ms += ks["sift"][0]
+            sift_n += ks["sift"][1]
+            sift_bytes += ks["sift"][2]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    sample_sifts = (M - 1) * I * sum(lengths) * S
+    res = {
+        "plan": plan, "xd": xd, "out": out, "tuning": tuning_of(icm, plan, tune_ms),
+        "value": sample_sifts / (wall / steps),
+        "ms_per_step": 1e3 * wall / steps,
+        "sample_sifts_per_step": sample_sifts,
+        "phases_ms": {"noise_modes": float(np.mean([t["noise_ms"] for t in totals])),
+                      "ceemdan": float(np.mean([t["signal_ms"] for t in totals])),
+                      "total": float(np.mean([t["total_ms"] for t in totals]))},
+        "output_finite": all(bool(torch.isfinite(o).all()) for o in out),
+    }
+    res["ceemdan_phase_rate"] = sample_sifts / (1e-3 * res["phases_ms"]["ceemdan"])
+    if not graph and sift_n:
+        avg_ms = sift_ms / sift_n
+        bpl = sift_bytes / sift_n
+        achieved = bpl / (avg_ms * 1e-3) / 1e9
+        res["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
+                           "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                           "traffic": measured_traffic(name, W["dtype"]),
+                           "traffic_source": traffic_file(name, W["dtype"])
+                           if measured_traffic(name, W["dtype"]) else None,
+                           "kernel": "k_sift_w (+ k_sift_list on small-table passes)",
+                           "avg_launch_ms": avg_ms, "alg_bytes_per_launch": bpl,
+                           "launches": int(sift_n)}
+    return res
+
+
+def graph_replay_ms(icm, torch, dev, name, xs, steps):
+    """ms per forward of the hipGraph replay (no per-kernel events)."""
+    W = WORKLOADS[name]
+    dtype = _torch_dtype(torch, W["dtype"])
+    lengths = [x.size for x in xs]
+    plan = icm.Plan(lengths, W["I"], W["M"], dtype=dtype, max_sifts=W["S"], use_graph=True,
+                    device=dev)
+    xd = [torch.tensor(x, dtype=dtype, device=dev) for x in xs]
+    out = [torch.empty((W["M"], n), dtype=dtype, device=dev) for n in lengths]
+    tune_forwards(plan, xd, out)
+    plan.forward(xd, out=out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.forward(xd, out=out)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    plan.close()
+    return ms
+
+
+def per_signal(icm, torch, dev, name, xs, batched_value):
+    """The plain per-signal path on the same channels: one plan per channel
+    (hipGraph replay, same settings, autotuned like the batched plan), each
+    timed on the forward after the one that captures its graph, run back to
+    back.  Plan creation and tuning are outside the timed region, as the
+    batched plan's are."""
+    W = WORKLOADS[name]
+    dtype = _torch_dtype(torch, W["dtype"])
+    I, M, S = W["I"], W["M"], W["S"]
+    total = 0.0
+    for x in xs:
+        n = x.size
+        p = icm.Plan([n], I, M, dtype=dtype, max_sifts=S, use_graph=True, device=dev)
+        xd = [torch.tensor(x, dtype=dtype, device=dev)]
+        out = [torch.empty((M, n), dtype=dtype, device=dev)]
+        tune_forwards(p, xd, out)
+        p.forward(xd, out=out)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        p.forward(xd, out=out)
+        torch.cuda.synchronize()
+        total += time.perf_counter() - t0
+        p.close()
+    sifts = (M - 1) * I * sum(x.size for x in xs) * S
+    ps = sifts / total
+    return {"workload": f"the same {len(xs)} channels, one plan per channel, hipGraph replay",
+            "batched": batched_value, "per_signal": ps, "ratio": batched_value / ps,
+            "per_signal_ms": 1e3 * total}
+
+
+def fp32_error(icm, torch, dev, name, xs, out0):
+    """Per-mode error of the FP32 decomposition of channel 0 against the
+    FP64 decomposition of the same float32 input and the same float32 noise
+    (the FP64 path equals the FP64 oracle bit for bit), same settings."""
+    W = WORKLOADS[name]
+    I, M, S = W["I"], W["M"], W["S"]
+    n0 = xs[0].size
+    p64 = icm.Plan([n0], I, M, dtype=torch.float64, max_sifts=S, use_graph=False, device=dev)
+    x32 = torch.tensor(xs[0], dtype=torch.float32, device=dev)
+    w32 = icm.noise(I, n0, seed=1234, dtype=torch.float32, device=dev)
+    ref = p64.forward([x32.to(torch.float64)], noise=w32.to(torch.float64))[0]
+    p64.close()
+    got = out0.to(torch.float64)
+    # null where the FP64 row is zero (a stage with too few extrema
+    # subtracts nothing, and its residue row is exactly 0)
+    rel = [float((got[k] - ref[k]).abs().max() / ref[k].abs().max())
+           if float(ref[k].abs().max()) > 0 else None for k in range(M)]
+    rms = [float((got[k] - ref[k]).norm() / ref[k].norm())
+           if float(ref[k].norm()) > 0 else None for k in range(M)]
+    return {"channel": 0, "samples": n0, "reference": "FP64 decomposition of the float32 input",
+            "per_mode_max_rel": rel, "per_mode_rms_rel": rms}
+
+
+def backward_workload(icm, torch, dev, name, xs, steps):
+    """Forward + adjoint of L = sum_k w_k |IMF_k|^2 (w_k ~ U(0,1), seed 5)
+    with a plan that keeps every pass's knot records."""
+    import numpy as np
+    W = WORKLOADS[name]
+    dtype = _torch_dtype(torch, W["dtype"])
+    lengths = [x.size for x in xs]
+    I, M, S = W["I"], W["M"], W["S"]
+    records = W.get("records", True)
+    plan = icm.Plan(lengths, I, M, dtype=dtype, max_sifts=S, noise_strength=0.2, seed=1234,
+                    use_graph=False, device=dev, keep_records=records)
+    xd = [torch.tensor(x, dtype=dtype, device=dev) for x in xs]
+    out = [torch.empty((M, n), dtype=dtype, device=dev) for n in lengths]
+    wk = torch.tensor(np.random.default_rng(5).uniform(0, 1, M), dtype=dtype, device=dev)
+    tune_forwards(plan, xd, out)
+    plan.forward(xd, out=out)
+    grads = [2 * wk[:, None] * o for o in out]
+    gx = plan.backward(grads)
+    torch.cuda.synchronize()
+    fwd = bwd = 0.0
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        plan.forward(xd, out=out)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        gx = plan.backward(grads)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        fwd += t2 - t1
+        bwd += t3 - t2
+    plan.close()
+    sifts = (M - 1) * I * sum(lengths) * S
+    fwd_ms, bwd_ms = 1e3 * fwd / steps, 1e3 * bwd / steps
+    return {"workload": W["desc"], "forward_ms": fwd_ms, "backward_ms": bwd_ms,
+            "backward_forward_ratio": bwd_ms / fwd_ms, "keep_records": records,
+            "forward_sample_sifts_per_s": sifts / (1e-3 * fwd_ms),
+            "grad_finite": all(bool(torch.isfinite(g).all()) for g in gx)}
+
+
+def host_cpu():
+    """(physical cores, model name) of this host from lscpu."""
+    cores, model = os.cpu_count() or 1, "unknown"
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True).stdout
+        pairs = {l for l in out.splitlines() if l and not l.startswith("#")}
+        if pairs:
+            cores = len(pairs)
+        for l in subprocess.run(["lscpu"], capture_output=True, text=True).stdout.splitlines():
+            if l.startswith("Model name:"):
+                model = l.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return cores, model
+
+
+def cpu_baseline(x, I_full, M, S, sample_I, what):
+    """FP64 oracle (OpenMP over realizations) on sample_I of the
+    realizations of one channel, best of 3 after one warm-up run."""
     from oracle import oracle as orc
 
+    phys, model = host_cpu()
+    threads = min(int(os.environ.get("OMP_NUM_THREADS", phys)), phys)
     w = orc.noise(1234, sample_I, x.size)
-    t0 = time.perf_counter()
-    orc.iceemdan(x, w, M, S_or_cap=S, nthreads=threads)
-    dt = time.perf_counter() - t0
+    orc.iceemdan(x[: min(x.size, 4096)], w[:, : min(x.size, 4096)], M, S_or_cap=S,
+                 nthreads=threads)
+    best = float("inf")
+    for _ in range(3):
+        t0 = time.perf_counter()
+        orc.iceemdan(x, w, M, S_or_cap=S, nthreads=threads)
+        best = min(best, time.perf_counter() - t0)
     sifts = (M - 1) * sample_I * x.size * S
-    return {"value": sifts / dt, "unit": "sample-sifts/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle, one channel N={x.size}, {sample_I} of {I_full} realizations, "
-                      f"M={M}, {S} sifts (noise-mode + CEEMDAN stages), {dt:.1f} s"}
+    return {"value": sifts / best, "unit": "sample-sifts/s", "cores": threads, "kind": "port",
+            "physical_cores": phys, "cpu_model": model,
+            "sample": f"FP64 oracle, {what} N={x.size}, {sample_I} of {I_full} realizations, "
+                      f"M={M}, {S} sifts (noise-mode + CEEMDAN stages, CEEMDAN sample-sifts "
+                      f"counted), best of 3: {best:.2f} s"}
 
 
 def main():
@@ -125,13 +336,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-sample", type=int, default=24, help="realizations timed on the CPU")
-    ap.add_argument("--graph", action="store_true", help="time the hipGraph replay (no kernel events)")
-    ap.add_argument("--no-backward", action="store_true", help="skip the adjoint timing")
-    ap.add_argument("--no-batched", action="store_true",
+    ap.add_argument("--cpu-sample", type=int, default=48, help="realizations timed on the CPU")
+    ap.add_argument("--no-per-signal", action="store_true",
                     help="skip the batched vs per-signal comparison")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the C2 / C4 sub-measurements and the graph replay")
+    ap.add_argument("--c5", action="store_true", help="also time the C5 share forward + adjoint")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -144,131 +356,64 @@ def main():
 
     icm = importlib.import_module("ceemdan-gpu_amd")
     sig = importlib.import_module("ceemdan-gpu_amd.signals")
-    W = WORKLOADS[args.workload]
-    dtype = torch.float64 if W["dtype"] == "f64" else torch.float32
     dev = torch.device("cuda", 0)
-    xs = signals_for(args.workload, sig)
+    name = args.workload
+    W = WORKLOADS[name]
+    xs = signals_for(name, sig)
     lengths = [x.size for x in xs]
-    I, M, S = W["I"], W["M"], W["S"]
-    plan = icm.Plan(lengths, I, M, dtype=dtype, max_sifts=S, noise_strength=0.2, seed=1234,
-                    use_graph=args.graph, device=dev)
-    plan.set_profiling(not args.graph)
-    xd = [torch.tensor(x, dtype=dtype, device=dev) for x in xs]
-    out = [torch.empty((M, n), dtype=dtype, device=dev) for n in lengths]
 
-    for _ in range(args.warmup):
-        plan.forward(xd, out=out)
-    torch.cuda.synchronize()
-
-    sift_ms = sift_n = sift_bytes = 0.0
-    totals = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        plan.forward(xd, out=out)
-        tm = plan.timing()
-        totals.append(tm)
-        if not args.graph:
-            ks = plan.kernel_stats()
-            sift_ms += ks["sift"][0]
-            sift_n += ks["sift"][1]
-            sift_bytes += ks["sift"][2]
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-
-    sample_sifts = (M - 1) * I * sum(lengths) * S
-    ms_per_step = 1e3 * wall / args.steps
-    value = sample_sifts / (wall / args.steps)
-    finite = all(bool(torch.isfinite(o).all()) for o in out)
-
+    main_res = forward_workload(icm, torch, dev, name, xs, args.steps, args.warmup)
+    main_res["plan"].close()
     line = {
         "metric": METRIC,
-        "value": value,
+        "value": main_res["value"],
         "unit": "sample-sifts/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": main_res["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": W["dtype"],
         "data": "synthetic",
         "config": {"workload": W["desc"], "channels": len(lengths), "samples": sum(lengths),
-                   "realizations": I, "rows": M, "sifts_per_stage": S, "eps0": 0.2,
-                   "sample_sifts_per_step": sample_sifts},
-        "phases_ms": {"noise_modes": float(np.mean([t["noise_ms"] for t in totals])),
-                      "ceemdan": float(np.mean([t["signal_ms"] for t in totals])),
-                      "total": float(np.mean([t["total_ms"] for t in totals]))},
-        "ceemdan_phase_rate": sample_sifts / (1e-3 * float(np.mean([t["signal_ms"] for t in totals]))),
-        "output_finite": finite,
+                   "realizations": W["I"], "rows": W["M"], "sifts_per_stage": W["S"],
+                   "eps0": 0.2, "sample_sifts_per_step": main_res["sample_sifts_per_step"]},
+        "phases_ms": main_res["phases_ms"],
+        "ceemdan_phase_rate": main_res["ceemdan_phase_rate"],
+        "output_finite": main_res["output_finite"],
     }
-    if not args.graph and sift_n:
-        avg_ms = sift_ms / sift_n
-        bytes_per_launch = sift_bytes / sift_n
-        achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-        line["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
-                            "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                            "traffic": measured_traffic(args.workload, W["dtype"]),
-                            "kernel": "k_sift", "avg_launch_ms": avg_ms,
-                            "alg_bytes_per_launch": bytes_per_launch,
-                            "launches": int(sift_n)}
-    if not args.no_backward:
-        # adjoint of L = sum_k w_k |IMF_k|^2, w_k ~ U(0,1) (seed 5), with a plan
-        # whose forward keeps every pass's knot structure (keep_records), so
-        # the backward does not re-sift (unless the records do not fit); its
-        # forward is timed as well
-        records = W.get("records", True)
-        plan.close()
-        plan = icm.Plan(lengths, I, M, dtype=dtype, max_sifts=S, noise_strength=0.2, seed=1234,
-                        use_graph=False, device=dev, keep_records=records)
-        wk = torch.tensor(np.random.default_rng(5).uniform(0, 1, M), dtype=dtype, device=dev)
-        plan.forward(xd, out=out)
-        grads = [2 * wk[:, None] * o for o in out]
-        gx = plan.backward(grads)
-        torch.cuda.synchronize()
-        fwd_r = bwd = 0.0
-        for _ in range(args.steps):
-            t1 = time.perf_counter()
-            plan.forward(xd, out=out)
-            torch.cuda.synchronize()
-            t2 = time.perf_counter()
-            gx = plan.backward(grads)
-            torch.cuda.synchronize()
-            t3 = time.perf_counter()
-            fwd_r += t2 - t1
-            bwd += t3 - t2
-        fwd_ms = 1e3 * fwd_r / args.steps
-        bwd_ms = 1e3 * bwd / args.steps
-        line["backward"] = {"ms_per_call": bwd_ms, "forward_ms": fwd_ms, "keep_records": records,
-                            "backward_forward_ratio": bwd_ms / fwd_ms,
-                            "backward_vs_plain_forward": bwd_ms / ms_per_step,
-                            "loss": "sum_k w_k |IMF_k|^2, w_k ~ U(0,1) seed 5",
-                            "grad_finite": all(bool(torch.isfinite(g).all()) for g in gx)}
+    if "roofline" in main_res:
+        line["roofline"] = main_res["roofline"]
+    line["autotune"] = main_res["tuning"]
     if W["dtype"] == "f32":
-        # per-mode error of the FP32 decomposition against FP64 (channel 0,
-        # same settings; the FP64 GPU path equals the FP64 oracle bit for bit)
-        n0 = lengths[0]
-        p64 = icm.Plan([n0], I, M, dtype=torch.float64, max_sifts=S, noise_strength=0.2, seed=1234,
-                       use_graph=False, device=dev)
-        ref = p64.forward([torch.tensor(xs[0], dtype=torch.float64, device=dev)])[0]
-        p64.close()
-        got = out[0].to(torch.float64)
-        # (null where the FP64 row is zero: a stage with too few extrema
-        # subtracts nothing, and its residue row is exactly 0)
-        rel = [float((got[k] - ref[k]).abs().max() / ref[k].abs().max())
-               if float(ref[k].abs().max()) > 0 else None for k in range(M)]
-        rms = [float((got[k] - ref[k]).norm() / ref[k].norm())
-               if float(ref[k].norm()) > 0 else None for k in range(M)]
-        line["fp32_vs_fp64"] = {"channel": 0, "samples": n0, "per_mode_max_rel": rel,
-                                "per_mode_rms_rel": rms}
-    if not args.no_batched:
-        line["batched_vs_per_signal"] = batched_vs_per_signal(icm, sig, torch, dev)
+        line["fp32_vs_fp64"] = fp32_error(icm, torch, dev, name, xs, main_res["out"][0])
+    del main_res
+    torch.cuda.empty_cache()
+    if not args.no_extra:
+        line["graph_replay_ms_per_step"] = graph_replay_ms(icm, torch, dev, name, xs, 2)
+    if not args.no_per_signal and len(xs) > 1:
+        line["batched_vs_per_signal"] = per_signal(icm, torch, dev, name, xs, line["value"])
+    if not args.no_extra:
+        if name != "c2":
+            c2 = forward_workload(icm, torch, dev, "c2", signals_for("c2", sig), 3, 1)
+            c2["plan"].close()
+            line["c2"] = {"workload": WORKLOADS["c2"]["desc"], "value": c2["value"],
+                          "ms_per_step": c2["ms_per_step"],
+                          "roofline_frac": c2.get("roofline", {}).get("frac"),
+                          "sift_avg_launch_ms": c2.get("roofline", {}).get("avg_launch_ms")}
+            del c2
+            torch.cuda.empty_cache()
+        line["c4"] = backward_workload(icm, torch, dev, "c4", signals_for("c4", sig), 3)
+        torch.cuda.empty_cache()
+    if args.c5:
+        line["c5_share"] = backward_workload(icm, torch, dev, "c5", signals_for("c5", sig), 1)
+        torch.cuda.empty_cache()
     if not args.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        sample_I = max(1, min(args.cpu_sample, I))
-        line["cpu_baseline"] = cpu_baseline(np.asarray(xs[0], np.float64), I, M, S, sample_I,
-                                            threads)
-    plan.close()
+        sample_I = max(1, min(args.cpu_sample, W["I"]))
+        line["cpu_baseline"] = cpu_baseline(np.asarray(xs[0], np.float64), W["I"], W["M"], W["S"],
+                                            sample_I, f"channel 0 of the {name} workload")
     print(json.dumps(line))
 
 

@@ -72,6 +72,12 @@ _SIGS = {
     "icm_forward": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                               C.c_void_p, C.c_void_p]),
     "icm_last_timing": (C.c_int, [C.c_void_p, C.POINTER(icm_timing)]),
+    "icm_set_autotune": (C.c_int, [C.c_void_p, C.c_int]),
+    "icm_get_tuning": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int32), C.c_int64]),
+    "icm_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int32), C.c_int64]),
+    "icm_tuning_candidate": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "icm_get_tuning_left": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "icm_get_graph_status": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "icm_set_trace": (C.c_int, [C.c_void_p, C.c_int]),
     "icm_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "icm_get_kernel_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double),
@@ -408,6 +414,39 @@ class Plan:
         _check(lib().icm_backward(self._h, gp, op, _stream_ptr(gs[0].device)), "icm_backward")
         return out
 
+    # ---- autotuner of the sift pass
+    def set_autotune(self, forwards):
+        """Time the sift candidates over the next `forwards` forwards (0: off,
+        default geometry)."""
+        _check(lib().icm_set_autotune(self._h, int(forwards)), "icm_set_autotune")
+
+    def graph_status(self):
+        """1: the phases replay captured hipGraphs; 0: eager; -1: a capture
+        failed (the plan runs eagerly)."""
+        n = C.c_int()
+        _check(lib().icm_get_graph_status(self._h, C.byref(n)), "icm_get_graph_status")
+        return n.value
+
+    def tuning_left(self):
+        """Forwards that will still time the sift candidates."""
+        n = C.c_int()
+        _check(lib().icm_get_tuning_left(self._h, C.byref(n)), "icm_get_tuning_left")
+        return n.value
+
+    def tuning(self, phase):
+        """Chosen candidate per stage of a phase (-1 = default geometry)."""
+        import numpy as np
+        out = np.zeros(self.num_imfs - 1, np.int32)
+        _check(lib().icm_get_tuning(self._h, phase, out.ctypes.data_as(C.POINTER(C.c_int32)),
+                                    out.size), "icm_get_tuning")
+        return out
+
+    def set_tuning(self, phase, cands):
+        import numpy as np
+        a = np.ascontiguousarray(cands, dtype=np.int32)
+        _check(lib().icm_set_tuning(self._h, phase, a.ctypes.data_as(C.POINTER(C.c_int32)),
+                                    a.size), "icm_set_tuning")
+
     def set_profiling(self, on):
         _check(lib().icm_set_profiling(self._h, 1 if on else 0), "icm_set_profiling")
 
@@ -434,6 +473,17 @@ class Plan:
         _check(lib().icm_get_noise_modes(self._h, C.c_void_p(out.data_ptr()), n,
                                          _stream_ptr(self.device)), "icm_get_noise_modes")
         return out
+
+
+def tuning_candidate(cand):
+    """(tiles per segment or 0 = default, small tables 1 / full 0 / default -1)."""
+    seg, small = C.c_int(), C.c_int()
+    _check(lib().icm_tuning_candidate(int(cand), C.byref(seg), C.byref(small)),
+           "icm_tuning_candidate")
+    return seg.value, small.value
+
+
+NUM_TUNING_CANDIDATES = 8
 
 
 def noise(num_noise, length, seed=1234, dtype=None, device=0):

@@ -69,6 +69,11 @@ struct PassCtl {
     int H;                 // halo rows on each side of a window
     int* err;              // device error word
     unsigned* dense;       // [epochs] tiles whose knots overflow the small sift table
+    // RILLING: series of the phase still sifting after the previous pass
+    // (null in FIXED mode); 0 makes the pass's kernels return at once, so a
+    // fixed number of passes can be launched (or captured) without asking
+    // the host whether any series is left
+    const unsigned* run;
 };
 
 enum InitMode : int {
@@ -77,6 +82,7 @@ enum InitMode : int {
                        //  (the next noise residual, formed where it is first read)
     INIT_STAGE0 = 1,   // h0 = xt[c][p] + (eps0 * E[p]) * inv[s]  (cudaICEEMDAN.cu:68-78)
     INIT_STAGEK = 2,   // h0 = r[c][p] + (eps0 * E[p]) * sq[c]    (cudaICEEMDAN.cu:80-90)
+    INIT_KNOTS = 4,    // knots of src only (no samples written: the pass reads src itself)
 };
 
 template <typename real>
@@ -115,6 +121,9 @@ struct SiftArgs {
     // them (*dense, counted by the knot prefix); else full tables throughout
     const unsigned* dense;
     unsigned dense_max;
+    // work queue of the pass (zero before the launch): a resident grid of
+    // wavefronts takes segments from it one at a time until none is left
+    unsigned* queue;
 };
 
 template <typename real>
@@ -129,6 +138,7 @@ struct SolveArgs {
     int* err;
     int* fixf;              // optional [nseries][2][2]: endpoint fix applied (for the adjoint)
     int nak;                // not-a-knot spline ends
+    const unsigned* run;    // RILLING: as PassCtl::run
 };
 
 // ---------------------------------------------------------------- adjoint
@@ -204,7 +214,8 @@ hipError_t launch_knot_prefix(const SeriesDev& sd, const KnotsDev<real>& k, cons
 template <typename real>
 hipError_t launch_rilling_decide(const SeriesDev& sd, const int* nk, const int* sx_cnt,
                                  const int* sx_mark, int* active, int* stop_pass, int* subs,
-                                 int pass, int cap, unsigned* n_active, hipStream_t st);
+                                 int pass, int cap, const unsigned* run, unsigned* n_active,
+                                 hipStream_t st);
 template <typename real>
 hipError_t launch_series_var(const real* base, const int64_t* off, int64_t stride,
                              const int* len, int nser, real* var_out, hipStream_t st);

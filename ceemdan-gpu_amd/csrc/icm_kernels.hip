@@ -27,7 +27,10 @@
 #include "icm_device.h"
 #include "icm_kernels.h"
 
+#include <map>
+#include <mutex>
 #include <type_traits>
+#include <utility>
 #include "icm_errors.h"
 
 namespace icm {
@@ -160,6 +163,7 @@ __global__ __launch_bounds__(kPT) void k_knot_prefix_b(SeriesDev sd, KnotsDev<re
     __shared__ int s_min[NWV];
     __shared__ unsigned s_wbase;
     const int s = blockIdx.x;
+    if (pc.run && load_scalar(pc.run) == 0u) return;
     if (active && !active[s]) return;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int tb = sd.tile0[s], nt = sd.ntl[s];
@@ -319,6 +323,7 @@ __global__ __launch_bounds__(kPT) void k_knot_prefix_1(SeriesDev sd, KnotsDev<re
     __shared__ unsigned s_wbase;
     __shared__ unsigned s_dense[NWV];
     const int s = blockIdx.x;
+    if (pc.run && load_scalar(pc.run) == 0u) return;
     if (active && !active[s]) return;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int tb = sd.tile0[s], nt = sd.ntl[s];
@@ -778,6 +783,7 @@ __global__ __launch_bounds__(kThreads) void k_solve(SolveArgs<real> a)
     real* sy = s_y[wv];
     // window items and series metadata through the scalar cache (written
     // by earlier launches): one round trip for the item, one for its series
+    if (a.run && load_scalar(a.run) == 0u) return;
     const unsigned total = load_scalar(a.wincount);
     for (unsigned item = blockIdx.x * NW + wv; item < total; item += gridDim.x * NW) {
         const int s = load_scalar(a.winlist + kWinItem * item);
@@ -890,10 +896,11 @@ __global__ __launch_bounds__(kThreads) void k_solve(SolveArgs<real> a)
 template <typename real>
 __global__ void k_rilling_decide(SeriesDev sd, const int* nk, const int* sx_cnt,
                                  const int* sx_mark, int* active, int* stop_pass, int* subs,
-                                 int pass, int cap, unsigned* n_active)
+                                 int pass, int cap, const unsigned* run, unsigned* n_active)
 {
     const int s = blockIdx.x;
     const int lane = threadIdx.x;
+    if (load_scalar(run) == 0u) return;
     if (!active[s]) return;
     long long c = 0;
     int mk = 0;
@@ -1001,10 +1008,6 @@ __device__ __forceinline__ void sift_segment(const SiftArgs<real>& a, const Segm
 {
     const SeriesDev& sd = a.sd;
     const Segment seg = segs[sid];
-    // a small-table pass whose input has too many over-full tiles is left
-    // to k_sift_list.  The test also reads the segment record (its pad word
-    // is 0), so that the two loads go out together and share one wait.
-    if (KC == kWKS && load_scalar(a.dense) > a.dense_max + (unsigned)seg.pad) return;
     const int s = seg.s;
     if (a.active && !a.active[s]) return;
     const int N = sd.len[s];
@@ -1282,8 +1285,14 @@ __device__ __forceinline__ void sift_segment(const SiftArgs<real>& a, const Segm
     }
 }
 
-// One wavefront per segment.  The small-table instance is held to 96
-// registers, so that five of its waves fit on a SIMD (its tables allow eight).
+// A resident grid of wavefronts working through the pass's segments: each
+// wave takes the next segment from the pass's queue (one atomic increment,
+// requested while the previous segment is sifted, so its latency overlaps
+// that segment's loads) until the queue is exhausted.  Segments differ in
+// cost (uneven lengths, knot counts and series ends), and a wave that drew
+// cheap ones simply takes more.  Without a queue (a.queue null) the grid has
+// one wave per segment.  The small-table instance is held to 96 registers,
+// so that five of its waves fit on a SIMD (its tables allow eight).
 template <typename real, bool RILL, int KC>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(KC == kWKS ? 5 : 1)))
 void k_sift_w(SiftArgs<real> a, int nsegs)
@@ -1292,8 +1301,23 @@ void k_sift_w(SiftArgs<real> a, int nsegs)
     __shared__ int s_tp[NW][2][KC];
     __shared__ Coef<real> s_tc[NW][2][KC];
     const int lane = threadIdx.x & 63, wv = wave_index();
-    const int sid = blockIdx.x * NW + wv;
-    if (sid < nsegs) sift_segment<real, RILL, KC>(a, a.segs, sid, lane, s_tp[wv], s_tc[wv]);
+    if (a.pc.run && load_scalar(a.pc.run) == 0u) return;
+    // a small-table pass whose input has too many over-full tiles is left
+    // to k_sift_list
+    if (KC == kWKS && load_scalar(a.dense) > a.dense_max) return;
+    if (!a.queue) {
+        const int sid = blockIdx.x * NW + wv;
+        if (sid < nsegs) sift_segment<real, RILL, KC>(a, a.segs, sid, lane, s_tp[wv], s_tc[wv]);
+        return;
+    }
+    unsigned got = lane == 0 ? atomicAdd(a.queue, 1u) : 0u;
+    int sid = __builtin_amdgcn_readlane((int)got, 0);
+    while (sid < nsegs) {
+        const unsigned next = lane == 0 ? atomicAdd(a.queue, 1u) : 0u;
+        sift_segment<real, RILL, KC>(a, a.segs, sid, lane, s_tp[wv], s_tc[wv]);
+        wave_sync();
+        sid = __builtin_amdgcn_readlane((int)next, 0);
+    }
 }
 
 // Second launch of a small-table pass, with full tables, as a grid-stride
@@ -1307,6 +1331,7 @@ __global__ __launch_bounds__(kThreads) void k_sift_list(SiftArgs<real> a, int ns
     __shared__ int s_tp[NW][2][kWKC];
     __shared__ Coef<real> s_tc[NW][2][kWKC];
     const int lane = threadIdx.x & 63, wv = wave_index();
+    if (a.pc.run && load_scalar(a.pc.run) == 0u) return;
     const bool all = load_scalar(a.dense) > a.dense_max;
     const Segment* segs = all ? a.segs : a.ovf_segs;
     const int ns = all ? nsegs : (int)load_scalar(a.ovf_count);
@@ -1321,24 +1346,21 @@ __global__ __launch_bounds__(kThreads) void k_sift_list(SiftArgs<real> a, int ns
 // MODE is ia.mode as a compile-time constant: one straight-line body per
 // mode, so that every load of the tile goes out before the first wait.
 template <typename real, int MODE>
-__global__ __launch_bounds__(kThreads) void k_init_w(SeriesDev sd, InitArgs<real> ia, real* h_out,
-                                                     KnotsDev<real> kout)
+__device__ __forceinline__ void init_tile(const SeriesDev& sd, const InitArgs<real>& ia, real* h_out,
+                                          const KnotsDev<real>& kout, int gt, int lane)
 {
     using P = typename Promote<real>::type;
-    const int lane = threadIdx.x & 63, wv = wave_index();
-    const int gt = blockIdx.x * (kThreads / kWave) + wv;
-    if (gt >= sd.ntiles) return;
     const int s = sd.tile_series[gt];
     const int t0 = (gt - sd.tile0[s]) * kWT;
     const int N = sd.len[s];
     const int nt = min(kWT, N - t0);
-    real* out = h_out + sd.off[s];
+    real* out = MODE == INIT_KNOTS ? nullptr : h_out + sd.off[s];
     const real* src = nullptr;
     const real* base = nullptr;
     const real* E = nullptr;
     P scl0 = (P)0;
     real sclk = (real)0;
-    if (MODE == INIT_COPY) {
+    if (MODE == INIT_COPY || MODE == INIT_KNOTS) {
         src = ia.src + sd.off[s];
     } else if (MODE == INIT_RESID) {
         src = ia.src + sd.off[s];
@@ -1351,7 +1373,7 @@ __global__ __launch_bounds__(kThreads) void k_init_w(SeriesDev sd, InitArgs<real
         else sclk = (real)ia.scale[c];
     }
     auto val = [&](int p) -> real {
-        if (MODE == INIT_COPY) return src[p];
+        if (MODE == INIT_COPY || MODE == INIT_KNOTS) return src[p];
         if (MODE == INIT_RESID) return src[p] - E[p];
         const real e = E[p];
         if (MODE == INIT_STAGE0) return (real)((P)base[p] + (P)(ia.eps0 * e) * scl0);
@@ -1377,7 +1399,7 @@ __global__ __launch_bounds__(kThreads) void k_init_w(SeriesDev sd, InitArgs<real
     for (int r = 0; r < kWR; ++r) {
         const int u = r * kWave + lane;
         if (u < nt) {
-            out[t0 + u] = x[r];
+            if (MODE != INIT_KNOTS) out[t0 + u] = x[r];
             if (resid) resid[t0 + u] = x[r];
         }
     }
@@ -1407,6 +1429,18 @@ __global__ __launch_bounds__(kThreads) void k_init_w(SeriesDev sd, InitArgs<real
     }
 }
 
+// one wavefront per tile, grid-stride over the tiles (the grid is capped,
+// so any tile count fits one launch)
+template <typename real, int MODE>
+__global__ __launch_bounds__(kThreads) void k_init_w(SeriesDev sd, InitArgs<real> ia, real* h_out,
+                                                     KnotsDev<real> kout)
+{
+    constexpr int NW = kThreads / kWave;
+    const int lane = threadIdx.x & 63, wv = wave_index();
+    for (int gt = blockIdx.x * NW + wv; gt < sd.ntiles; gt += gridDim.x * NW)
+        init_tile<real, MODE>(sd, ia, h_out, kout, gt, lane);
+}
+
 // ================================================================ adjoint
 // With the extrema sets of every pass frozen (recorded by the forward), one
 // sift pass is linear: h' = h - (U + L) / 2.  Its transpose maps the
@@ -1423,27 +1457,21 @@ __global__ __launch_bounds__(kThreads) void k_init_w(SeriesDev sd, InitArgs<real
 // pass, then the moments of mu over the knot intervals of the moment pass:
 // each lane sums its own samples run by run; the run a lane ends with is
 // continued across lanes by a segmented scan keyed by the run number.
+constexpr int kAdjKC = kWT / 2 + 2;
+
 template <typename real>
-__global__ __launch_bounds__(kThreads) void k_adj_tile(AdjTileArgs<real> a)
+__device__ __forceinline__ void adj_tile(const AdjTileArgs<real>& a, int gt, int lane, real* delta,
+                                         int* tp)
 {
     constexpr int T = kWT;
-    constexpr int NW = kThreads / kWave;
-    constexpr int KC = T / 2 + 2;
     constexpr int NOKEY = 1 << 28;
-    __shared__ real s_delta[NW][T];
-    __shared__ int s_tp[NW][KC];
-    const int lane = threadIdx.x & 63, wv = wave_index();
-    const int gt = blockIdx.x * NW + wv;
     const SeriesDev& sd = a.sd;
-    if (gt >= sd.ntiles) return;
     const int s = sd.tile_series[gt];
     const int t = gt - sd.tile0[s];
     const int N = sd.len[s];
     const int t0 = t * T;
     const int nt = min(T, N - t0);
     real* mu = a.mu + sd.off[s];
-    real* delta = s_delta[wv];
-    int* tp = s_tp[wv];
     const int ub = 4 * lane;
 
     real v[4];
@@ -1608,6 +1636,21 @@ __global__ __launch_bounds__(kThreads) void k_adj_tile(AdjTileArgs<real> a)
 #pragma unroll
             for (int z = 0; z < 4; ++z) d[z] = sc[z];
         }
+    }
+}
+
+// one wavefront per tile, grid-stride over the tiles; the wave's LDS slice
+// is reused by its next tile only after a wave barrier
+template <typename real>
+__global__ __launch_bounds__(kThreads) void k_adj_tile(AdjTileArgs<real> a)
+{
+    constexpr int NW = kThreads / kWave;
+    __shared__ real s_delta[NW][kWT];
+    __shared__ int s_tp[NW][kAdjKC];
+    const int lane = threadIdx.x & 63, wv = wave_index();
+    for (int gt = blockIdx.x * NW + wv; gt < a.sd.ntiles; gt += gridDim.x * NW) {
+        adj_tile<real>(a, gt, lane, s_delta[wv], s_tp[wv]);
+        wave_sync();
     }
 }
 
@@ -2023,18 +2066,21 @@ __global__ void k_inv_sqrt(const real* var, double* out, int n, int take_sqrt)
     out[i] = take_sqrt ? (double)sd : 1.0 / (double)sd;
 }
 
+// elementwise kernels: grid-stride with 64-bit indices (capped grids)
 template <typename real>
 __global__ void k_sub(real* w, const real* e, int64_t n)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) w[i] = w[i] - e[i];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        w[i] = w[i] - e[i];
 }
 
 template <typename real>
 __global__ void k_copy(real* dst, const real* src, int64_t n)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[i];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 
 // dst[s] = (parity[s] odd ? hB : hA)[s] for every series (RILLING export)
@@ -2064,8 +2110,9 @@ __global__ void k_rec_left(const int* nbx, int* left, int* right, int ntiles)
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
 }
 
 // Order-independent 64-bit digest of the knot sets of a pass input, per
@@ -2102,6 +2149,36 @@ __global__ void k_check_index(const int* x, int64_t N, int* err)
 // ---------------------------------------------------------------- launchers
 
 static inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+// grid of a grid-stride launch: at most 2^20 workgroups (2^28 work-items)
+static inline unsigned capped_blocks(int64_t n, int b)
+{
+    const int64_t g = (n + b - 1) / b;
+    return (unsigned)(g < 1 ? 1 : (g > (1 << 20) ? (1 << 20) : g));
+}
+
+// Workgroups of `kernel` (kThreads each) that the device keeps resident at
+// once, per device (the grid of a persistent launch).
+template <typename K>
+static int resident_groups(K kernel)
+{
+    static std::mutex mu;
+    static std::map<std::pair<const void*, int>, int> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    const auto key = std::make_pair((const void*)kernel, dev);
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int cus = 0, per = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kThreads, 0) != hipSuccess ||
+        per <= 0)
+        per = 4;
+    cache[key] = cus * per;
+    return cus * per;
+}
 
 template <typename real> struct Geom;
 template <> struct Geom<double> { static constexpr int B = 8, H = 32; };
@@ -2112,7 +2189,7 @@ template <typename real>
 hipError_t launch_init(const SeriesDev& sd, const InitArgs<real>& ia, real* h_out,
                        const KnotsDev<real>& kout, const PassCtl&, hipStream_t st)
 {
-    const dim3 grid(blocks_for(sd.ntiles, kThreads / kWave));
+    const dim3 grid(capped_blocks(sd.ntiles, kThreads / kWave));
     switch (ia.mode) {
     case INIT_COPY:
         hipLaunchKernelGGL((k_init_w<real, INIT_COPY>), grid, dim3(kThreads), 0, st, sd, ia, h_out, kout);
@@ -2122,6 +2199,9 @@ hipError_t launch_init(const SeriesDev& sd, const InitArgs<real>& ia, real* h_ou
         break;
     case INIT_STAGE0:
         hipLaunchKernelGGL((k_init_w<real, INIT_STAGE0>), grid, dim3(kThreads), 0, st, sd, ia, h_out, kout);
+        break;
+    case INIT_KNOTS:
+        hipLaunchKernelGGL((k_init_w<real, INIT_KNOTS>), grid, dim3(kThreads), 0, st, sd, ia, h_out, kout);
         break;
     default:
         hipLaunchKernelGGL((k_init_w<real, INIT_STAGEK>), grid, dim3(kThreads), 0, st, sd, ia, h_out, kout);
@@ -2143,15 +2223,27 @@ hipError_t launch_knot_prefix(const SeriesDev& sd, const KnotsDev<real>& k, cons
     return hipGetLastError();
 }
 
+// grid of a k_sift_w launch: with a work queue, what the device keeps
+// resident (no more than one wave per segment); without, one wave per segment
+template <typename K>
+static dim3 sift_grid(K kernel, const unsigned* queue, int nsegs)
+{
+    const int64_t need = blocks_for(nsegs, kThreads / kWave);
+    if (!queue) return dim3((unsigned)need);
+    return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(need, resident_groups(kernel))));
+}
+
 template <typename real>
 hipError_t launch_sift(const SiftArgs<real>& a, int nsegs, hipStream_t st)
 {
-    const dim3 grid(blocks_for(nsegs, kThreads / kWave));
     if (!a.ovf_segs) {
-        if (a.sx_cnt)
-            hipLaunchKernelGGL((k_sift_w<real, true, kWKC>), grid, dim3(kThreads), 0, st, a, nsegs);
-        else
-            hipLaunchKernelGGL((k_sift_w<real, false, kWKC>), grid, dim3(kThreads), 0, st, a, nsegs);
+        if (a.sx_cnt) {
+            auto k = k_sift_w<real, true, kWKC>;
+            hipLaunchKernelGGL(k, sift_grid(k, a.queue, nsegs), dim3(kThreads), 0, st, a, nsegs);
+        } else {
+            auto k = k_sift_w<real, false, kWKC>;
+            hipLaunchKernelGGL(k, sift_grid(k, a.queue, nsegs), dim3(kThreads), 0, st, a, nsegs);
+        }
         return hipGetLastError();
     }
     // sparse pass: if few tiles of the input overflow the small knot tables
@@ -2160,12 +2252,14 @@ hipError_t launch_sift(const SiftArgs<real>& a, int nsegs, hipStream_t st)
     // launch returns at once and the second launch sifts every segment.
     // Its grid is what the full-table kernel keeps resident (four
     // workgroups per CU), so it costs little when it has nothing to do.
-    const dim3 lgrid(std::min<int>(grid.x, 1024));
+    const dim3 lgrid(std::min<unsigned>(blocks_for(nsegs, kThreads / kWave), 1024u));
     if (a.sx_cnt) {
-        hipLaunchKernelGGL((k_sift_w<real, true, kWKS>), grid, dim3(kThreads), 0, st, a, nsegs);
+        auto k = k_sift_w<real, true, kWKS>;
+        hipLaunchKernelGGL(k, sift_grid(k, a.queue, nsegs), dim3(kThreads), 0, st, a, nsegs);
         hipLaunchKernelGGL((k_sift_list<real, true>), lgrid, dim3(kThreads), 0, st, a, nsegs);
     } else {
-        hipLaunchKernelGGL((k_sift_w<real, false, kWKS>), grid, dim3(kThreads), 0, st, a, nsegs);
+        auto k = k_sift_w<real, false, kWKS>;
+        hipLaunchKernelGGL(k, sift_grid(k, a.queue, nsegs), dim3(kThreads), 0, st, a, nsegs);
         hipLaunchKernelGGL((k_sift_list<real, false>), lgrid, dim3(kThreads), 0, st, a, nsegs);
     }
     return hipGetLastError();
@@ -2182,10 +2276,11 @@ hipError_t launch_solve(const SolveArgs<real>& a, int grid, hipStream_t st)
 template <typename real>
 hipError_t launch_rilling_decide(const SeriesDev& sd, const int* nk, const int* sx_cnt,
                                  const int* sx_mark, int* active, int* stop_pass, int* subs,
-                                 int pass, int cap, unsigned* n_active, hipStream_t st)
+                                 int pass, int cap, const unsigned* run, unsigned* n_active,
+                                 hipStream_t st)
 {
     hipLaunchKernelGGL((k_rilling_decide<real>), dim3(sd.nseries), dim3(kWave), 0, st, sd, nk,
-                       sx_cnt, sx_mark, active, stop_pass, subs, pass, cap, n_active);
+                       sx_cnt, sx_mark, active, stop_pass, subs, pass, cap, run, n_active);
     return hipGetLastError();
 }
 
@@ -2241,14 +2336,14 @@ hipError_t launch_inv_sqrt(const real* var, double* out, int n, int take_sqrt, h
 template <typename real>
 hipError_t launch_sub(real* w, const real* e, int64_t n, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_sub<real>), dim3(blocks_for(n, 256)), dim3(256), 0, st, w, e, n);
+    hipLaunchKernelGGL((k_sub<real>), dim3(capped_blocks(n, 256)), dim3(256), 0, st, w, e, n);
     return hipGetLastError();
 }
 
 template <typename real>
 hipError_t launch_copy(real* dst, const real* src, int64_t n, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_copy<real>), dim3(blocks_for(n, 256)), dim3(256), 0, st, dst, src, n);
+    hipLaunchKernelGGL((k_copy<real>), dim3(capped_blocks(n, 256)), dim3(256), 0, st, dst, src, n);
     return hipGetLastError();
 }
 
@@ -2266,7 +2361,7 @@ template <typename real>
 hipError_t launch_adj_tile(const AdjTileArgs<real>& a, hipStream_t st)
 {
     if (2 * (a.sd.cap - 2) != kWT) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_adj_tile<real>), dim3(blocks_for(a.sd.ntiles, kThreads / kWave)),
+    hipLaunchKernelGGL((k_adj_tile<real>), dim3(capped_blocks(a.sd.ntiles, kThreads / kWave)),
                        dim3(kThreads), 0, st, a);
     return hipGetLastError();
 }
@@ -2327,7 +2422,7 @@ hipError_t launch_rec_left(const int* nbx, int* left, int* right, int ntiles, hi
 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, v, n);
+    hipLaunchKernelGGL(k_fill_u32, dim3(capped_blocks(n, 256)), dim3(256), 0, st, p, v, n);
     return hipGetLastError();
 }
 
@@ -2359,7 +2454,7 @@ hipError_t launch_check_index(const int* d_x, int64_t N, int* err, hipStream_t s
     template hipError_t launch_solve<real>(const SolveArgs<real>&, int, hipStream_t);              \
     template hipError_t launch_rilling_decide<real>(const SeriesDev&, const int*, const int*,      \
                                                     const int*, int*, int*, int*, int, int,        \
-                                                    unsigned*, hipStream_t);                       \
+                                                    const unsigned*, unsigned*, hipStream_t);      \
     template hipError_t launch_series_var<real>(const real*, const int64_t*, int64_t, const int*,  \
                                                 int, real*, hipStream_t);                          \
     template hipError_t launch_noise<real>(uint64_t, int, int64_t, real*, int64_t, hipStream_t);  \

@@ -21,7 +21,9 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "../../include/iceemdan_mi355x.h"
@@ -48,6 +50,8 @@ const uint32_t kEpochs = 65536;
 const int kMaxSifts = 32000;
 // Channels: the per-channel launches put the channel on gridDim.y.
 const int kMaxChannels = 65535;
+// tiles of one phase (series x tiles per series)
+const int64_t kMaxTiles = (int64_t)1 << 28;
 
 // --------------------------------------------------------------- geometry
 
@@ -79,18 +83,33 @@ int validate(const icm_config* c)
     int64_t total = 0;
     for (int i = 0; i < c->num_channels; ++i) total += c->lengths[i];
     if (total * (int64_t)c->num_noise > (int64_t)1 << 40) return ICM_ERR_INVALID_ARG;
-    // one launch covers every tile of a phase: tiles x 256 work-items < 2^32
+    // tiles of a phase are indexed with 32-bit integers (eight mask words per
+    // tile and type pairs included): at most 2^28 tiles.  Launch grids are
+    // capped (grid-stride and work-queue kernels), so this is the only limit
+    // on the work of one launch.
     int64_t tiles = 0;
     const int T = tile_of(c->dtype);
     for (int i = 0; i < c->num_channels; ++i) tiles += (c->lengths[i] + T - 1) / T;
-    if (tiles * c->num_noise * 256 >= ((int64_t)1 << 32)) return ICM_ERR_INVALID_ARG;
+    if (tiles * c->num_noise > kMaxTiles) return ICM_ERR_INVALID_ARG;
     return ICM_OK;
 }
+
+// ---------------------------------------------------------- sift geometry
+// The sift pass is scheduled as segments (runs of consecutive tiles of one
+// series) taken from a work queue.  Segment list 0 is the default geometry
+// (clamp(tiles / 56000, 1, 32) tiles, or ICM_SEGMENT_TILES); lists 1..kNSeg
+// have kSegCand[l - 1] tiles per segment and are the autotuner's candidates.
+// Tuning candidate c (0 .. kNCand-1): list 1 + c / 2, small knot tables if c
+// is odd; -1 = the default (list 0, small tables from stage sparse_stage on).
+constexpr int kNSeg = 4;
+constexpr int kSegCand[kNSeg] = {4, 8, 16, 32};
+constexpr int kNCand = 2 * kNSeg;
+constexpr int kNLists = 1 + kNSeg;
 
 // Series tables of one phase (host side), built from (length, count) groups.
 struct HostSeries {
     std::vector<int> tile_order, tile_series, tile0, ntl, len, chan, nidx;
-    std::vector<Segment> segs;
+    std::vector<Segment> segs[kNLists];
     std::vector<int64_t> off, kbase, eoff;
     int64_t samples = 0, kcap = 0, maxwin = 0;
     int ntiles = 0;
@@ -137,19 +156,22 @@ static void build_series(HostSeries& hs, const std::vector<int64_t>& lens, int I
     // segments of consecutive tiles (one wavefront each for 256-sample
     // tiles, one workgroup each otherwise): long enough to amortize the
     // prefetch pipeline, short enough to fill every CU
-    int seg = 1;
+    int seg0 = 1;
     const char* env = std::getenv("ICM_SEGMENT_TILES");
-    if (env && std::atoi(env) > 0) seg = std::atoi(env);
-    else seg = std::max(1, std::min(32, tiles / (T == 256 ? 56000 : 6144)));
-    for (size_t s = 0; s < hs.ntl.size(); ++s)
-        for (int t = 0; t < hs.ntl[s]; t += seg) {
-            Segment g;
-            g.s = (int)s;
-            g.t_first = t;
-            g.count = std::min(seg, hs.ntl[s] - t);
-            g.pad = 0;
-            hs.segs.push_back(g);
-        }
+    if (env && std::atoi(env) > 0) seg0 = std::atoi(env);
+    else seg0 = std::max(1, std::min(32, tiles / (T == 256 ? 56000 : 6144)));
+    for (int l = 0; l < kNLists; ++l) {
+        const int seg = l == 0 ? seg0 : kSegCand[l - 1];
+        for (size_t s = 0; s < hs.ntl.size(); ++s)
+            for (int t = 0; t < hs.ntl[s]; t += seg) {
+                Segment g;
+                g.s = (int)s;
+                g.t_first = t;
+                g.count = std::min(seg, hs.ntl[s] - t);
+                g.pad = 0;
+                hs.segs[l].push_back(g);
+            }
+    }
     // launch order of the tile kernels: tile t of every series before tile t+1
     int maxt = 0;
     for (int nt : hs.ntl) maxt = std::max(maxt, nt);
@@ -159,11 +181,32 @@ static void build_series(HostSeries& hs, const std::vector<int64_t>& lens, int I
             if (t < hs.ntl[s]) hs.tile_order.push_back(hs.tile0[s] + t);
 }
 
+// --------------------------------------------------------- tuning table
+// Chosen sift candidates per stage, by plan shape, shared by the plans of
+// this process (a plan of a shape already tuned starts tuned).
+std::mutex g_tune_mu;
+std::map<std::string, std::vector<int>> g_tune_table;
+
+bool tune_table_get(const std::string& key, std::vector<int>& out)
+{
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto it = g_tune_table.find(key);
+    if (it == g_tune_table.end() || it->second.size() != out.size()) return false;
+    out = it->second;
+    return true;
+}
+
+void tune_table_put(const std::string& key, const std::vector<int>& v)
+{
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune_table[key] = v;
+}
+
 // ------------------------------------------------------------------ plan
 
 struct DevSeries {
-    Segment* segs = nullptr;
-    int nsegs = 0;
+    Segment* segs[kNLists] = {};
+    int nsegs[kNLists] = {};
     int *tile_order = nullptr, *tile_series = nullptr, *tile0 = nullptr, *ntl = nullptr, *len = nullptr, *chan = nullptr,
         *nidx = nullptr;
     int64_t *off = nullptr, *kbase = nullptr, *eoff = nullptr;
@@ -171,7 +214,7 @@ struct DevSeries {
     int64_t samples = 0, kcap = 0, maxwin = 0;
     SeriesDev view(int64_t kstride, int cap) const
     {
-        SeriesDev v;
+        SeriesDev v{};
         v.cap = cap;
         v.tile_order = tile_order; v.tile_series = tile_series; v.tile0 = tile0; v.ntl = ntl; v.off = off; v.len = len;
         v.kbase = kbase; v.chan = chan; v.eoff = eoff; v.nidx = nidx;
@@ -201,8 +244,10 @@ struct PlanBase {
     unsigned* wincount = nullptr;
     int* winlist = nullptr;
     int* err = nullptr;
-    unsigned* n_active = nullptr;
-    unsigned* h_n_active = nullptr;   // pinned
+    unsigned* n_active = nullptr;     // [2] RILLING: series still sifting, by pass parity
+    unsigned* h_n_active = nullptr;   // [2] pinned copies read by the host (one per poll slot)
+    hipEvent_t poll_ev[2] = {nullptr, nullptr};
+    bool capturing = false;           // a phase is being captured into a graph
     int* h_err = nullptr;             // pinned
     // rilling
     int *active = nullptr, *stop_pass = nullptr, *subs = nullptr, *sx_cnt = nullptr,
@@ -233,6 +278,157 @@ struct PlanBase {
     double kern_ms[3] = {0, 0, 0};
     int64_t kern_n[3] = {0, 0, 0};
     int64_t kern_bytes[3] = {0, 0, 0};
+    // autotuner of the sift pass: per phase and stage, the segment list and
+    // knot-table size (candidate index, -1 = default geometry)
+    int tune_left = 0;              // icm_forward calls that still time candidates
+    int tune_round = 0;             // tuning forwards completed
+    bool tuning = false;            // the running forward times candidates
+    std::vector<int> choice[2];     // [M-1]
+    std::vector<double> tsum[2];    // [(M-1) * kNCand] ms of timed launches
+    std::vector<int> tcnt[2];
+    std::vector<hipEvent_t> tev;    // event pairs around timed sift launches
+    std::vector<int> ttag;          // (phase, stage, candidate) of each pair
+    size_t tused = 0;
+    std::string tkey[2];            // shape keys of the process-wide table
+
+    int tune_candidate(int phase, int stage, int pass) const
+    {
+        if (tuning) return (pass + 3 * tune_round + stage) % kNCand;
+        if (stage >= 0 && stage < (int)choice[phase].size()) return choice[phase][stage];
+        return -1;
+    }
+    int tune_begin(int phase, int stage, int cand, hipStream_t st)
+    {
+        if (!tuning || cand < 0) return ICM_OK;
+        while (tev.size() < 2 * (tused + 1)) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return ICM_ERR_HIP;
+            tev.push_back(e);
+        }
+        ttag.resize(tused + 1);
+        ttag[tused] = (phase * 65536 + stage) * kNCand + cand;
+        return hipEventRecord(tev[2 * tused], st) == hipSuccess ? ICM_OK : ICM_ERR_HIP;
+    }
+    int tune_end(int cand, hipStream_t st)
+    {
+        if (!tuning || cand < 0) return ICM_OK;
+        const hipError_t e = hipEventRecord(tev[2 * tused + 1], st);
+        ++tused;
+        return e == hipSuccess ? ICM_OK : ICM_ERR_HIP;
+    }
+    // after a completed tuning forward (its stream synchronised)
+    void tune_collect()
+    {
+        for (size_t i = 0; i < tused; ++i) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, tev[2 * i], tev[2 * i + 1]) != hipSuccess) continue;
+            const int tag = ttag[i];
+            const int cand = tag % kNCand, ps = tag / kNCand;
+            const int phase = ps / 65536, stage = ps % 65536;
+            const size_t at = (size_t)stage * kNCand + cand;
+            if (phase < 2 && at < tsum[phase].size()) {
+                tsum[phase][at] += ms;
+                tcnt[phase][at] += 1;
+            }
+        }
+        tused = 0;
+        tuning = false;
+        ++tune_round;
+        if (tune_left > 0 && --tune_left == 0) tune_finish();
+    }
+    // per stage, the candidate with the least mean time; stored in the
+    // process-wide table under the plan's shape
+    void tune_finish()
+    {
+        tune_left = 0;
+        if (tune_round == 0) return;
+        for (int ph = 0; ph < 2; ++ph) {
+            for (size_t k = 0; k < choice[ph].size(); ++k) {
+                int best = -1;
+                double bm = 0;
+                for (int c = 0; c < kNCand; ++c) {
+                    const size_t at = k * kNCand + c;
+                    if (!tcnt[ph][at]) continue;
+                    const double m = tsum[ph][at] / tcnt[ph][at];
+                    if (best < 0 || m < bm) { best = c; bm = m; }
+                }
+                choice[ph][k] = best;
+            }
+            tune_table_put(tkey[ph], choice[ph]);
+        }
+    }
+    void drop_graphs()
+    {
+        for (int i = 0; i < 2; ++i)
+            if (gexec[i]) {
+                (void)hipGraphExecDestroy(gexec[i]);
+                gexec[i] = nullptr;
+            }
+    }
+    // the plan's shape, as the key of the process-wide tuning table
+    std::string shape_key(int phase) const
+    {
+        const HostSeries& h = phase == ICM_PHASE_NOISE ? hn : hsig;
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        char buf[256];
+        std::snprintf(buf, sizeof buf, "d%d|t%d|m%d|b%d|S%d|M%d|p%d|n%zu|T%d|L%d|X%lld", dev,
+                      cfg.dtype, cfg.stop_mode, cfg.boundary,
+                      phase == ICM_PHASE_NOISE ? cfg.noise_max_sifts : cfg.max_sifts,
+                      cfg.num_imfs, phase, h.len.size(), h.ntiles, h.max_len,
+                      (long long)h.samples);
+        return buf;
+    }
+    // starts tuned when the table holds this shape; else tunes over the
+    // first ICM_AUTOTUNE (default 2) forwards
+    void tune_init()
+    {
+        const int M = cfg.num_imfs;
+        for (int ph = 0; ph < 2; ++ph) {
+            choice[ph].assign(M - 1, -1);
+            tsum[ph].assign((size_t)(M - 1) * kNCand, 0.0);
+            tcnt[ph].assign((size_t)(M - 1) * kNCand, 0);
+            tkey[ph] = shape_key(ph);
+        }
+        int n = 2;
+        if (const char* env = std::getenv("ICM_AUTOTUNE")) n = std::max(0, std::atoi(env));
+        tune_left = n;
+        if (n > 0 && tune_table_get(tkey[0], choice[0]) && tune_table_get(tkey[1], choice[1]))
+            tune_left = 0;
+        else if (tune_left > 0)
+            for (int ph = 0; ph < 2; ++ph) choice[ph].assign(M - 1, -1);
+    }
+    int set_autotune(int forwards)
+    {
+        if (forwards < 0) return ICM_ERR_INVALID_ARG;
+        for (int ph = 0; ph < 2; ++ph) {
+            std::fill(tsum[ph].begin(), tsum[ph].end(), 0.0);
+            std::fill(tcnt[ph].begin(), tcnt[ph].end(), 0);
+            if (forwards == 0) std::fill(choice[ph].begin(), choice[ph].end(), -1);
+        }
+        tune_left = forwards;
+        tune_round = 0;
+        drop_graphs();
+        return ICM_OK;
+    }
+    int get_tuning(int phase, int32_t* out, int64_t count) const
+    {
+        if (phase < 0 || phase > 1 || !out || count < (int64_t)choice[phase].size())
+            return ICM_ERR_INVALID_ARG;
+        for (size_t k = 0; k < choice[phase].size(); ++k) out[k] = choice[phase][k];
+        return ICM_OK;
+    }
+    int set_tuning(int phase, const int32_t* in, int64_t count)
+    {
+        if (phase < 0 || phase > 1 || !in || count != (int64_t)choice[phase].size())
+            return ICM_ERR_INVALID_ARG;
+        for (int64_t k = 0; k < count; ++k)
+            if (in[k] < -1 || in[k] >= kNCand) return ICM_ERR_INVALID_ARG;
+        for (int64_t k = 0; k < count; ++k) choice[phase][k] = in[k];
+        tune_left = 0;
+        drop_graphs();
+        return ICM_OK;
+    }
 
     int prof_begin(int kind, int64_t bytes, hipStream_t st)
     {
@@ -305,8 +501,10 @@ struct PlanBase {
     int upload_series(DevSeries& d, const HostSeries& h)
     {
         int rc;
-        if ((rc = upload(&d.segs, h.segs))) return rc;
-        d.nsegs = (int)h.segs.size();
+        for (int l = 0; l < kNLists; ++l) {
+            if ((rc = upload(&d.segs[l], h.segs[l]))) return rc;
+            d.nsegs[l] = (int)h.segs[l].size();
+        }
         if ((rc = upload(&d.tile_order, h.tile_order))) return rc;
         if ((rc = upload(&d.tile_series, h.tile_series))) return rc;
         if ((rc = upload(&d.tile0, h.tile0))) return rc;
@@ -333,7 +531,10 @@ struct PlanBase {
             if (gexec[i]) (void)hipGraphExecDestroy(gexec[i]);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
+        for (auto& e : poll_ev)
+            if (e) (void)hipEventDestroy(e);
         for (auto& e : prof_ev) (void)hipEventDestroy(e);
+        for (auto& e : tev) (void)hipEventDestroy(e);
         for (void* p : allocs) (void)hipFree(p);
         allocs.clear();
         if (h_n_active) (void)hipHostFree(h_n_active);
@@ -413,6 +614,9 @@ int64_t plan_bytes_estimate(const icm_config* c)
     b += 2 * P.ntiles_max * (6 * (4 + s) + 8 * 8);  // neighbour knots, knot bitmasks
     b += (int64_t)kEpochs * 4 + (int64_t)kWinItem * P.maxwin * 4;
     b += 2 * (int64_t)kEpochs * 4 + P.ntiles_max * (int64_t)sizeof(Segment);  // sparse-pass tile lists
+    b += (int64_t)kEpochs * 4;                                                // work-queue counters
+    for (int l = 0; l < kNLists; ++l)                                         // segment lists
+        b += (int64_t)(P.hn.segs[l].size() + P.hsig.segs[l].size()) * (int64_t)sizeof(Segment);
     b += 5 * P.chan_total * s + P.chan_total * M * s;
     b += (int64_t)(M - 1) * P.chan_total * s;     // residue history
     if (c->keep_records && c->stop_mode == ICM_STOP_FIXED)   // per-pass knot records
@@ -452,6 +656,7 @@ struct Plan : PlanBase {
     Segment* ovfsegs = nullptr;        // [ntiles] tiles a sparse sift pass left to the full tables
     unsigned* ovfcount = nullptr;      // [kEpochs] their number per pass
     unsigned* densecnt = nullptr;      // [kEpochs] tiles of a pass input that overflow small tables
+    unsigned* squeue = nullptr;        // [kEpochs] work-queue counter of each sift launch
     int sparse_stage = 1;              // first stage of a phase sifted with small knot tables
     int64_t sparse_dense_max = -1;     // over-full tiles tolerated by them (-1: tiles / 64)
     std::vector<Rec> rec;                  // re-sift records of one stage (backward)
@@ -484,6 +689,7 @@ struct Plan : PlanBase {
         if ((rc = alloc(&wincount, kEpochs))) return rc;
         if ((rc = alloc(&ovfcount, kEpochs))) return rc;
         if ((rc = alloc(&densecnt, kEpochs))) return rc;
+        if ((rc = alloc(&squeue, kEpochs))) return rc;
         // ICM_SPARSE_STAGE: first stage sifted with small knot tables (a
         // tuning knob; 0 also sends the dense first stage through them, and
         // most of its tiles through the full-table launch)
@@ -496,7 +702,7 @@ struct Plan : PlanBase {
         if ((rc = alloc(&ovfsegs, ntiles_max))) return rc;
         if ((rc = alloc(&winlist, (int64_t)kWinItem * maxwin))) return rc;
         if ((rc = alloc(&err, 1))) return rc;
-        if ((rc = alloc(&n_active, 1))) return rc;
+        if ((rc = alloc(&n_active, 2))) return rc;
         if ((rc = alloc(&y, chan_total))) return rc;
         if ((rc = alloc(&xt, chan_total))) return rc;
         if ((rc = alloc(&r, chan_total))) return rc;
@@ -523,19 +729,23 @@ struct Plan : PlanBase {
                 if ((rc = alloc_recs(krec[k]))) return rc;
             krec_valid.assign(M - 1, 0);
         }
-        if (hipHostMalloc((void**)&h_n_active, sizeof(unsigned), hipHostMallocDefault) != hipSuccess)
+        if (hipHostMalloc((void**)&h_n_active, 2 * sizeof(unsigned), hipHostMallocDefault) !=
+            hipSuccess)
             return ICM_ERR_OUT_OF_MEMORY;
         if (hipHostMalloc((void**)&h_err, sizeof(int), hipHostMallocDefault) != hipSuccess)
             return ICM_ERR_OUT_OF_MEMORY;
         if (hipStreamCreateWithFlags(&own, hipStreamNonBlocking) != hipSuccess) return ICM_ERR_HIP;
         for (auto& e : ev)
             if (hipEventCreate(&e) != hipSuccess) return ICM_ERR_HIP;
+        for (auto& e : poll_ev)
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return ICM_ERR_HIP;
+        tune_init();
         return ICM_OK;
     }
 
     PassCtl ctl(uint32_t e)
     {
-        PassCtl pc;
+        PassCtl pc{};
         pc.epoch = e;
         pc.winlist = winlist;
         pc.wincount = wincount;
@@ -547,7 +757,7 @@ struct Plan : PlanBase {
     }
     KnotsDev<real> knots(int b)
     {
-        KnotsDev<real> k;
+        KnotsDev<real> k{};
         k.cnt = kcnt[b];
         k.pre = kpre[b];
         k.n = nk[b];
@@ -571,6 +781,7 @@ struct Plan : PlanBase {
         ICM_TRY(launch_fill_u32((uint32_t*)wincount, 0u, kEpochs, st));
         ICM_TRY(launch_fill_u32((uint32_t*)ovfcount, 0u, kEpochs, st));
         ICM_TRY(launch_fill_u32((uint32_t*)densecnt, 0u, kEpochs, st));
+        ICM_TRY(launch_fill_u32((uint32_t*)squeue, 0u, kEpochs, st));
         epoch = 0;
         return ICM_OK;
     }
@@ -582,15 +793,34 @@ struct Plan : PlanBase {
     // launch, unless more than 1/64 of the input's tiles (counted by the knot
     // prefix of epoch e_in) have more knots: then with full tables.  The
     // result is the same either way.
-    void sparse_pass(SiftArgs<real>& a, int stage, const DevSeries& d, uint32_t e_in, uint32_t e)
+    void sparse_pass(SiftArgs<real>& a, const DevSeries& d, uint32_t e_in, uint32_t e)
     {
-        if (stage < sparse_stage) return;
         a.ovf_segs = ovfsegs;
         a.ovf_count = ovfcount + e;
         a.dense = densecnt + e_in;
         a.dense_max = sparse_dense_max >= 0
                           ? (unsigned)std::min<int64_t>(sparse_dense_max, 0xFFFFFFFFll)
                           : (unsigned)(d.ntiles / 64);
+    }
+
+    // The sift launch of pass `pass` of a stage: the segment list and knot
+    // tables of the stage's tuning candidate (or, while tuning, of the
+    // candidate whose turn it is, timed with events), the pass's work queue.
+    int sift_launch(SiftArgs<real>& a, int phase, int stage, int pass, const DevSeries& d,
+                    uint32_t e_in, uint32_t e, int64_t bytes, hipStream_t st)
+    {
+        const int cand = tune_candidate(phase, stage, pass);
+        const int l = cand < 0 ? 0 : 1 + cand / 2;
+        const bool small = cand < 0 ? stage >= sparse_stage : (cand & 1) != 0;
+        a.segs = d.segs[l];
+        a.queue = squeue + e;
+        if (small) sparse_pass(a, d, e_in, e);
+        int rc;
+        if ((rc = prof_begin(2, bytes, st))) return rc;
+        if ((rc = tune_begin(phase, stage, cand, st))) return rc;
+        ICM_TRY(launch_sift<real>(a, d.nsegs[l], st));
+        if ((rc = tune_end(cand, st))) return rc;
+        return prof_end(st);
     }
 
     int trace_copy(int phase, int stage, int pass, const DevSeries& d, int b, hipStream_t st)
@@ -652,7 +882,6 @@ struct Plan : PlanBase {
             if (record && !full && j == S - 1) break;
             SiftArgs<real> a{};
             a.sd = sd;
-            a.segs = d.segs;
             a.h_in = b ? hB : hA;
             a.h_out = (j == S - 1 && out) ? out : (b ? hA : hB);
             a.kin = knots(b);
@@ -665,10 +894,7 @@ struct Plan : PlanBase {
             a.active = nullptr;
             a.sx_cnt = nullptr;
             a.sx_mark = nullptr;
-            sparse_pass(a, stage, d, e_in, e_emit);
-            if ((rc = prof_begin(2, pass_bytes, st))) return rc;
-            ICM_TRY(launch_sift<real>(a, d.nsegs, st));
-            if ((rc = prof_end(st))) return rc;
+            if ((rc = sift_launch(a, phase, stage, j, d, e_in, e_emit, pass_bytes, st))) return rc;
             if (j < S - 1) {
                 ICM_TRY(launch_knot_prefix<real>(sd, knots(b ^ 1), a.h_out, ctl(e_emit), nullptr, st));
                 if ((rc = trace_copy(phase, stage, j + 1, d, b ^ 1, st))) return rc;
@@ -680,6 +906,15 @@ struct Plan : PlanBase {
     // RILLING mode: passes until every series stopped or the cap is reached.
     // Returns with stop_pass[s] = pass whose input h is the exported mode
     // (buffer stop_pass & 1).
+    // The passes' kernels return at once when no series of the phase is left
+    // (n_active[(j + 1) & 1], the count after pass j - 1, is 0), so the pass
+    // loop needs no answer from the device: it is captured whole into the
+    // phase's graph when the caps are small (graph_rilling), and otherwise
+    // launched with one look at the count per kRillPoll passes, a batch
+    // behind (the host reads the count of the previous batch while the
+    // current one runs; the batch launched after the last series stopped
+    // returns at once).
+    static constexpr int kRillPoll = 8;
     int stage_rilling(int phase, int stage, const DevSeries& d, const InitArgs<real>& ia, int max_pass,
                       hipStream_t st)
     {
@@ -687,6 +922,8 @@ struct Plan : PlanBase {
         ICM_TRY(launch_fill_u32((uint32_t*)active, 1u, d.nseries, st));
         ICM_TRY(launch_fill_u32((uint32_t*)subs, 0u, d.nseries, st));
         ICM_TRY(launch_fill_u32((uint32_t*)stop_pass, 0u, d.nseries, st));
+        // before pass 0 every series sifts
+        ICM_TRY(launch_fill_u32((uint32_t*)n_active + 1, (uint32_t)d.nseries, 1, st));
         if (epoch + 2 * (uint32_t)max_pass + 8 >= kEpochs - 1) {
             int rc = reset_control(st);
             if (rc) return rc;
@@ -696,8 +933,11 @@ struct Plan : PlanBase {
         ICM_TRY(launch_knot_prefix<real>(sd, knots(0), hA, ctl(e_emit), nullptr, st));
         int rc;
         if ((rc = trace_copy(phase, stage, 0, d, 0, st))) return rc;
+        int pending = -1;   // poll slot whose count the host has not read yet
         for (int j = 0; j < max_pass; ++j) {
             const int b = j & 1;
+            const unsigned* run_prev = n_active + ((j + 1) & 1);
+            unsigned* run_this = n_active + (j & 1);
             SolveArgs<real> sa{};
             sa.sd = sd;
             sa.k = knots(b);
@@ -708,10 +948,10 @@ struct Plan : PlanBase {
             sa.active = active;
             sa.err = err;
             sa.nak = cfg.boundary == ICM_BOUNDARY_NOT_A_KNOT;
+            sa.run = run_prev;
             ICM_TRY(launch_solve<real>(sa, solve_grid(), st));
             SiftArgs<real> a{};
             a.sd = sd;
-            a.segs = d.segs;
             a.h_in = b ? hB : hA;
             a.h_out = b ? hA : hB;
             a.kin = knots(b);
@@ -721,24 +961,33 @@ struct Plan : PlanBase {
             const uint32_t e_in = e_emit;
             e_emit = next_epoch();
             a.pc = ctl(e_emit);
+            a.pc.run = run_prev;
             a.active = active;
             a.sx_cnt = sx_cnt;
             a.sx_mark = sx_mark;
-            sparse_pass(a, stage, d, e_in, e_emit);
-            ICM_TRY(launch_sift<real>(a, d.nsegs, st));
-            ICM_TRY(launch_fill_u32((uint32_t*)n_active, 0u, 1, st));
+            if ((rc = sift_launch(a, phase, stage, j, d, e_in, e_emit,
+                                  2 * (int64_t)sizeof(real) * d.samples, st)))
+                return rc;
+            ICM_TRY(launch_fill_u32((uint32_t*)run_this, 0u, 1, st));
             ICM_TRY(launch_rilling_decide<real>(sd, nk[b], sx_cnt, sx_mark, active, stop_pass, subs,
-                                                j, max_pass, n_active, st));
-            ICM_TRY(launch_knot_prefix<real>(sd, knots(b ^ 1), a.h_out, ctl(e_emit), active, st));
+                                                j, max_pass, run_prev, run_this, st));
+            PassCtl pn = ctl(e_emit);
+            pn.run = run_this;
+            ICM_TRY(launch_knot_prefix<real>(sd, knots(b ^ 1), a.h_out, pn, active, st));
             if ((rc = trace_copy(phase, stage, j + 1, d, b ^ 1, st))) return rc;
             // early exit: results are identical to running every pass to the
-            // cap with finished series masked (they never change again)
-            const bool poll = (j < 16) || ((j & 3) == 3) || (j == max_pass - 1);
-            if (poll) {
-                ICM_TRY(hipMemcpyAsync(h_n_active, n_active, sizeof(unsigned),
+            // cap with finished series masked (they never change again).
+            // Not while a graph is being captured: it holds every pass.
+            if (!capturing && ((j + 1) % kRillPoll == 0 || j == max_pass - 1)) {
+                const int slot = (j / kRillPoll) & 1;
+                ICM_TRY(hipMemcpyAsync(h_n_active + slot, run_this, sizeof(unsigned),
                                        hipMemcpyDeviceToHost, st));
-                ICM_TRY(hipStreamSynchronize(st));
-                if (*h_n_active == 0) break;
+                ICM_TRY(hipEventRecord(poll_ev[slot], st));
+                if (pending >= 0) {
+                    ICM_TRY(hipEventSynchronize(poll_ev[pending]));
+                    if (h_n_active[pending] == 0) break;
+                }
+                pending = slot;
             }
         }
         if (stops[phase]) {
@@ -859,33 +1108,50 @@ struct Plan : PlanBase {
         return signal_end(st);
     }
 
+    // FIXED: always; RILLING: when both caps are at most kRillGraphCap passes
+    // (the graph then holds every pass up to the caps; passes after the last
+    // series stopped return at once)
+    static constexpr int kRillGraphCap = 64;
+    bool graph_mode() const
+    {
+        return cfg.stop_mode == ICM_STOP_FIXED ||
+               (cfg.max_sifts <= kRillGraphCap && cfg.noise_max_sifts <= kRillGraphCap);
+    }
+
     int capture_or_run(int which, hipStream_t st)
     {
         auto body = [&](hipStream_t s) -> int {
             return which == 0 ? noise_phase(s) : signal_phase(s);
         };
-        const bool graph = cfg.use_graph && cfg.stop_mode == ICM_STOP_FIXED && !trace_passes &&
-                           !prof && !graph_failed;
+        const bool graph = cfg.use_graph && graph_mode() && !trace_passes && !prof &&
+                           !graph_failed && !tuning;
         if (!graph) return body(st);
         if (!gexec[which]) {
             // capture on the plan's own stream, replay on the caller's
             const uint32_t e0 = epoch;
-            if (hipStreamBeginCapture(own, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            const hipError_t be = hipStreamBeginCapture(own, hipStreamCaptureModeThreadLocal);
+            if (be != hipSuccess) {
                 graph_failed = true;
+                (void)hip_fail(be, __LINE__);
                 return body(st);
             }
+            capturing = true;
             int rc = body(own);
+            capturing = false;
             hipGraph_t g = nullptr;
             hipError_t ce = hipStreamEndCapture(own, &g);
             if (rc != ICM_OK || ce != hipSuccess || !g) {
                 if (g) (void)hipGraphDestroy(g);
+                (void)hip_fail(ce != hipSuccess ? ce : hipErrorUnknown, __LINE__);
                 (void)hipGetLastError();
                 graph_failed = true;
                 epoch = e0;
                 return body(st);
             }
             hipGraphExec_t ex = nullptr;
-            if (hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess) {
+            const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            if (ie != hipSuccess) {
+                (void)hip_fail(ie, __LINE__);
                 (void)hipGraphDestroy(g);
                 (void)hipGetLastError();
                 graph_failed = true;
@@ -910,10 +1176,13 @@ struct Plan : PlanBase {
         // again only after this forward has completed
         have_forward = false;
         std::fill(krec_valid.begin(), krec_valid.end(), (char)0);
-        ICM_TRY(hipEventRecord(ev[0], st));
-        ICM_TRY(launch_fill_u32((uint32_t*)err, 0u, 1, st));
         for (int c = 0; c < C; ++c)
             if (!out[c]) return ICM_ERR_INVALID_ARG;
+        // a tuning forward times the sift candidates (eagerly, with events)
+        tuning = tune_left > 0;
+        tused = 0;
+        ICM_TRY(hipEventRecord(ev[0], st));
+        ICM_TRY(launch_fill_u32((uint32_t*)err, 0u, 1, st));
         int rc;
         if ((rc = inputs_and_noise(sig, noise, st))) return rc;
         if ((rc = reset_control(st))) return rc;
@@ -934,7 +1203,12 @@ struct Plan : PlanBase {
         timing.signal_ms = t12;
         timing.total_ms = t03;
         prof_collect();
-        if (*h_err) return ICM_ERR_DEVICE;
+        if (*h_err) {
+            tuning = false;
+            tused = 0;
+            return ICM_ERR_DEVICE;
+        }
+        if (tuning) tune_collect();
         have_forward = true;
         return ICM_OK;
     }
@@ -974,6 +1248,7 @@ struct Plan : PlanBase {
     // stages a few at a time, with the state between stages saveable
     int forward_begin(const void* const* sig, const void* noise, hipStream_t st) override
     {
+        tuning = false;
         *h_err = 0;
         have_forward = false;
         std::fill(krec_valid.begin(), krec_valid.end(), (char)0);
@@ -1079,6 +1354,7 @@ struct Plan : PlanBase {
     }
     int state_load(const void* host, int64_t bytes, const void* noise, hipStream_t st) override
     {
+        tuning = false;
         if (!host || bytes < state_bytes()) return ICM_ERR_INVALID_ARG;
         StateHeader h;
         std::memcpy(&h, host, sizeof h);
@@ -1154,6 +1430,7 @@ struct Plan : PlanBase {
     // Reverse mode over the signal phase of the last forward.
     int backward(const void* const* gimf, void* const* gsig, hipStream_t st)
     {
+        tuning = false;
         if (!have_forward || cfg.stop_mode != ICM_STOP_FIXED ||
             cfg.boundary != ICM_BOUNDARY_NATURAL)
             return ICM_ERR_STATE;
@@ -1260,6 +1537,7 @@ struct Plan : PlanBase {
     // one pass over the noise-phase series (I series of length N, back to back)
     int sift_once(const real* h, real* hn, int32_t* nk_out, hipStream_t st)
     {
+        tuning = false;
         int rc;
         ICM_TRY(launch_fill_u32((uint32_t*)err, 0u, 1, st));
         if ((rc = reset_control(st))) return rc;
@@ -1372,6 +1650,8 @@ int icm_plan_create(const icm_config* c, icm_plan* out)
 int icm_plan_destroy(icm_plan plan)
 {
     if (!plan) return ICM_ERR_INVALID_ARG;
+    // a plan destroyed before its last tuning forward leaves what it measured
+    if (plan->p->tune_left > 0 && plan->p->tune_round > 0) plan->p->tune_finish();
     plan->p->free_all();
     delete plan->p;
     delete plan;
@@ -1424,6 +1704,48 @@ int icm_get_kernel_stats(icm_plan plan, int kind, double* total_ms, int64_t* lau
     *total_ms = plan->p->kern_ms[kind];
     *launches = plan->p->kern_n[kind];
     *alg_bytes = plan->p->kern_bytes[kind];
+    return ICM_OK;
+}
+
+int icm_set_autotune(icm_plan plan, int forwards)
+{
+    if (!plan) return ICM_ERR_INVALID_ARG;
+    return plan->p->set_autotune(forwards);
+}
+
+int icm_get_tuning(icm_plan plan, int phase, int32_t* host_out, int64_t count)
+{
+    if (!plan) return ICM_ERR_INVALID_ARG;
+    return plan->p->get_tuning(phase, host_out, count);
+}
+
+int icm_set_tuning(icm_plan plan, int phase, const int32_t* host_in, int64_t count)
+{
+    if (!plan) return ICM_ERR_INVALID_ARG;
+    return plan->p->set_tuning(phase, host_in, count);
+}
+
+int icm_get_graph_status(icm_plan plan, int* status)
+{
+    if (!plan || !status) return ICM_ERR_INVALID_ARG;
+    const PlanBase& P = *plan->p;
+    if (P.graph_failed) *status = -1;
+    else *status = (P.gexec[0] && P.gexec[1]) ? 1 : 0;
+    return ICM_OK;
+}
+
+int icm_get_tuning_left(icm_plan plan, int* forwards)
+{
+    if (!plan || !forwards) return ICM_ERR_INVALID_ARG;
+    *forwards = plan->p->tune_left;
+    return ICM_OK;
+}
+
+int icm_tuning_candidate(int cand, int* segment_tiles, int* small_tables)
+{
+    if (cand < -1 || cand >= kNCand) return ICM_ERR_INVALID_ARG;
+    if (segment_tiles) *segment_tiles = cand < 0 ? 0 : kSegCand[cand / 2];
+    if (small_tables) *small_tables = cand < 0 ? -1 : (cand & 1);
     return ICM_OK;
 }
 

@@ -60,7 +60,8 @@ typedef struct icm_config {
     int noise_max_sifts;        /* the same for the noise-mode stages (reference: 2000), 1 .. 32000 */
     double noise_strength;      /* eps0 */
     uint64_t seed;              /* Philox key of the white noise */
-    int use_graph;              /* FIXED only: replay the forward as a captured hipGraph */
+    int use_graph;              /* replay the forward as a captured hipGraph (FIXED; RILLING with
+                                   both caps <= 64 passes) */
     int boundary;               /* ICM_BOUNDARY_*: spline end condition (envelopes with >= 4
                                    knots; fewer knots keep natural ends) */
     int keep_records;           /* FIXED: the forward keeps every pass's knot structure so that
@@ -117,6 +118,13 @@ int icm_forward(icm_plan plan, const void* const* d_signals, void* const* d_imfs
                 const void* d_noise, void* stream);
 int icm_last_timing(icm_plan plan, icm_timing* out);
 
+/* Graph state of the plan after its last forward: 1 = both phases replay a
+ * captured hipGraph; 0 = launched eagerly (use_graph off, or the forward was
+ * tuning, profiled or traced, or RILLING caps above 64 passes); -1 = a
+ * capture failed and the plan runs eagerly from then on (the reason is in
+ * icm_last_error()). */
+int icm_get_graph_status(icm_plan plan, int* status);
+
 /* Per-kernel timing with HIP events on the forward's stream (disables the
  * hipGraph replay while on).  kind: 0 = stage init (extrema of the stage
  * input), 1 = spline solve, 2 = sift pass.  Totals of the last forward;
@@ -127,6 +135,31 @@ int icm_last_timing(icm_plan plan, icm_timing* out);
 int icm_set_profiling(icm_plan plan, int on);
 int icm_get_kernel_stats(icm_plan plan, int kind, double* total_ms, int64_t* launches,
                          int64_t* alg_bytes);
+
+/* Autotuner of the sift pass (no reference counterpart: it replaces the
+ * reference's fixed launch geometry, cudaICEEMDAN.cu:767-830).  For each
+ * phase (noise modes, CEEMDAN) and stage it picks the sift's segment length
+ * (consecutive tiles per work-queue item: 4, 8, 16 or 32) and knot-table size
+ * (full tables, or small tables with a second full-table launch for the
+ * tiles that overflow them) by timing the candidates, in rotation over the
+ * passes, during the plan's next `forwards` icm_forward calls (HIP events on
+ * the forward's stream; those forwards run without the hipGraph, which is
+ * captured once tuning ends).  The choice is kept in a process-wide table
+ * keyed by the plan's shape and device, so later plans of the same shape
+ * start tuned.  Every candidate computes the same bits.  forwards = 0: the
+ * default geometry (segments of clamp(tiles / 56000, 1, 32) tiles, small
+ * tables from the second stage on).  New plans tune over 2 forwards, or over
+ * ICM_AUTOTUNE forwards when that environment variable is set (0: off). */
+int icm_set_autotune(icm_plan plan, int forwards);
+/* Forwards that will still time candidates (0: tuned or tuning off). */
+int icm_get_tuning_left(icm_plan plan, int* forwards);
+/* Candidate per stage of a phase (host_out[k], k < M-1; -1 = default). */
+int icm_get_tuning(icm_plan plan, int phase, int32_t* host_out, int64_t count);
+/* Sets every stage's candidate (-1 .. 7; count = M-1) and ends tuning. */
+int icm_set_tuning(icm_plan plan, int phase, const int32_t* host_in, int64_t count);
+/* Meaning of candidate `cand`: tiles per segment (0 = default) and small
+ * knot tables (1), full tables (0) or the default rule (-1). */
+int icm_tuning_candidate(int cand, int* segment_tiles, int* small_tables);
 
 /* Per-pass knot counts (debug/parity): enable before a forward with the
  * number of passes to keep per stage; read back [M-1][passes][series][2]

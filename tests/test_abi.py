@@ -96,6 +96,33 @@ def test_host_validation(icm):
         assert _cfg(icm, **ok) == 0, ok
 
 
+def test_full_128_channel_job_fits_one_plan(icm):
+    """The whole 128 x 244032, I = 500 forward is one plan (61M tiles, 1.56e10
+    samples per launch: grid-stride and work-queue kernels, no per-launch
+    work-item limit); it needs about 200 GB in float32, within one MI355X's
+    288 GB.  Phases of more than 2^28 tiles are refused (32-bit tile
+    indices)."""
+    full = dict(lengths=[244032] * 128, num_noise=500, num_imfs=16, dtype_code=0)
+    assert icm.validate_config(**full) == 0
+    b = icm.plan_bytes(**full)
+    assert 150e9 < b < 288e9, b
+    assert icm.validate_config(lengths=[244032] * 1200, num_noise=1000, num_imfs=4) == -1
+
+
+def test_tuning_candidates(icm):
+    """Candidate c: segments of 4 << (c / 2) tiles, small tables when c is
+    odd; -1 is the default geometry."""
+    seen = set()
+    for c in range(icm.NUM_TUNING_CANDIDATES):
+        seg, small = icm.tuning_candidate(c)
+        assert seg == 4 << (c // 2) and small == (c & 1)
+        seen.add((seg, small))
+    assert len(seen) == icm.NUM_TUNING_CANDIDATES
+    assert icm.tuning_candidate(-1) == (0, -1)
+    with pytest.raises(RuntimeError):
+        icm.tuning_candidate(icm.NUM_TUNING_CANDIDATES)
+
+
 def test_plan_bytes_estimate(icm):
     b = icm.plan_bytes(lengths=[244032], num_noise=500, num_imfs=16)
     # h ping-pong + 15 noise modes + residual, float64: about 18 GB plus knots
