@@ -344,7 +344,11 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the C2 / C4 sub-measurements and the graph replay")
     ap.add_argument("--c5", action="store_true", help="also time the C5 share forward + adjoint")
+    ap.add_argument("--forward-only", action="store_true",
+                    help="only the timed forwards of the workload (for profiler runs)")
     args = ap.parse_args()
+    if args.forward_only:
+        args.no_cpu = args.no_per_signal = args.no_extra = True
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -387,7 +391,7 @@ def main():
     if "roofline" in main_res:
         line["roofline"] = main_res["roofline"]
     line["autotune"] = main_res["tuning"]
-    if W["dtype"] == "f32":
+    if W["dtype"] == "f32" and not args.forward_only:
         line["fp32_vs_fp64"] = fp32_error(icm, torch, dev, name, xs, main_res["out"][0])
     del main_res
     torch.cuda.empty_cache()

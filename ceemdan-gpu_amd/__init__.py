@@ -32,8 +32,10 @@ class ICMError(RuntimeError):
     def __init__(self, status, what=""):
         self.status = status
         msg = lib().icm_status_string(status).decode()
-        if status == -3:
-            msg += f" [{lib().icm_last_error().decode()}]"
+        if status in (-2, -3):
+            detail = lib().icm_last_error().decode()
+            if detail:
+                msg += f" [{detail}]"
         super().__init__(f"{what}: {msg} ({status})" if what else f"{msg} ({status})")
 
 
@@ -78,6 +80,7 @@ _SIGS = {
     "icm_tuning_candidate": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "icm_get_tuning_left": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "icm_get_graph_status": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "icm_set_overlap": (C.c_int, [C.c_void_p, C.c_int]),
     "icm_set_trace": (C.c_int, [C.c_void_p, C.c_int]),
     "icm_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "icm_get_kernel_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double),
@@ -419,6 +422,11 @@ class Plan:
         """Time the sift candidates over the next `forwards` forwards (0: off,
         default geometry)."""
         _check(lib().icm_set_autotune(self._h, int(forwards)), "icm_set_autotune")
+
+    def set_overlap(self, on):
+        """FIXED mode: the noise-mode phase beside the CEEMDAN phase on a
+        second stream (default), or one after the other."""
+        _check(lib().icm_set_overlap(self._h, 1 if on else 0), "icm_set_overlap")
 
     def graph_status(self):
         """1: the phases replay captured hipGraphs; 0: eager; -1: a capture

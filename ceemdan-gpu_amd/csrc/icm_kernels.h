@@ -121,8 +121,9 @@ struct SiftArgs {
     // them (*dense, counted by the knot prefix); else full tables throughout
     const unsigned* dense;
     unsigned dense_max;
-    // work queue of the pass (zero before the launch): a resident grid of
-    // wavefronts takes segments from it one at a time until none is left
+    // work queue of the pass (zero before the launch; null: one wavefront
+    // per segment): a resident grid of wavefronts takes segments from it one
+    // at a time until none is left
     unsigned* queue;
 };
 

@@ -585,11 +585,16 @@ __device__ __forceinline__ void wave_sync()
 // knot glo, or -1 (64-ary search).  NIT >= the knots of one window / 64.
 // (inlined: as a called function it cost the solve 8 % and the adjoint
 // window solve 1 %)
+// row masks of one staged tile in LDS, padded from 4 to 5 words: the 64
+// lanes store their tiles' rows at a 40-byte stride instead of 32, which
+// spreads the stores over the banks
+constexpr int kTMaskW = kWR + 1;
+
 template <int NIT>
 __device__ __forceinline__ void gather_positions(const SeriesDev& sd, const uint64_t* kmask,
                                                  const int* kpre, const int* kcnt, int s, int type,
                                                  long lo, int glo, int ghi, int* sp, int* tpre,
-                                                 uint64_t (*tmask)[kWR], int hint)
+                                                 uint64_t (*tmask)[kTMaskW], int hint)
 {
     const int lane = threadIdx.x & 63;
     const int tb = load_scalar(sd.tile0 + s), ntl = load_scalar(sd.ntl + s);
@@ -777,7 +782,7 @@ __global__ __launch_bounds__(kThreads) void k_solve(SolveArgs<real> a)
     __shared__ int s_pos[NW][W + 2];
     __shared__ real s_y[NW][W + 2];
     __shared__ int s_tpre[NW][kWave];
-    __shared__ uint64_t s_tmask[NW][kWave][kWR];
+    __shared__ uint64_t s_tmask[NW][kWave][kTMaskW];
     const int wv = wave_index(), lane = threadIdx.x & 63;
     int* sp = s_pos[wv];
     real* sy = s_y[wv];
@@ -1285,15 +1290,19 @@ __device__ __forceinline__ void sift_segment(const SiftArgs<real>& a, const Segm
     }
 }
 
-// A resident grid of wavefronts working through the pass's segments: each
-// wave takes the next segment from the pass's queue (one atomic increment,
-// requested while the previous segment is sifted, so its latency overlaps
-// that segment's loads) until the queue is exhausted.  Segments differ in
-// cost (uneven lengths, knot counts and series ends), and a wave that drew
-// cheap ones simply takes more.  Without a queue (a.queue null) the grid has
-// one wave per segment.  The small-table instance is held to 96 registers,
-// so that five of its waves fit on a SIMD (its tables allow eight).
-template <typename real, bool RILL, int KC>
+// One wavefront per segment (QUEUE = false), or (QUEUE = true) a resident
+// grid of wavefronts working through the pass's segments: each wave takes
+// the next segment from the pass's queue (one atomic increment, requested
+// while the previous segment is sifted, so its latency overlaps that
+// segment's loads) until the queue is exhausted.  The queue instance keeps
+// the work of a launch independent of its grid (any number of segments);
+// its loop keeps ~15 more registers live than the one-segment body, so it is
+// used only where the one-wave-per-segment grid would exceed a launch's
+// 2^32 work-items (or when ICM_SIFT_QUEUE=1).  Workgroups of the
+// one-segment grid are dispatched as slots free up, which balances uneven
+// segments as well.  The small-table instances are held to 96 registers, so
+// that five of their waves fit on a SIMD (their tables allow eight).
+template <typename real, bool RILL, int KC, bool QUEUE>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(KC == kWKS ? 5 : 1)))
 void k_sift_w(SiftArgs<real> a, int nsegs)
 {
@@ -1305,7 +1314,7 @@ void k_sift_w(SiftArgs<real> a, int nsegs)
     // a small-table pass whose input has too many over-full tiles is left
     // to k_sift_list
     if (KC == kWKS && load_scalar(a.dense) > a.dense_max) return;
-    if (!a.queue) {
+    if (!QUEUE) {
         const int sid = blockIdx.x * NW + wv;
         if (sid < nsegs) sift_segment<real, RILL, KC>(a, a.segs, sid, lane, s_tp[wv], s_tc[wv]);
         return;
@@ -1722,7 +1731,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) v
     __shared__ real s_f[NW][W];            // knot-value gradients of the window rows
     __shared__ int s_pos[NW][W + 2];       // knot positions g0-1 .. g0+W
     __shared__ int s_tpre[NW][kWave];
-    __shared__ uint64_t s_tmask[NW][kWave][kWR];
+    __shared__ uint64_t s_tmask[NW][kWave][kTMaskW];
     const int wv = wave_index(), lane = threadIdx.x & 63;
     real* sf = s_f[wv];
     int* sp = s_pos[wv];
@@ -2233,17 +2242,24 @@ static dim3 sift_grid(K kernel, const unsigned* queue, int nsegs)
     return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(need, resident_groups(kernel))));
 }
 
+template <typename real, bool RILL, int KC>
+static void launch_sift_w(const SiftArgs<real>& a, int nsegs, hipStream_t st)
+{
+    if (a.queue) {
+        auto k = k_sift_w<real, RILL, KC, true>;
+        hipLaunchKernelGGL(k, sift_grid(k, a.queue, nsegs), dim3(kThreads), 0, st, a, nsegs);
+    } else {
+        auto k = k_sift_w<real, RILL, KC, false>;
+        hipLaunchKernelGGL(k, sift_grid(k, a.queue, nsegs), dim3(kThreads), 0, st, a, nsegs);
+    }
+}
+
 template <typename real>
 hipError_t launch_sift(const SiftArgs<real>& a, int nsegs, hipStream_t st)
 {
     if (!a.ovf_segs) {
-        if (a.sx_cnt) {
-            auto k = k_sift_w<real, true, kWKC>;
-            hipLaunchKernelGGL(k, sift_grid(k, a.queue, nsegs), dim3(kThreads), 0, st, a, nsegs);
-        } else {
-            auto k = k_sift_w<real, false, kWKC>;
-            hipLaunchKernelGGL(k, sift_grid(k, a.queue, nsegs), dim3(kThreads), 0, st, a, nsegs);
-        }
+        if (a.sx_cnt) launch_sift_w<real, true, kWKC>(a, nsegs, st);
+        else launch_sift_w<real, false, kWKC>(a, nsegs, st);
         return hipGetLastError();
     }
     // sparse pass: if few tiles of the input overflow the small knot tables
@@ -2254,12 +2270,10 @@ hipError_t launch_sift(const SiftArgs<real>& a, int nsegs, hipStream_t st)
     // workgroups per CU), so it costs little when it has nothing to do.
     const dim3 lgrid(std::min<unsigned>(blocks_for(nsegs, kThreads / kWave), 1024u));
     if (a.sx_cnt) {
-        auto k = k_sift_w<real, true, kWKS>;
-        hipLaunchKernelGGL(k, sift_grid(k, a.queue, nsegs), dim3(kThreads), 0, st, a, nsegs);
+        launch_sift_w<real, true, kWKS>(a, nsegs, st);
         hipLaunchKernelGGL((k_sift_list<real, true>), lgrid, dim3(kThreads), 0, st, a, nsegs);
     } else {
-        auto k = k_sift_w<real, false, kWKS>;
-        hipLaunchKernelGGL(k, sift_grid(k, a.queue, nsegs), dim3(kThreads), 0, st, a, nsegs);
+        launch_sift_w<real, false, kWKS>(a, nsegs, st);
         hipLaunchKernelGGL((k_sift_list<real, false>), lgrid, dim3(kThreads), 0, st, a, nsegs);
     }
     return hipGetLastError();

@@ -248,6 +248,13 @@ struct PlanBase {
     unsigned* h_n_active = nullptr;   // [2] pinned copies read by the host (one per poll slot)
     hipEvent_t poll_ev[2] = {nullptr, nullptr};
     bool capturing = false;           // a phase is being captured into a graph
+    // the two phases side by side: the noise-mode phase's stream, the
+    // events after each of its stages, the fork and the CEEMDAN phase's start
+    hipStream_t aux = nullptr;
+    std::vector<hipEvent_t> stage_ev;
+    hipEvent_t ev_fork = nullptr, ev_sig = nullptr;
+    bool overlap_on = true;
+    bool sift_queue = false;          // every sift pass from a work queue (ICM_SIFT_QUEUE=1)
     int* h_err = nullptr;             // pinned
     // rilling
     int *active = nullptr, *stop_pass = nullptr, *subs = nullptr, *sx_cnt = nullptr,
@@ -260,8 +267,8 @@ struct PlanBase {
     // timing
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     icm_timing timing{};
-    // graphs
-    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    // graphs: [0] noise-mode phase, [1] CEEMDAN phase, [2] both side by side
+    hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
     bool graph_failed = false;
     uint32_t epoch = 0;
     int64_t* d_chan_off = nullptr;
@@ -359,7 +366,7 @@ struct PlanBase {
     }
     void drop_graphs()
     {
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < 3; ++i)
             if (gexec[i]) {
                 (void)hipGraphExecDestroy(gexec[i]);
                 gexec[i] = nullptr;
@@ -527,8 +534,13 @@ struct PlanBase {
     }
     void free_all()
     {
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < 3; ++i)
             if (gexec[i]) (void)hipGraphExecDestroy(gexec[i]);
+        for (auto& e : stage_ev)
+            if (e) (void)hipEventDestroy(e);
+        if (ev_sig) (void)hipEventDestroy(ev_sig);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (aux) (void)hipStreamDestroy(aux);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         for (auto& e : poll_ev)
@@ -606,15 +618,18 @@ int64_t plan_bytes_estimate(const icm_config* c)
     const int64_t s = c->dtype == ICM_DTYPE_F64 ? 8 : 4;
     const int M = c->num_imfs;
     int64_t b = 0;
-    b += 2 * P.samples_max * s;                    // h ping-pong
+    for (const HostSeries* h : {&P.hn, &P.hsig}) {  // pass state of each phase's lane
+        const int64_t nt = h->ntiles, ns = (int64_t)h->len.size();
+        b += 2 * h->samples * s;                    // h ping-pong
+        b += 2 * h->kcap * s;                       // c
+        b += 2 * (2 * 2 * nt * 4 + 2 * ns * 4 + 4 * ns * s);   // counts, prefixes, n, fix
+        b += 2 * nt * (6 * (4 + s) + 8 * 8);        // neighbour knots, knot bitmasks
+        b += (int64_t)kWinItem * h->maxwin * 4;     // window list
+        b += 4 * (int64_t)kEpochs * 4;              // control words
+        b += nt * (int64_t)sizeof(Segment);         // sparse-pass tile list
+    }
     b += (int64_t)(M - 1) * P.hn.samples * s;      // noise modes
     b += 2 * P.hn.samples * s;                     // noise residuals (two buffers)
-    b += 2 * P.kstride * s;                        // c
-    b += 2 * 2 * 2 * P.ntiles_max * 4 + 2 * 2 * P.nser_max * (4 + 2 * s);
-    b += 2 * P.ntiles_max * (6 * (4 + s) + 8 * 8);  // neighbour knots, knot bitmasks
-    b += (int64_t)kEpochs * 4 + (int64_t)kWinItem * P.maxwin * 4;
-    b += 2 * (int64_t)kEpochs * 4 + P.ntiles_max * (int64_t)sizeof(Segment);  // sparse-pass tile lists
-    b += (int64_t)kEpochs * 4;                                                // work-queue counters
     for (int l = 0; l < kNLists; ++l)                                         // segment lists
         b += (int64_t)(P.hn.segs[l].size() + P.hsig.segs[l].size()) * (int64_t)sizeof(Segment);
     b += 5 * P.chan_total * s + P.chan_total * M * s;
@@ -630,15 +645,32 @@ int64_t plan_bytes_estimate(const icm_config* c)
 
 template <typename real>
 struct Plan : PlanBase {
-    real *hA = nullptr, *hB = nullptr, *modes = nullptr, *wres = nullptr, *wres2 = nullptr;
-    int* kcnt[2] = {nullptr, nullptr};
-    int* kpre[2] = {nullptr, nullptr};
-    int* nk[2] = {nullptr, nullptr};
-    real* kfix[2] = {nullptr, nullptr};
-    int* knbx[2] = {nullptr, nullptr};
-    real* knby[2] = {nullptr, nullptr};
-    uint64_t* kmask[2] = {nullptr, nullptr};
-    real* kc = nullptr;
+    // Pass state of one phase: the samples' ping-pong buffers, the knots of
+    // a pass's input and output, the spline coefficients c, the solve's
+    // window list and the per-pass control words (indexed by the lane's
+    // epoch, cleared by fill kernels).  The noise-mode phase and the CEEMDAN
+    // phase have one each, so that they can run side by side.
+    struct Lane {
+        real *hA = nullptr, *hB = nullptr;
+        int* kcnt[2] = {};
+        int* kpre[2] = {};
+        int* nk[2] = {};
+        real* kfix[2] = {};
+        int* knbx[2] = {};
+        real* knby[2] = {};
+        uint64_t* kmask[2] = {};
+        real* kc = nullptr;                // [2][kstride]
+        int64_t kstride = 0;               // per-type stride of kc (the phase's knot capacity)
+        int* winlist = nullptr;            // [kWinItem * maxwin] window items of the next solve
+        unsigned* wincount = nullptr;      // [kEpochs] their number per pass
+        unsigned* ovfcount = nullptr;      // [kEpochs] tiles a sparse pass left to the full tables
+        unsigned* densecnt = nullptr;      // [kEpochs] tiles of a pass input over the small tables
+        unsigned* squeue = nullptr;        // [kEpochs] work-queue counter of each sift launch
+        Segment* ovfsegs = nullptr;        // [ntiles] the tiles a sparse pass left
+        uint32_t epoch = 0;
+    };
+    Lane ln[2];                            // [ICM_PHASE_NOISE], [ICM_PHASE_SIGNAL]
+    real *modes = nullptr, *wres = nullptr, *wres2 = nullptr;
     real *y = nullptr, *xt = nullptr, *r = nullptr, *imf = nullptr;
     real *var_chan = nullptr, *var_noise = nullptr;
     double *inv_noise = nullptr, *sq_chan = nullptr;
@@ -653,10 +685,6 @@ struct Plan : PlanBase {
         int* winlist = nullptr;
         unsigned* wincount = nullptr;
     };
-    Segment* ovfsegs = nullptr;        // [ntiles] tiles a sparse sift pass left to the full tables
-    unsigned* ovfcount = nullptr;      // [kEpochs] their number per pass
-    unsigned* densecnt = nullptr;      // [kEpochs] tiles of a pass input that overflow small tables
-    unsigned* squeue = nullptr;        // [kEpochs] work-queue counter of each sift launch
     int sparse_stage = 1;              // first stage of a phase sifted with small knot tables
     int64_t sparse_dense_max = -1;     // over-full tiles tolerated by them (-1: tiles / 64)
     std::vector<Rec> rec;                  // re-sift records of one stage (backward)
@@ -671,25 +699,11 @@ struct Plan : PlanBase {
         const int M = cfg.num_imfs;
         if ((rc = upload_series(dn, hn))) return rc;
         if ((rc = upload_series(dsig, hsig))) return rc;
-        if ((rc = alloc(&hA, samples_max))) return rc;
-        if ((rc = alloc(&hB, samples_max))) return rc;
+        if ((rc = alloc_lane(ln[ICM_PHASE_NOISE], hn))) return rc;
+        if ((rc = alloc_lane(ln[ICM_PHASE_SIGNAL], hsig))) return rc;
         if ((rc = alloc(&modes, (int64_t)(M - 1) * hn.samples))) return rc;
         if ((rc = alloc(&wres, hn.samples))) return rc;
         if ((rc = alloc(&wres2, hn.samples))) return rc;
-        for (int b = 0; b < 2; ++b) {
-            if ((rc = alloc(&kcnt[b], 2 * ntiles_max))) return rc;
-            if ((rc = alloc(&kpre[b], 2 * ntiles_max))) return rc;
-            if ((rc = alloc(&nk[b], 2 * nser_max))) return rc;
-            if ((rc = alloc(&kfix[b], 4 * nser_max))) return rc;
-            if ((rc = alloc(&knbx[b], 6 * ntiles_max))) return rc;
-            if ((rc = alloc(&knby[b], 6 * ntiles_max))) return rc;
-            if ((rc = alloc(&kmask[b], 8 * ntiles_max))) return rc;
-        }
-        if ((rc = alloc(&kc, 2 * kstride))) return rc;
-        if ((rc = alloc(&wincount, kEpochs))) return rc;
-        if ((rc = alloc(&ovfcount, kEpochs))) return rc;
-        if ((rc = alloc(&densecnt, kEpochs))) return rc;
-        if ((rc = alloc(&squeue, kEpochs))) return rc;
         // ICM_SPARSE_STAGE: first stage sifted with small knot tables (a
         // tuning knob; 0 also sends the dense first stage through them, and
         // most of its tiles through the full-table launch)
@@ -699,8 +713,6 @@ struct Plan : PlanBase {
         // before it runs with full tables throughout (default tiles / 64)
         if (const char* env = std::getenv("ICM_SPARSE_DENSE_MAX"))
             if (std::atoll(env) >= 0) sparse_dense_max = std::atoll(env);
-        if ((rc = alloc(&ovfsegs, ntiles_max))) return rc;
-        if ((rc = alloc(&winlist, (int64_t)kWinItem * maxwin))) return rc;
         if ((rc = alloc(&err, 1))) return rc;
         if ((rc = alloc(&n_active, 2))) return rc;
         if ((rc = alloc(&y, chan_total))) return rc;
@@ -739,32 +751,69 @@ struct Plan : PlanBase {
             if (hipEventCreate(&e) != hipSuccess) return ICM_ERR_HIP;
         for (auto& e : poll_ev)
             if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return ICM_ERR_HIP;
+        if (hipEventCreate(&ev_sig) != hipSuccess) return ICM_ERR_HIP;
+        if (hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming) != hipSuccess) return ICM_ERR_HIP;
+        stage_ev.assign(M - 1, nullptr);
+        for (auto& e : stage_ev)
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return ICM_ERR_HIP;
+        if (hipStreamCreateWithFlags(&aux, hipStreamNonBlocking) != hipSuccess) return ICM_ERR_HIP;
+        // ICM_OVERLAP=0 runs the two phases one after the other
+        if (const char* env = std::getenv("ICM_OVERLAP")) overlap_on = std::atoi(env) != 0;
+        // ICM_SIFT_QUEUE=1: every sift pass from the work queue on a resident
+        // grid (else only passes whose one-wave-per-segment grid is too large)
+        if (const char* env = std::getenv("ICM_SIFT_QUEUE")) sift_queue = std::atoi(env) != 0;
         tune_init();
         return ICM_OK;
     }
 
-    PassCtl ctl(uint32_t e)
+    int alloc_lane(Lane& L, const HostSeries& h)
+    {
+        int rc;
+        const int64_t nt = h.ntiles, ns = (int64_t)h.len.size();
+        if ((rc = alloc(&L.hA, h.samples))) return rc;
+        if ((rc = alloc(&L.hB, h.samples))) return rc;
+        for (int b = 0; b < 2; ++b) {
+            if ((rc = alloc(&L.kcnt[b], 2 * nt))) return rc;
+            if ((rc = alloc(&L.kpre[b], 2 * nt))) return rc;
+            if ((rc = alloc(&L.nk[b], 2 * ns))) return rc;
+            if ((rc = alloc(&L.kfix[b], 4 * ns))) return rc;
+            if ((rc = alloc(&L.knbx[b], 6 * nt))) return rc;
+            if ((rc = alloc(&L.knby[b], 6 * nt))) return rc;
+            if ((rc = alloc(&L.kmask[b], 8 * nt))) return rc;
+        }
+        L.kstride = h.kcap;
+        if ((rc = alloc(&L.kc, 2 * h.kcap))) return rc;
+        if ((rc = alloc(&L.winlist, (int64_t)kWinItem * h.maxwin))) return rc;
+        if ((rc = alloc(&L.wincount, kEpochs))) return rc;
+        if ((rc = alloc(&L.ovfcount, kEpochs))) return rc;
+        if ((rc = alloc(&L.densecnt, kEpochs))) return rc;
+        if ((rc = alloc(&L.squeue, kEpochs))) return rc;
+        if ((rc = alloc(&L.ovfsegs, nt))) return rc;
+        return ICM_OK;
+    }
+
+    PassCtl ctl(const Lane& L, uint32_t e)
     {
         PassCtl pc{};
         pc.epoch = e;
-        pc.winlist = winlist;
-        pc.wincount = wincount;
+        pc.winlist = L.winlist;
+        pc.wincount = L.wincount;
         pc.K = solve_K(cfg.dtype);
         pc.H = solve_H(cfg.dtype);
         pc.err = err;
-        pc.dense = densecnt;
+        pc.dense = L.densecnt;
         return pc;
     }
-    KnotsDev<real> knots(int b)
+    KnotsDev<real> knots(const Lane& L, int b)
     {
         KnotsDev<real> k{};
-        k.cnt = kcnt[b];
-        k.pre = kpre[b];
-        k.n = nk[b];
-        k.fix = kfix[b];
-        k.nbx = knbx[b];
-        k.nby = knby[b];
-        k.mask = kmask[b];
+        k.cnt = L.kcnt[b];
+        k.pre = L.kpre[b];
+        k.n = L.nk[b];
+        k.fix = L.kfix[b];
+        k.nbx = L.knbx[b];
+        k.nby = L.knby[b];
+        k.mask = L.kmask[b];
         return k;
     }
 
@@ -776,16 +825,22 @@ struct Plan : PlanBase {
         if (e_ != hipSuccess) return hip_fail(e_, __LINE__); \
     } while (0)
 
-    int reset_control(hipStream_t st)
+    int reset_control(Lane& L, hipStream_t st)
     {
-        ICM_TRY(launch_fill_u32((uint32_t*)wincount, 0u, kEpochs, st));
-        ICM_TRY(launch_fill_u32((uint32_t*)ovfcount, 0u, kEpochs, st));
-        ICM_TRY(launch_fill_u32((uint32_t*)densecnt, 0u, kEpochs, st));
-        ICM_TRY(launch_fill_u32((uint32_t*)squeue, 0u, kEpochs, st));
-        epoch = 0;
+        ICM_TRY(launch_fill_u32((uint32_t*)L.wincount, 0u, kEpochs, st));
+        ICM_TRY(launch_fill_u32((uint32_t*)L.ovfcount, 0u, kEpochs, st));
+        ICM_TRY(launch_fill_u32((uint32_t*)L.densecnt, 0u, kEpochs, st));
+        ICM_TRY(launch_fill_u32((uint32_t*)L.squeue, 0u, kEpochs, st));
+        L.epoch = 0;
         return ICM_OK;
     }
-    uint32_t next_epoch() { return ++epoch; }
+    int reset_controls(hipStream_t st)
+    {
+        int rc;
+        if ((rc = reset_control(ln[0], st))) return rc;
+        return reset_control(ln[1], st);
+    }
+    static uint32_t next_epoch(Lane& L) { return ++L.epoch; }
 
     // Sift passes after the first stage of a phase meet a few tens of knots
     // per 256-sample tile: they run with small knot tables (more waves per
@@ -793,11 +848,11 @@ struct Plan : PlanBase {
     // launch, unless more than 1/64 of the input's tiles (counted by the knot
     // prefix of epoch e_in) have more knots: then with full tables.  The
     // result is the same either way.
-    void sparse_pass(SiftArgs<real>& a, const DevSeries& d, uint32_t e_in, uint32_t e)
+    void sparse_pass(SiftArgs<real>& a, const Lane& L, const DevSeries& d, uint32_t e_in, uint32_t e)
     {
-        a.ovf_segs = ovfsegs;
-        a.ovf_count = ovfcount + e;
-        a.dense = densecnt + e_in;
+        a.ovf_segs = L.ovfsegs;
+        a.ovf_count = L.ovfcount + e;
+        a.dense = L.densecnt + e_in;
         a.dense_max = sparse_dense_max >= 0
                           ? (unsigned)std::min<int64_t>(sparse_dense_max, 0xFFFFFFFFll)
                           : (unsigned)(d.ntiles / 64);
@@ -812,9 +867,13 @@ struct Plan : PlanBase {
         const int cand = tune_candidate(phase, stage, pass);
         const int l = cand < 0 ? 0 : 1 + cand / 2;
         const bool small = cand < 0 ? stage >= sparse_stage : (cand & 1) != 0;
+        const Lane& L = ln[phase];
         a.segs = d.segs[l];
-        a.queue = squeue + e;
-        if (small) sparse_pass(a, d, e_in, e);
+        // one wavefront per segment unless that grid would exceed a launch's
+        // 2^32 work-items: then the work-queue instance on a resident grid
+        const bool queue = sift_queue || (int64_t)d.nsegs[l] * 64 >= ((int64_t)1 << 32);
+        a.queue = queue ? L.squeue + e : nullptr;
+        if (small) sparse_pass(a, L, d, e_in, e);
         int rc;
         if ((rc = prof_begin(2, bytes, st))) return rc;
         if ((rc = tune_begin(phase, stage, cand, st))) return rc;
@@ -828,9 +887,10 @@ struct Plan : PlanBase {
         if (!trace_passes || pass >= trace_passes || !trace[phase]) return ICM_OK;
         const int64_t n = 2 * (int64_t)d.nseries;
         int32_t* dst = trace[phase] + ((int64_t)stage * trace_passes + pass) * n;
-        ICM_TRY(launch_copy<float>((float*)dst, (const float*)nk[b], n, st));
+        ICM_TRY(launch_copy<float>((float*)dst, (const float*)ln[phase].nk[b], n, st));
         uint64_t* dh = trace_hash[phase] + ((int64_t)stage * trace_passes + pass) * d.nseries;
-        ICM_TRY(launch_knot_digest(d.tile_series, d.tile0, kmask[b], d.ntiles, dh, d.nseries, st));
+        ICM_TRY(launch_knot_digest(d.tile_series, d.tile0, ln[phase].kmask[b], d.ntiles, dh,
+                                   d.nseries, st));
         return ICM_OK;
     }
 
@@ -840,25 +900,26 @@ struct Plan : PlanBase {
     int stage_fixed(int phase, int stage, const DevSeries& d, const InitArgs<real>& ia, int S,
                     real* out, hipStream_t st, Rec* record = nullptr, bool full = false)
     {
-        const SeriesDev sd = d.view(kstride, cap);
+        Lane& L = ln[phase];
+        const SeriesDev sd = d.view(L.kstride, cap);
         int rc;
-        if (epoch + (uint32_t)S + 8 >= kEpochs - 1 && (rc = reset_control(st))) return rc;
+        if (L.epoch + (uint32_t)S + 8 >= kEpochs - 1 && (rc = reset_control(L, st))) return rc;
         const int64_t pass_bytes = 2 * (int64_t)sizeof(real) * d.samples;
-        uint32_t e_emit = next_epoch();
+        uint32_t e_emit = next_epoch(L);
         if ((rc = prof_begin(0, pass_bytes, st))) return rc;
-        ICM_TRY(launch_init<real>(sd, ia, hA, knots(0), ctl(e_emit), st));
+        ICM_TRY(launch_init<real>(sd, ia, L.hA, knots(L, 0), ctl(L, e_emit), st));
         if ((rc = prof_end(st))) return rc;
-        ICM_TRY(launch_knot_prefix<real>(sd, knots(0), hA, ctl(e_emit), nullptr, st));
+        ICM_TRY(launch_knot_prefix<real>(sd, knots(L, 0), L.hA, ctl(L, e_emit), nullptr, st));
         if ((rc = trace_copy(phase, stage, 0, d, 0, st))) return rc;
         for (int j = 0; j < S; ++j) {
             const int b = j & 1;
             SolveArgs<real> sa{};
             sa.sd = sd;
-            sa.k = knots(b);
-            sa.h = b ? hB : hA;
-            sa.c = kc;
-            sa.winlist = winlist;
-            sa.wincount = wincount + e_emit;
+            sa.k = knots(L, b);
+            sa.h = b ? L.hB : L.hA;
+            sa.c = L.kc;
+            sa.winlist = L.winlist;
+            sa.wincount = L.wincount + e_emit;
             sa.active = nullptr;
             sa.err = err;
             sa.nak = cfg.boundary == ICM_BOUNDARY_NOT_A_KNOT;
@@ -866,14 +927,14 @@ struct Plan : PlanBase {
                 // structure of pass j's input for the adjoint
                 const Rec& R = record[j];
                 sa.fixf = R.fixf;
-                ICM_TRY(launch_copy<float>((float*)R.mask, (const float*)kmask[b], 16 * (int64_t)d.ntiles, st));
-                ICM_TRY(launch_rec_left(knbx[b], R.left, R.right, d.ntiles, st));
-                ICM_TRY(launch_copy<float>((float*)R.cnt, (const float*)kcnt[b], 2 * (int64_t)d.ntiles, st));
-                ICM_TRY(launch_copy<float>((float*)R.pre, (const float*)kpre[b], 2 * (int64_t)d.ntiles, st));
-                ICM_TRY(launch_copy<float>((float*)R.n, (const float*)nk[b], 2 * (int64_t)d.nseries, st));
-                ICM_TRY(launch_copy<float>((float*)R.winlist, (const float*)winlist,
+                ICM_TRY(launch_copy<float>((float*)R.mask, (const float*)L.kmask[b], 16 * (int64_t)d.ntiles, st));
+                ICM_TRY(launch_rec_left(L.knbx[b], R.left, R.right, d.ntiles, st));
+                ICM_TRY(launch_copy<float>((float*)R.cnt, (const float*)L.kcnt[b], 2 * (int64_t)d.ntiles, st));
+                ICM_TRY(launch_copy<float>((float*)R.pre, (const float*)L.kpre[b], 2 * (int64_t)d.ntiles, st));
+                ICM_TRY(launch_copy<float>((float*)R.n, (const float*)L.nk[b], 2 * (int64_t)d.nseries, st));
+                ICM_TRY(launch_copy<float>((float*)R.winlist, (const float*)L.winlist,
                                            (int64_t)kWinItem * d.maxwin, st));
-                ICM_TRY(launch_copy<float>((float*)R.wincount, (const float*)(wincount + e_emit), 1, st));
+                ICM_TRY(launch_copy<float>((float*)R.wincount, (const float*)(L.wincount + e_emit), 1, st));
             }
             if ((rc = prof_begin(1, 0, st))) return rc;
             ICM_TRY(launch_solve<real>(sa, solve_grid(), st));
@@ -882,21 +943,21 @@ struct Plan : PlanBase {
             if (record && !full && j == S - 1) break;
             SiftArgs<real> a{};
             a.sd = sd;
-            a.h_in = b ? hB : hA;
-            a.h_out = (j == S - 1 && out) ? out : (b ? hA : hB);
-            a.kin = knots(b);
-            a.c = kc;
-            a.kout = knots(b ^ 1);
+            a.h_in = b ? L.hB : L.hA;
+            a.h_out = (j == S - 1 && out) ? out : (b ? L.hA : L.hB);
+            a.kin = knots(L, b);
+            a.c = L.kc;
+            a.kout = knots(L, b ^ 1);
             a.emit = (j < S - 1) ? 1 : 0;
             const uint32_t e_in = e_emit;
-            e_emit = next_epoch();
-            a.pc = ctl(e_emit);
+            e_emit = next_epoch(L);
+            a.pc = ctl(L, e_emit);
             a.active = nullptr;
             a.sx_cnt = nullptr;
             a.sx_mark = nullptr;
             if ((rc = sift_launch(a, phase, stage, j, d, e_in, e_emit, pass_bytes, st))) return rc;
             if (j < S - 1) {
-                ICM_TRY(launch_knot_prefix<real>(sd, knots(b ^ 1), a.h_out, ctl(e_emit), nullptr, st));
+                ICM_TRY(launch_knot_prefix<real>(sd, knots(L, b ^ 1), a.h_out, ctl(L, e_emit), nullptr, st));
                 if ((rc = trace_copy(phase, stage, j + 1, d, b ^ 1, st))) return rc;
             }
         }
@@ -918,19 +979,20 @@ struct Plan : PlanBase {
     int stage_rilling(int phase, int stage, const DevSeries& d, const InitArgs<real>& ia, int max_pass,
                       hipStream_t st)
     {
-        const SeriesDev sd = d.view(kstride, cap);
+        Lane& L = ln[phase];
+        const SeriesDev sd = d.view(L.kstride, cap);
         ICM_TRY(launch_fill_u32((uint32_t*)active, 1u, d.nseries, st));
         ICM_TRY(launch_fill_u32((uint32_t*)subs, 0u, d.nseries, st));
         ICM_TRY(launch_fill_u32((uint32_t*)stop_pass, 0u, d.nseries, st));
         // before pass 0 every series sifts
         ICM_TRY(launch_fill_u32((uint32_t*)n_active + 1, (uint32_t)d.nseries, 1, st));
-        if (epoch + 2 * (uint32_t)max_pass + 8 >= kEpochs - 1) {
-            int rc = reset_control(st);
+        if (L.epoch + 2 * (uint32_t)max_pass + 8 >= kEpochs - 1) {
+            int rc = reset_control(L, st);
             if (rc) return rc;
         }
-        uint32_t e_emit = next_epoch();
-        ICM_TRY(launch_init<real>(sd, ia, hA, knots(0), ctl(e_emit), st));
-        ICM_TRY(launch_knot_prefix<real>(sd, knots(0), hA, ctl(e_emit), nullptr, st));
+        uint32_t e_emit = next_epoch(L);
+        ICM_TRY(launch_init<real>(sd, ia, L.hA, knots(L, 0), ctl(L, e_emit), st));
+        ICM_TRY(launch_knot_prefix<real>(sd, knots(L, 0), L.hA, ctl(L, e_emit), nullptr, st));
         int rc;
         if ((rc = trace_copy(phase, stage, 0, d, 0, st))) return rc;
         int pending = -1;   // poll slot whose count the host has not read yet
@@ -940,11 +1002,11 @@ struct Plan : PlanBase {
             unsigned* run_this = n_active + (j & 1);
             SolveArgs<real> sa{};
             sa.sd = sd;
-            sa.k = knots(b);
-            sa.h = b ? hB : hA;
-            sa.c = kc;
-            sa.winlist = winlist;
-            sa.wincount = wincount + e_emit;
+            sa.k = knots(L, b);
+            sa.h = b ? L.hB : L.hA;
+            sa.c = L.kc;
+            sa.winlist = L.winlist;
+            sa.wincount = L.wincount + e_emit;
             sa.active = active;
             sa.err = err;
             sa.nak = cfg.boundary == ICM_BOUNDARY_NOT_A_KNOT;
@@ -952,15 +1014,15 @@ struct Plan : PlanBase {
             ICM_TRY(launch_solve<real>(sa, solve_grid(), st));
             SiftArgs<real> a{};
             a.sd = sd;
-            a.h_in = b ? hB : hA;
-            a.h_out = b ? hA : hB;
-            a.kin = knots(b);
-            a.c = kc;
-            a.kout = knots(b ^ 1);
+            a.h_in = b ? L.hB : L.hA;
+            a.h_out = b ? L.hA : L.hB;
+            a.kin = knots(L, b);
+            a.c = L.kc;
+            a.kout = knots(L, b ^ 1);
             a.emit = 1;
             const uint32_t e_in = e_emit;
-            e_emit = next_epoch();
-            a.pc = ctl(e_emit);
+            e_emit = next_epoch(L);
+            a.pc = ctl(L, e_emit);
             a.pc.run = run_prev;
             a.active = active;
             a.sx_cnt = sx_cnt;
@@ -969,11 +1031,11 @@ struct Plan : PlanBase {
                                   2 * (int64_t)sizeof(real) * d.samples, st)))
                 return rc;
             ICM_TRY(launch_fill_u32((uint32_t*)run_this, 0u, 1, st));
-            ICM_TRY(launch_rilling_decide<real>(sd, nk[b], sx_cnt, sx_mark, active, stop_pass, subs,
+            ICM_TRY(launch_rilling_decide<real>(sd, L.nk[b], sx_cnt, sx_mark, active, stop_pass, subs,
                                                 j, max_pass, run_prev, run_this, st));
-            PassCtl pn = ctl(e_emit);
+            PassCtl pn = ctl(L, e_emit);
             pn.run = run_this;
-            ICM_TRY(launch_knot_prefix<real>(sd, knots(b ^ 1), a.h_out, pn, active, st));
+            ICM_TRY(launch_knot_prefix<real>(sd, knots(L, b ^ 1), a.h_out, pn, active, st));
             if ((rc = trace_copy(phase, stage, j + 1, d, b ^ 1, st))) return rc;
             // early exit: results are identical to running every pass to the
             // cap with finished series masked (they never change again).
@@ -997,8 +1059,9 @@ struct Plan : PlanBase {
         return ICM_OK;
     }
 
-    // Noise phase: E_{k+1}(w^i) for k = 0..M-2 (cudaICEEMDAN.cu:845-994)
-    int noise_phase(hipStream_t st)
+    // Noise phase: E_{k+1}(w^i) for k = 0..M-2 (cudaICEEMDAN.cu:845-994).
+    // With `done`, the event done[k] is recorded after stage k.
+    int noise_phase(hipStream_t st, hipEvent_t* done = nullptr)
     {
         const int M = cfg.num_imfs;
         int rc;
@@ -1024,9 +1087,11 @@ struct Plan : PlanBase {
             } else {
                 if ((rc = stage_rilling(ICM_PHASE_NOISE, k, dn, ia, cfg.noise_max_sifts, st)))
                     return rc;
-                ICM_TRY(launch_select_copy<real>(dn.view(kstride, cap), dn.max_len, hA, hB, stop_pass,
+                ICM_TRY(launch_select_copy<real>(dn.view(ln[0].kstride, cap), dn.max_len, ln[0].hA,
+                                                 ln[0].hB, stop_pass,
                                                  Ek, st));
             }
+            if (done) ICM_TRY(hipEventRecord(done[k], st));
         }
         return ICM_OK;
     }
@@ -1081,7 +1146,8 @@ struct Plan : PlanBase {
             parity = stop_pass;
         }
         ICM_TRY(launch_ensemble<real>(k, C, I, max_chan_len, d_chan_len, d_chan_off, dsig.off,
-                                      dsig.eoff, dsig.nidx, hA, hB, parity, fixed_parity, Ek,
+                                      dsig.eoff, dsig.nidx, ln[1].hA, ln[1].hB, parity,
+                                      fixed_parity, Ek,
                                       inv_noise, sq_chan, eps0, xt, r, imf, d_chan_imf_off, st));
         // residues kept for the adjoint (r_0 = x~ is xt)
         if (k + 1 <= M - 2)
@@ -1099,13 +1165,40 @@ struct Plan : PlanBase {
         return ICM_OK;
     }
 
-    int signal_phase(hipStream_t st)
+    // With `ready`, the stage-0 statistics wait for ready[0] and stage k for
+    // ready[k] (the noise mode E_{k+1} it reads).
+    int signal_phase(hipStream_t st, hipEvent_t* ready = nullptr)
     {
         int rc;
+        if (ready) ICM_TRY(hipStreamWaitEvent(st, ready[0], 0));
         if ((rc = signal_begin(st))) return rc;
-        for (int k = 0; k < cfg.num_imfs - 1; ++k)
+        for (int k = 0; k < cfg.num_imfs - 1; ++k) {
+            if (ready && k > 0) ICM_TRY(hipStreamWaitEvent(st, ready[k], 0));
             if ((rc = signal_stage(k, st))) return rc;
+        }
         return signal_end(st);
+    }
+
+    // Both phases side by side (FIXED mode): the noise-mode phase on the
+    // plan's second stream, the CEEMDAN phase on `st`; CEEMDAN stage k waits
+    // only for noise-mode stage k (it reads E_{k+1}), so the latency-bound
+    // launches of one phase (knot prefix, solve, stage start, ensemble) run
+    // beside the other's sift passes.  The result is the same bits as one
+    // phase after the other: the phases share no buffer but the noise modes,
+    // which the CEEMDAN phase reads after the event that follows their write.
+    bool overlap() const
+    {
+        return cfg.stop_mode == ICM_STOP_FIXED && overlap_on && aux && !tuning && !trace_passes;
+    }
+    int both_phases(hipStream_t st)
+    {
+        int rc;
+        ICM_TRY(hipEventRecord(ev_fork, st));
+        ICM_TRY(hipStreamWaitEvent(aux, ev_fork, 0));
+        if ((rc = noise_phase(aux, stage_ev.data()))) return rc;
+        if (!capturing) ICM_TRY(hipEventRecord(ev[1], aux));
+        if (!capturing) ICM_TRY(hipEventRecord(ev_sig, st));
+        return signal_phase(st, stage_ev.data());
     }
 
     // FIXED: always; RILLING: when both caps are at most kRillGraphCap passes
@@ -1118,17 +1211,21 @@ struct Plan : PlanBase {
                (cfg.max_sifts <= kRillGraphCap && cfg.noise_max_sifts <= kRillGraphCap);
     }
 
+    bool graph_on() const
+    {
+        return cfg.use_graph && graph_mode() && !trace_passes && !prof && !graph_failed && !tuning;
+    }
+
     int capture_or_run(int which, hipStream_t st)
     {
         auto body = [&](hipStream_t s) -> int {
-            return which == 0 ? noise_phase(s) : signal_phase(s);
+            return which == 0 ? noise_phase(s) : which == 1 ? signal_phase(s) : both_phases(s);
         };
-        const bool graph = cfg.use_graph && graph_mode() && !trace_passes && !prof &&
-                           !graph_failed && !tuning;
-        if (!graph) return body(st);
+        if (!graph_on()) return body(st);
         if (!gexec[which]) {
-            // capture on the plan's own stream, replay on the caller's
-            const uint32_t e0 = epoch;
+            // capture on the plan's own stream (and, for both phases, on
+            // the second stream that joins it), replay on the caller's
+            const uint32_t e0[2] = {ln[0].epoch, ln[1].epoch};
             const hipError_t be = hipStreamBeginCapture(own, hipStreamCaptureModeThreadLocal);
             if (be != hipSuccess) {
                 graph_failed = true;
@@ -1145,7 +1242,8 @@ struct Plan : PlanBase {
                 (void)hip_fail(ce != hipSuccess ? ce : hipErrorUnknown, __LINE__);
                 (void)hipGetLastError();
                 graph_failed = true;
-                epoch = e0;
+                ln[0].epoch = e0[0];
+                ln[1].epoch = e0[1];
                 return body(st);
             }
             hipGraphExec_t ex = nullptr;
@@ -1155,7 +1253,8 @@ struct Plan : PlanBase {
                 (void)hipGraphDestroy(g);
                 (void)hipGetLastError();
                 graph_failed = true;
-                epoch = e0;
+                ln[0].epoch = e0[0];
+                ln[1].epoch = e0[1];
                 return body(st);
             }
             (void)hipGraphDestroy(g);
@@ -1185,10 +1284,17 @@ struct Plan : PlanBase {
         ICM_TRY(launch_fill_u32((uint32_t*)err, 0u, 1, st));
         int rc;
         if ((rc = inputs_and_noise(sig, noise, st))) return rc;
-        if ((rc = reset_control(st))) return rc;
-        if ((rc = capture_or_run(0, st))) return rc;
-        ICM_TRY(hipEventRecord(ev[1], st));
-        if ((rc = capture_or_run(1, st))) return rc;
+        if ((rc = reset_controls(st))) return rc;
+        const bool both = overlap();
+        const bool split = !(both && graph_on());   // phase times measured
+        if (both) {
+            if ((rc = capture_or_run(2, st))) return rc;
+        } else {
+            if ((rc = capture_or_run(0, st))) return rc;
+            ICM_TRY(hipEventRecord(ev[1], st));
+            ICM_TRY(hipEventRecord(ev_sig, st));
+            if ((rc = capture_or_run(1, st))) return rc;
+        }
         ICM_TRY(hipEventRecord(ev[2], st));
         for (int c = 0; c < C; ++c)
             ICM_TRY(launch_copy<real>((real*)out[c], imf + chan_imf_off[c], (int64_t)M * lens[c], st));
@@ -1196,11 +1302,13 @@ struct Plan : PlanBase {
         ICM_TRY(hipMemcpyAsync(h_err, err, sizeof(int), hipMemcpyDeviceToHost, st));
         ICM_TRY(hipStreamSynchronize(st));
         float t01 = 0, t12 = 0, t03 = 0;
-        (void)hipEventElapsedTime(&t01, ev[0], ev[1]);
-        (void)hipEventElapsedTime(&t12, ev[1], ev[2]);
         (void)hipEventElapsedTime(&t03, ev[0], ev[3]);
-        timing.noise_ms = t01;
-        timing.signal_ms = t12;
+        if (split) {
+            (void)hipEventElapsedTime(&t01, ev[0], ev[1]);
+            (void)hipEventElapsedTime(&t12, ev_sig, ev[2]);
+        }
+        timing.noise_ms = split ? t01 : -1.0;
+        timing.signal_ms = split ? t12 : -1.0;
         timing.total_ms = t03;
         prof_collect();
         if (*h_err) {
@@ -1255,7 +1363,7 @@ struct Plan : PlanBase {
         ICM_TRY(launch_fill_u32((uint32_t*)err, 0u, 1, st));
         int rc;
         if ((rc = inputs_and_noise(sig, noise, st))) return rc;
-        if ((rc = reset_control(st))) return rc;
+        if ((rc = reset_controls(st))) return rc;
         if ((rc = noise_phase(st))) return rc;
         if ((rc = signal_begin(st))) return rc;
         if ((rc = sync_err(st))) return rc;
@@ -1380,7 +1488,7 @@ struct Plan : PlanBase {
         std::fill(krec_valid.begin(), krec_valid.end(), (char)0);
         ICM_TRY(launch_fill_u32((uint32_t*)err, 0u, 1, st));
         if ((rc = inputs_and_noise(nullptr, noise, st))) return rc;
-        if ((rc = reset_control(st))) return rc;
+        if ((rc = reset_controls(st))) return rc;
         if ((rc = noise_phase(st))) return rc;
         if ((rc = signal_begin(st))) return rc;
         if ((rc = sync_err(st))) return rc;
@@ -1417,9 +1525,9 @@ struct Plan : PlanBase {
         const int M = cfg.num_imfs;
         int rc;
         if ((rc = alloc(&mu, dsig.samples))) return rc;
-        if ((rc = alloc(&mom, 8 * kstride))) return rc;
+        if ((rc = alloc(&mom, 8 * ln[ICM_PHASE_SIGNAL].kstride))) return rc;
         if ((rc = alloc(&edge, 16 * (int64_t)dsig.ntiles))) return rc;
-        if ((rc = alloc(&ga, 2 * kstride))) return rc;
+        if ((rc = alloc(&ga, 2 * ln[ICM_PHASE_SIGNAL].kstride))) return rc;
         if ((rc = alloc(&gr, (int64_t)M * chan_total))) return rc;
         if ((rc = alloc(&gacc, chan_total))) return rc;
         if ((rc = alloc(&dots, (int64_t)dsig.nseries))) return rc;
@@ -1447,8 +1555,8 @@ struct Plan : PlanBase {
             ICM_TRY(launch_adj_seed<real>((const real*)gimf[c], M, lens[c], chan_total,
                                           gr + chan_off[c], st));
         }
-        if ((rc = reset_control(st))) return rc;
-        const SeriesDev sd = dsig.view(kstride, cap);
+        if ((rc = reset_control(ln[ICM_PHASE_SIGNAL], st))) return rc;
+        const SeriesDev sd = dsig.view(ln[ICM_PHASE_SIGNAL].kstride, cap);
         for (int k = M - 2; k >= 0; --k) {
             const real* Ek = modes + (int64_t)k * hn.samples;
             const real* rk = k == 0 ? xt : r_hist + (int64_t)k * chan_total;
@@ -1540,12 +1648,14 @@ struct Plan : PlanBase {
         tuning = false;
         int rc;
         ICM_TRY(launch_fill_u32((uint32_t*)err, 0u, 1, st));
-        if ((rc = reset_control(st))) return rc;
+        if ((rc = reset_control(ln[ICM_PHASE_NOISE], st))) return rc;
         InitArgs<real> ia{};
         ia.mode = INIT_COPY;
         ia.src = h;
         if ((rc = stage_fixed(ICM_PHASE_NOISE, 0, dn, ia, 1, hn, st))) return rc;
-        if (nk_out) ICM_TRY(launch_copy<float>((float*)nk_out, (const float*)nk[0], 2 * dn.nseries, st));
+        if (nk_out)
+            ICM_TRY(launch_copy<float>((float*)nk_out, (const float*)ln[ICM_PHASE_NOISE].nk[0],
+                                       2 * dn.nseries, st));
         ICM_TRY(hipMemcpyAsync(h_err, err, sizeof(int), hipMemcpyDeviceToHost, st));
         ICM_TRY(hipStreamSynchronize(st));
         return *h_err ? ICM_ERR_DEVICE : ICM_OK;
@@ -1622,6 +1732,26 @@ int icm_plan_create(const icm_config* c, icm_plan* out)
     *out = nullptr;
     int rc = validate(c);
     if (rc) return rc;
+    // refuse a plan that cannot fit before allocating any of it: against
+    // ICM_MEMORY_BUDGET (bytes) when set, else the device's free memory
+    const int64_t need = plan_bytes_estimate(c);
+    int64_t budget = -1;
+    const char* what = "budget";
+    if (const char* env = std::getenv("ICM_MEMORY_BUDGET")) {
+        budget = std::atoll(env);
+    } else {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess) {
+            budget = (int64_t)fr;
+            what = "free device memory";
+        }
+    }
+    if (budget >= 0 && need > budget) {
+        std::snprintf(g_last_error, sizeof g_last_error,
+                      "plan needs %lld bytes, %s is %lld bytes", (long long)need, what,
+                      (long long)budget);
+        return ICM_ERR_OUT_OF_MEMORY;
+    }
     PlanBase* p = nullptr;
     if (c->dtype == ICM_DTYPE_F64) p = new (std::nothrow) Plan<double>();
     else p = new (std::nothrow) Plan<float>();
@@ -1725,12 +1855,20 @@ int icm_set_tuning(icm_plan plan, int phase, const int32_t* host_in, int64_t cou
     return plan->p->set_tuning(phase, host_in, count);
 }
 
+int icm_set_overlap(icm_plan plan, int on)
+{
+    if (!plan) return ICM_ERR_INVALID_ARG;
+    plan->p->overlap_on = on != 0;
+    plan->p->drop_graphs();
+    return ICM_OK;
+}
+
 int icm_get_graph_status(icm_plan plan, int* status)
 {
     if (!plan || !status) return ICM_ERR_INVALID_ARG;
     const PlanBase& P = *plan->p;
     if (P.graph_failed) *status = -1;
-    else *status = (P.gexec[0] && P.gexec[1]) ? 1 : 0;
+    else *status = (P.gexec[2] || (P.gexec[0] && P.gexec[1])) ? 1 : 0;
     return ICM_OK;
 }
 
@@ -1754,6 +1892,7 @@ int icm_set_trace(icm_plan plan, int passes)
     if (!plan || passes < 0) return ICM_ERR_INVALID_ARG;
     PlanBase& P = *plan->p;
     P.trace_passes = passes;
+    P.drop_graphs();   // a captured graph holds the buffers it was captured with
     const int M = P.cfg.num_imfs;
     if (passes > 0) {
         const int64_t nn = (int64_t)(M - 1) * passes * 2 * (int64_t)P.hn.len.size();

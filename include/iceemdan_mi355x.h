@@ -85,8 +85,10 @@ typedef struct icm_plan_info {
 } icm_plan_info;
 
 typedef struct icm_timing {
-    double noise_ms;            /* noise generation + noise-mode stages */
-    double signal_ms;           /* CEEMDAN stages (reference timer, cudaICEEMDAN.cu:1002-1297) */
+    double noise_ms;            /* noise generation + noise-mode stages (-1: not measured) */
+    double signal_ms;           /* CEEMDAN stages (reference timer, cudaICEEMDAN.cu:1002-1297;
+                                   with the phases side by side it includes waits for the
+                                   noise modes; -1: not measured) */
     double total_ms;            /* whole forward, input copy to output copy */
     double backward_ms;         /* last icm_backward, gradient in to gradient out */
 } icm_timing;
@@ -117,6 +119,16 @@ int icm_plan_get_info(icm_plan plan, icm_plan_info* info);
 int icm_forward(icm_plan plan, const void* const* d_signals, void* const* d_imfs,
                 const void* d_noise, void* stream);
 int icm_last_timing(icm_plan plan, icm_timing* out);
+
+/* FIXED mode runs the noise-mode phase on a second stream of the plan beside
+ * the CEEMDAN phase (stage k of which waits only for noise-mode stage k), so
+ * that one phase's latency-bound launches run beside the other's sift
+ * passes; the result is the same bits.  on = 0 runs the phases one after
+ * the other (also ICM_OVERLAP=0 in the environment when a plan is made).
+ * Tuning and traced forwards always run one phase after the other.  With
+ * the phases side by side and a captured graph, icm_last_timing reports
+ * noise_ms = signal_ms = -1 (only the total is measured). */
+int icm_set_overlap(icm_plan plan, int on);
 
 /* Graph state of the plan after its last forward: 1 = both phases replay a
  * captured hipGraph; 0 = launched eagerly (use_graph off, or the forward was

@@ -138,3 +138,16 @@ def test_product_path_has_no_cpu_fallback(icm):
     x = torch.zeros(100, dtype=torch.float64)
     with pytest.raises((ValueError, RuntimeError)):
         icm.sift_pass(x.reshape(1, -1))
+
+
+def test_plan_above_memory_budget_is_refused_before_allocating(icm, monkeypatch):
+    """ICM_MEMORY_BUDGET: a plan whose size estimate exceeds it is refused with
+    ICM_ERR_OUT_OF_MEMORY and a message, before any device call (this runs
+    without a GPU)."""
+    monkeypatch.setenv("ICM_MEMORY_BUDGET", str(10**9))
+    cfg, _keep = icm.make_config([244032], 500, 16)
+    h = C.c_void_p()
+    assert icm.lib().icm_plan_create(C.byref(cfg), C.byref(h)) == -2
+    assert not h.value
+    assert b"budget" in icm.lib().icm_last_error()
+    assert "budget" in str(icm.ICMError(-2))

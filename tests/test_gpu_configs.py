@@ -161,6 +161,7 @@ def test_rilling_graph_replay_matches_oracle(icm, orc, dev):
     plan = icm.Plan(lens, I, M, dtype=torch.float64, stop_mode=icm.STOP_RILLING, max_sifts=cap,
                     noise_max_sifts=ncap, use_graph=True, device=dev)
     plan.set_autotune(0)
+    plan.set_trace(0)   # records the stop counts, traces no pass (which would stop the graph)
     xd = [torch.tensor(x, device=dev) for x in xs]
     for k in range(3):
         out = plan.forward(xd)
@@ -204,6 +205,38 @@ def test_graph_replay_matches_oracle(icm, orc, dev):
         torch.cuda.synchronize()
         assert torch.isfinite(junk)
     plan.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_phases_side_by_side_match_oracle(icm, orc, dev, graph):
+    """The noise-mode phase on the plan's second stream beside the CEEMDAN
+    phase (each CEEMDAN stage waiting for its noise mode): every forward,
+    eager or replayed from the captured fork-join graph, equals the oracle
+    per channel, and equals the phases run one after the other."""
+    import torch
+    sig = _sig()
+    lens, I, M, S = [6000, 2500, 6000], 5, 6, 8
+    xs = [sig.eeg_like(n, seed=95 + c) for c, n in enumerate(lens)]
+    refs = [orc.iceemdan(x, _gpu_noise(icm, dev, I, n), M, S_or_cap=S)["imfs"]
+            for n, x in zip(lens, xs)]
+    print(f"case: phases side by side graph={graph}", flush=True)
+    plan = icm.Plan(lens, I, M, dtype=torch.float64, max_sifts=S, use_graph=graph, device=dev)
+    plan.set_autotune(0)
+    xd = [torch.tensor(x, device=dev) for x in xs]
+    for k in range(3):
+        out = plan.forward(xd)
+        t = plan.timing()
+        if graph:
+            assert plan.graph_status() == 1 and t["noise_ms"] == -1.0, (k, t)
+        else:
+            assert t["noise_ms"] > 0 and t["signal_ms"] > 0, (k, t)
+        for c in range(len(lens)):
+            assert np.array_equal(out[c].cpu().numpy(), refs[c]), (k, c)
+    plan.set_overlap(False)
+    out = plan.forward(xd)
+    plan.close()
+    for c in range(len(lens)):
+        assert np.array_equal(out[c].cpu().numpy(), refs[c]), c
 
 
 def test_resumed_run_matches_oracle(icm, orc, dev):

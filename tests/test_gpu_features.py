@@ -36,12 +36,17 @@ def test_fp32_sift_pass_close_to_fp64_oracle(icm, orc, dev):
         # the oracle is fed the same float32-rounded samples
         ref, nmax, nmin, _, _ = orc.sift_pass(h[i].astype(np.float32).astype(np.float64))
         assert (nk[i, 0], nk[i, 1]) == (nmax, nmin)
-        assert relerr(hn[i], ref) < 1e-4
+        e = relerr(hn[i], ref)
+        print(f"fp32 sift pass {i}: max rel err {e:.2e}", flush=True)
+        assert e < 1e-4
 
 
 def test_fp32_iceemdan_vs_fp64_oracle(icm, orc, dev):
     """Per-mode max relative error of the FP32 path against the FP64 oracle
-    (reported; the first modes are held to 1e-3)."""
+    fed the same float32 input and noise, at a size where no extrema set
+    flips between the precisions: measured 1.1e-6, 2.7e-6, 1.2e-6, 6.0e-7,
+    6.4e-7 and 5.0e-6 (residue); held to 1e-5 on the first two modes and
+    5e-5 on every row."""
     import torch
     N, I, M, S = 4096, 16, 6, 10
     x = _sig().three_tone(N, seed=0)
@@ -55,7 +60,8 @@ def test_fp32_iceemdan_vs_fp64_oracle(icm, orc, dev):
     errs = [relerr(out[k], ref[k]) for k in range(M)]
     print("fp32 per-mode max rel err:", " ".join(f"{e:.2e}" for e in errs))
     assert np.all(np.isfinite(out))
-    assert max(errs[:2]) < 1e-3
+    assert max(errs[:2]) < 1e-5
+    assert max(errs) < 5e-5
 
 
 def test_rilling_fp64_matches_oracle(icm, orc, dev):
@@ -212,3 +218,19 @@ def test_invalid_arguments_raise(icm, dev):
         plan.forward([torch.zeros(99, dtype=torch.float64, device=dev)])
     plan.close()
 
+
+
+@pytest.mark.parametrize("bad", [float("nan"), float("inf")])
+def test_non_finite_input_matches_oracle(icm, orc, dev, bad):
+    """A NaN or an infinity in the input: the forward completes (no fault, no
+    hang) and its rows equal the oracle's, NaN for NaN (std(x) is not finite,
+    so the standardised input and every row are NaN)."""
+    import torch
+    N, I, M, S = 3000, 4, 4, 6
+    x = _sig().eeg_like(N, seed=77)
+    x[N // 3] = bad
+    print(f"case: non-finite input {bad}", flush=True)
+    w = icm.noise(I, N, seed=1234, dtype=torch.float64, device=dev)
+    got = icm.iceemdan(torch.tensor(x, device=dev), I, M, max_sifts=S, noise=w).cpu().numpy()
+    ref = orc.iceemdan(x, w.cpu().numpy(), M, S_or_cap=S)["imfs"]
+    assert np.array_equal(got, ref, equal_nan=True)
