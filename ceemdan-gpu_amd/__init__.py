@@ -92,6 +92,8 @@ _SIGS = {
     "icm_noise": (C.c_int, [C.c_int, C.c_uint64, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p]),
     "icm_sift_pass": (C.c_int, [C.c_int, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p,
                                 C.c_void_p, C.c_void_p]),
+    "icm_sift_pass_static": (C.c_int, [C.c_int, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p,
+                                       C.c_void_p, C.c_void_p]),
     "icm_backward": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                C.c_void_p]),
     "icm_forward_begin": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p]),
@@ -519,6 +521,26 @@ def sift_pass(h):
                                C.c_void_p(hn.data_ptr()), C.c_void_p(nk.data_ptr()),
                                _stream_ptr(h.device)), "icm_sift_pass")
     return hn, nk
+
+
+def sift_pass_static(h, out=None, nk=None):
+    """sift_pass without any device allocation inside the library (its
+    scratch is a fixed per-GPU array): I * N <= 2^21, I <= 4096.  `out` and
+    `nk` may be given to avoid torch allocations too."""
+    torch = _torch()
+    _require_cuda(h)
+    if h.dim() == 1:
+        h = h.unsqueeze(0)
+    I, N = h.shape
+    hn = torch.empty_like(h) if out is None else out
+    k = torch.empty((I, 2), dtype=torch.int32, device=h.device) if nk is None else nk
+    _require_cuda(hn, k)
+    if hn.shape != h.shape or hn.dtype != h.dtype or k.numel() != 2 * I or k.dtype != torch.int32:
+        raise ValueError("out / nk do not match h")
+    _check(lib().icm_sift_pass_static(_dtype_code(h.dtype), C.c_void_p(h.data_ptr()), N, I,
+                                      C.c_void_p(hn.data_ptr()), C.c_void_p(k.data_ptr()),
+                                      _stream_ptr(h.device)), "icm_sift_pass_static")
+    return hn, k
 
 
 def iceemdan(x, num_noise, num_imfs, max_sifts=10, stop_mode=STOP_FIXED, noise_strength=0.2,

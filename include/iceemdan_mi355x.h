@@ -197,6 +197,18 @@ int icm_noise(int dtype, uint64_t seed, int64_t I, int64_t N, void* d_out, void*
 int icm_sift_pass(int dtype, const void* d_h, int64_t N, int64_t I, void* d_h_new, int32_t* d_nk,
                   void* stream);
 
+/* icm_sift_pass without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, kernels and result bit
+ * for bit as icm_sift_pass; every buffer beyond d_h, d_h_new and d_nk is
+ * carved from one file-scope device array of 24 MiB per GPU, held until the
+ * process ends.  Limits: I * N <= 2^21 samples and I <= 4096 series, else
+ * ICM_ERR_INVALID_ARG.  Calls on one GPU share that GPU's array (it keeps
+ * the last call's contents), so calls that overlap on one GPU race: the
+ * caller serialises them.  Synchronous. */
+int icm_sift_pass_static(int dtype, const void* d_h, int64_t N, int64_t I, void* d_h_new,
+                         int32_t* d_nk, void* stream);
+
 /* Staged forward with stage-boundary checkpoints (no reference counterpart;
  * the reference runs a decomposition in one call and cannot resume one).
  *   icm_forward_begin   inputs, noise modes and stage-0 statistics; the plan
