@@ -980,6 +980,15 @@ __global__ __launch_bounds__(1024) void k_series_var(const real* base, const int
 // the wave's own LDS slice, and no workgroup barrier is used, so the four
 // waves of a workgroup run independent segments.
 
+// base + the number of set bits of `mask` below this lane (two mbcnt steps;
+// running bases instead of a per-row prefix array keep eight scalar
+// registers free, which the sift loop would otherwise spill)
+__device__ __forceinline__ int rank_below(uint64_t mask, int base)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, (uint32_t)base));
+}
+
 // Lane 0 stores the eight wave-uniform row masks m[type][row] (no lane-
 // dependent indexing, which would put the array in scratch memory).
 __device__ __forceinline__ void store_masks(uint64_t* dst, const uint64_t (&m)[2][kWR], int lane)
@@ -1062,13 +1071,6 @@ __device__ __forceinline__ void sift_segment(const SiftArgs<real>& a, const Segm
         mcnt[0] = load_scalar(a.kin.cnt + 2 * gt);
         mcnt[1] = load_scalar(a.kin.cnt + 2 * gt + 1);
     }
-    if (seg.count > 1) {
-        const int gt = tb + seg.t_first + 1;
-        npre[0] = load_scalar(a.kin.pre + 2 * gt);
-        npre[1] = load_scalar(a.kin.pre + 2 * gt + 1);
-        ncnt[0] = load_scalar(a.kin.cnt + 2 * gt);
-        ncnt[1] = load_scalar(a.kin.cnt + 2 * gt + 1);
-    }
     WTile<real, KC> cur, nxt;
     load(seg.t_first, mpre, mcnt, cur);
     // the first tile is needed at once; waiting here also tells the compiler
@@ -1086,30 +1088,30 @@ __device__ __forceinline__ void sift_segment(const SiftArgs<real>& a, const Segm
         const int t0 = t * kWT;
         const int nt = min(kWT, N - t0);
         const int pre[2] = {mpre[0], mpre[1]}, cnt[2] = {mcnt[0], mcnt[1]};
-        int n2pre[2] = {0, 0}, n2cnt[2] = {0, 0};
         if (i + 1 < seg.count) {
+            // the next tile's prefixes and counts, then its prefetch (which
+            // needs them for its coefficient loads); loading them a tile
+            // earlier hid the scalar loads' latency but held four more scalar
+            // registers through the loop, which spilled (through vector
+            // lanes) at 5 % of the pass's time
+            const int g1 = gt + 1;
+            npre[0] = load_scalar(a.kin.pre + 2 * g1);
+            npre[1] = load_scalar(a.kin.pre + 2 * g1 + 1);
+            ncnt[0] = load_scalar(a.kin.cnt + 2 * g1);
+            ncnt[1] = load_scalar(a.kin.cnt + 2 * g1 + 1);
             load(t + 1, npre, ncnt, nxt);
-            if (i + 2 < seg.count) {
-                const int g2 = gt + 2;
-                n2pre[0] = load_scalar(a.kin.pre + 2 * g2);
-                n2pre[1] = load_scalar(a.kin.pre + 2 * g2 + 1);
-                n2cnt[0] = load_scalar(a.kin.cnt + 2 * g2);
-                n2cnt[1] = load_scalar(a.kin.cnt + 2 * g2 + 1);
-            }
         }
 
         // ---- knot flags of h (find_extrema_shfl_*, cudaICEEMDAN.cu:92-148):
         // the previous pass (or the stage start) found them when it wrote h,
         // from the same neighbour values, and stored them as bitmasks
         uint64_t bm[2][kWR];
-        int cb[2][kWR];
         int tot[2] = {0, 0};
 #pragma unroll
         for (int ty = 0; ty < 2; ++ty)
 #pragma unroll
             for (int r = 0; r < kWR; ++r) {
                 bm[ty][r] = lane_value(cur.mk, ty * kWR + r);
-                cb[ty][r] = tot[ty];
                 tot[ty] += __popcll(bm[ty][r]);
             }
         const bool over = cnt[0] + 3 > KC || cnt[1] + 3 > KC;
@@ -1123,13 +1125,14 @@ __device__ __forceinline__ void sift_segment(const SiftArgs<real>& a, const Segm
         auto tile = [&](auto full_c) {
             constexpr bool FULL = decltype(full_c)::value;
             // ---- knot table: index j <-> knot pre - 2 + j
+            int rb[2] = {2, 2};   // table index of the row's first knot
 #pragma unroll
             for (int r = 0; r < kWR; ++r) {
                 const int p = t0 + r * kWave + lane;
 #pragma unroll
                 for (int ty = 0; ty < 2; ++ty) {
                     if (__builtin_amdgcn_inverse_ballot_w64(bm[ty][r])) {
-                        const int j = cb[ty][r] + lanes_below(bm[ty][r]) + 2;
+                        const int j = rank_below(bm[ty][r], rb[ty]);
                         const int g = pre[ty] - 2 + j;
                         real y = cur.h[r];
                         if (!FULL) {   // knots 0 and n-1 lie in the series' end tiles
@@ -1139,6 +1142,7 @@ __device__ __forceinline__ void sift_segment(const SiftArgs<real>& a, const Segm
                         tp[ty][j] = p;
                         tc[ty][j].a = y;
                     }
+                    rb[ty] += __popcll(bm[ty][r]);
                 }
             }
             if (lane < 6) {
@@ -1171,8 +1175,22 @@ __device__ __forceinline__ void sift_segment(const SiftArgs<real>& a, const Segm
                 const real hk = (real)(tp[ty][j + 1] - tp[ty][j]);
                 const real a0 = tc[ty][j].a, a1 = tc[ty][j + 1].a;
                 const real c0 = tc[ty][j].c, c1 = tc[ty][j + 1].c;
-                tc[ty][j].b = (a1 - a0) / hk - hk * ((real)2 * c0 + c1) / (real)3;
-                tc[ty][j].d = (c1 - c0) / ((real)3 * hk);
+                if (std::is_same<real, double>::value) {
+                    // the reference's expressions, correctly rounded divisions:
+                    // FP64 equals the oracle bit for bit
+                    tc[ty][j].b = (a1 - a0) / hk - hk * ((real)2 * c0 + c1) / (real)3;
+                    tc[ty][j].d = (c1 - c0) / ((real)3 * hk);
+                } else {
+                    // FP32 (compared with the FP64 oracle by its error, not
+                    // bit for bit): the hardware reciprocal of the spacing
+                    // (1 ulp) and 1/3 as a product, instead of three
+                    // correctly rounded divisions (~9 instructions each) --
+                    // 9 % of the sift pass's time at the 64-channel FP32 job
+                    const real rh = (real)__builtin_amdgcn_rcpf((float)hk);
+                    const real third = (real)(1.0 / 3.0);
+                    tc[ty][j].b = (a1 - a0) * rh - hk * ((real)2 * c0 + c1) * third;
+                    tc[ty][j].d = (c1 - c0) * third * rh;
+                }
             }
             wave_sync();
 
@@ -1181,15 +1199,19 @@ __device__ __forceinline__ void sift_segment(const SiftArgs<real>& a, const Segm
             real hn[kWR];
             int sxc = 0;
             bool sxm = false;
+            int eb[2] = {1, 1};   // interval index of the row's first sample, less its rank
 #pragma unroll
             for (int r = 0; r < kWR; ++r) {
                 const int u = r * kWave + lane;
                 const int p = t0 + u;
                 hn[r] = cur.h[r];
+                const int ru = rank_below(bm[0][r], eb[0]), rl = rank_below(bm[1][r], eb[1]);
+                eb[0] += __popcll(bm[0][r]);
+                eb[1] += __popcll(bm[1][r]);
                 if (FULL || u < nt) {
                     const bool p0 = !FULL && p == 0;
-                    const int ju = p0 ? 2 : cb[0][r] + lanes_below(bm[0][r]) + 1;
-                    const int jl = p0 ? 2 : cb[1][r] + lanes_below(bm[1][r]) + 1;
+                    const int ju = p0 ? 2 : ru;
+                    const int jl = p0 ? 2 : rl;
                     const real U = horner(tc[0][ju], (real)(p - tp[0][ju]));
                     const real L = horner(tc[1][jl], (real)(p - tp[1][jl]));
                     const real mm = use_mean ? (U + L) / (real)2 : (real)0;
@@ -1283,10 +1305,6 @@ __device__ __forceinline__ void sift_segment(const SiftArgs<real>& a, const Segm
         mpre[1] = npre[1];
         mcnt[0] = ncnt[0];
         mcnt[1] = ncnt[1];
-        npre[0] = n2pre[0];
-        npre[1] = n2pre[1];
-        ncnt[0] = n2cnt[0];
-        ncnt[1] = n2cnt[1];
     }
 }
 

@@ -91,9 +91,16 @@ def test_c3_channels_fp64_bitexact(icm, orc, dev):
 def test_c3_channels_fp32_vs_oracle(icm, orc, dev):
     """The same channels in FP32.  The oracle is fed the same float32 input
     and the plan's float32 noise: the first pass of every noise mode sees
-    identical samples, so its knot counts are equal; the per-mode max
-    relative error is reported and held to what rounding-induced extrema
-    flips allow at this size (measured, with margin)."""
+    identical samples, so its knot counts are equal.  The per-mode error is
+    reported.  It is set by extrema flips: where two neighbouring samples
+    are within FP32 rounding of each other, the two precisions can find
+    different extrema, and the envelopes differ locally by up to the local
+    amplitude (weighted 1/I in the ensemble).  Their place and size are
+    chaotic (ulp-level changes of the arithmetic move them), so the maximum
+    over 2e5 samples varies from 4e-7 (no flip) to ~1e-2 between channels
+    and builds, while the RMS error stays small.  Held: RMS relative error
+    of modes 0 and 1 below 1e-3 and max relative error of mode 0 below 5e-2
+    (a wrong coefficient or stage term gives O(1) errors on every sample)."""
     import torch
     C, I, M, S = 6, 4, 5, 10
     lens, xs = _c3_channels(C)
@@ -114,14 +121,18 @@ def test_c3_channels_fp32_vs_oracle(icm, orc, dev):
         assert np.array_equal(trn[0, 0, off:off + I], ref["trace"][0][0, 0]), c
         off += I
         errs = [relerr(got[k], ref["imfs"][k]) for k in range(M)]
+        rms = [float(np.linalg.norm(got[k] - ref["imfs"][k]) / np.linalg.norm(ref["imfs"][k]))
+               for k in range(M)]
         print(f"c3 fp32 channel {c} N={n} per-mode max rel err:",
-              " ".join(f"{e:.2e}" for e in errs), flush=True)
+              " ".join(f"{e:.2e}" for e in errs), "rms:", " ".join(f"{e:.2e}" for e in rms),
+              flush=True)
         worst = np.maximum(worst, errs)
+        assert max(rms[:2]) < 1e-3, (c, rms)
         # completeness in float32: the rows telescope to x~ up to rounding
         xt = x32 / np.sqrt(np.var(x32, ddof=1))
         assert np.max(np.abs(got.sum(0) - xt)) < 2e-5 * np.max(np.abs(xt))
     print("c3 fp32 worst per-mode max rel err:", " ".join(f"{e:.2e}" for e in worst))
-    assert worst[0] < 5e-3, worst
+    assert worst[0] < 5e-2, worst
 
 
 def test_rilling_channels_vs_oracle(icm, orc, dev):
