@@ -16,9 +16,13 @@ Also in the one JSON line (rank 0):
   batched_vs_per_signal  the same 64 channels, one plan per channel run back
                          to back (the plain per-signal path), same settings
   roofline               the sift-pass kernel, HIP events on its stream over
-                         the timed steps; traffic from the committed
+                         the timed steps (the two phases run side by side,
+                         so each launch shares the GPU with the other
+                         phase's kernels); traffic from the committed
                          rocprofv3 FETCH_SIZE / WRITE_SIZE summary (its file
                          is named)
+  roofline_alone         the same kernel with the GPU to itself: one extra
+                         untimed forward with the phases one after the other
   fp32_vs_fp64           per-mode error of channel 0 against the FP64
                          decomposition (which equals the FP64 oracle bit for
                          bit)
@@ -165,6 +169,20 @@ def forward_workload(icm, torch, dev, name, xs, steps, warmup, graph=False):
     }
     res["ceemdan_phase_rate"] = sample_sifts / (1e-3 * res["phases_ms"]["ceemdan"])
     if not graph and sift_n:
+        # the same sift launches with the kernel alone on the GPU: one more
+        # (untimed) forward with the two phases one after the other
+        plan.set_overlap(False)
+        plan.forward(xd, out=out)
+        torch.cuda.synchronize()
+        ks = plan.kernel_stats()
+        plan.set_overlap(True)
+        alone_ms = ks["sift"][0] / ks["sift"][1]
+        alone_bpl = ks["sift"][2] / ks["sift"][1]
+        alone = alone_bpl / (alone_ms * 1e-3) / 1e9
+        res["roofline_alone"] = {"achieved": alone, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                 "frac": alone / PEAK_HBM_GBS, "avg_launch_ms": alone_ms,
+                                 "launches": int(ks["sift"][1]),
+                                 "note": "one extra forward, phases one after the other"}
         avg_ms = sift_ms / sift_n
         bpl = sift_bytes / sift_n
         achieved = bpl / (avg_ms * 1e-3) / 1e9
@@ -390,6 +408,7 @@ def main():
     }
     if "roofline" in main_res:
         line["roofline"] = main_res["roofline"]
+        line["roofline_alone"] = main_res["roofline_alone"]
     line["autotune"] = main_res["tuning"]
     if W["dtype"] == "f32" and not args.forward_only:
         line["fp32_vs_fp64"] = fp32_error(icm, torch, dev, name, xs, main_res["out"][0])
@@ -406,7 +425,9 @@ def main():
             line["c2"] = {"workload": WORKLOADS["c2"]["desc"], "value": c2["value"],
                           "ms_per_step": c2["ms_per_step"],
                           "roofline_frac": c2.get("roofline", {}).get("frac"),
-                          "sift_avg_launch_ms": c2.get("roofline", {}).get("avg_launch_ms")}
+                          "sift_avg_launch_ms": c2.get("roofline", {}).get("avg_launch_ms"),
+                          "roofline_frac_alone": c2.get("roofline_alone", {}).get("frac"),
+                          "sift_avg_launch_ms_alone": c2.get("roofline_alone", {}).get("avg_launch_ms")}
             del c2
             torch.cuda.empty_cache()
         line["c4"] = backward_workload(icm, torch, dev, "c4", signals_for("c4", sig), 3)

@@ -59,6 +59,13 @@ __device__ __forceinline__ int mask_last(const uint64_t (&m)[kWR])
     return last;
 }
 
+// orders this wave's LDS and memory accesses before and after it
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // tile offset of the first set bit, -1 if none
 __device__ __forceinline__ int mask_first(const uint64_t (&m)[kWR])
 {
@@ -484,6 +491,212 @@ __global__ __launch_bounds__(kPT) void k_knot_prefix_1(SeriesDev sd, KnotsDev<re
     }
 }
 
+// Series of at most kPT tiles: the same outputs with one wavefront per
+// series and no workgroup barrier.  Lane l owns tile 64 j + l of chunk j;
+// the wave scans each chunk (counts, and the last two knots before each
+// tile) and carries the totals to the next chunk, with the next chunk's
+// counts and bitmasks in flight meanwhile; a backward sweep over the chunks
+// (suffix minimum of the tiles' first knots) gives the first knot after each
+// tile.  Each window's first tile is known from the scan (the tile holding
+// the window's first gathered knot) and kept in LDS until the series' window
+// count is known.  Workgroups are four independent waves, so beside another
+// stream's sift the kernel runs in whatever wave slots that kernel leaves
+// free (a 1024-thread workgroup needs 16 free slots on one CU at once, and
+// then waited for the other stream's sift to drain: 3.5 ms per pass beside
+// it against 0.5 ms alone).
+struct Two {
+    int c, l1, l2;   // number of knots seen (at most 2), the last, the one before
+};
+
+__device__ __forceinline__ Two two_none() { return Two{0, -1, -1}; }
+
+__device__ __forceinline__ Two two_combine(const Two& A, const Two& B)
+{
+    Two r;
+    r.c = B.c >= 2 ? 2 : (B.c == 1 ? min(A.c + 1, 2) : A.c);
+    r.l1 = B.c >= 1 ? B.l1 : A.l1;
+    r.l2 = B.c >= 2 ? B.l2 : (B.c == 1 ? A.l1 : A.l2);
+    return r;
+}
+
+__device__ __forceinline__ Two two_shfl_up(const Two& v, int o)
+{
+    return Two{shfl_up(v.c, o), shfl_up(v.l1, o), shfl_up(v.l2, o)};
+}
+
+__device__ __forceinline__ Two two_lane(const Two& v, int l)
+{
+    return Two{__builtin_amdgcn_readlane(v.c, l), __builtin_amdgcn_readlane(v.l1, l),
+                __builtin_amdgcn_readlane(v.l2, l)};
+}
+
+// windows per type of a series of at most kPT tiles: <= 129 knots per tile
+// and type, K >= 448 rows kept per window
+constexpr int kPWin = (kPT * 129 + 447) / 448 + 1;
+// series length (tiles) up to which the one-wave kernel is used
+constexpr int kPWTiles = 128;
+
+template <typename real>
+__global__ __launch_bounds__(kThreads) void k_knot_prefix_w(SeriesDev sd, KnotsDev<real> k, const real* h,
+                                                            PassCtl pc, const int* active)
+{
+    constexpr int NW = kThreads / kWave;
+    constexpr int INF = 0x7FFFFFFF;
+    __shared__ int s_hint[NW][2][kPWin];
+    const int lane = threadIdx.x & 63, wv = wave_index();
+    const int s = blockIdx.x * NW + wv;
+    if (s >= sd.nseries) return;
+    if (pc.run && load_scalar(pc.run) == 0u) return;
+    if (active && !load_scalar(active + s)) return;
+    const int tb = load_scalar(sd.tile0 + s), nt = load_scalar(sd.ntl + s);
+    const real* hs = h + load_scalar(sd.off + s);
+    const int K = pc.K, H = pc.H;
+    const int nch = (nt + kWave - 1) / kWave;
+
+    // counts and the eight row bitmasks of tile 64 ch + lane (zero past the series)
+    auto load = [&](int ch, int (&c)[2], uint64_t (&m)[2 * kWR]) {
+        const int t = ch * kWave + lane;
+        const bool in = t < nt;
+        const int64_t gt = tb + (in ? t : 0);
+        c[0] = in ? k.cnt[2 * gt] : 0;
+        c[1] = in ? k.cnt[2 * gt + 1] : 0;
+#pragma unroll
+        for (int q = 0; q < 2 * kWR; ++q) m[q] = in ? k.mask[8 * gt + q] : 0ull;
+    };
+
+    int carry[2] = {0, 0};
+    Two carry_lt[2] = {two_none(), two_none()};
+    unsigned dense = 0;   // tiles the small sift table cannot hold
+    int c[2], cn[2] = {0, 0};
+    uint64_t m[2 * kWR], mn[2 * kWR] = {};
+    load(0, c, m);
+    for (int ch = 0; ch < nch; ++ch) {
+        if (ch + 1 < nch) load(ch + 1, cn, mn);
+        const int t = ch * kWave + lane;
+        const bool in = t < nt;
+        const int64_t gt = tb + t;
+        dense += (unsigned)__popcll(wave_ballot(in && (c[0] + 3 > kWKS || c[1] + 3 > kWKS)));
+        int ex[2], xb[2][2], fp[2];
+#pragma unroll
+        for (int ty = 0; ty < 2; ++ty) {
+            uint64_t mm[kWR];
+#pragma unroll
+            for (int r = 0; r < kWR; ++r) mm[r] = m[ty * kWR + r];
+            Two own = two_none();
+            fp[ty] = -1;
+            if (c[ty] > 0) {
+                fp[ty] = t * kWT + mask_first(mm);
+                const int b1 = mask_last(mm);
+                mm[b1 / kWave] ^= 1ull << (b1 % kWave);
+                own.c = min(c[ty], 2);
+                own.l1 = t * kWT + b1;
+                own.l2 = c[ty] >= 2 ? t * kWT + mask_last(mm) : -1;
+            }
+            int inc = c[ty];
+            Two sc = own;
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+                const int v = shfl_up(inc, o);
+                const Two w = two_shfl_up(sc, o);
+                if (lane >= o) {
+                    inc += v;
+                    sc = two_combine(w, sc);
+                }
+            }
+            ex[ty] = carry[ty] + inc - c[ty];
+            const Two xl = two_shfl_up(sc, 1);
+            const Two before = two_combine(carry_lt[ty], lane > 0 ? xl : two_none());
+            xb[ty][0] = before.l1;
+            xb[ty][1] = before.l2;
+            // the window whose first gathered knot (max(w K - H - 1, 0)) lies
+            // in this tile: at most one, as K exceeds a tile's knots
+            if (c[ty] > 0) {
+                if (ex[ty] == 0) s_hint[wv][ty][0] = t;
+                const int w = (ex[ty] + c[ty] + H) / K;
+                if (w >= 1 && w < kPWin && w * K - H - 1 >= ex[ty]) s_hint[wv][ty][w] = t;
+            }
+            carry[ty] += __builtin_amdgcn_readlane(inc, kWave - 1);
+            carry_lt[ty] = two_combine(carry_lt[ty], two_lane(sc, kWave - 1));
+        }
+        // values of the knots before the tile (positions of knots that exist,
+        // or 0), all loads before the stores
+        real yb[2][2];
+#pragma unroll
+        for (int ty = 0; ty < 2; ++ty)
+#pragma unroll
+            for (int w = 0; w < 2; ++w) yb[ty][w] = hs[max(xb[ty][w], 0)];
+        if (in) {
+#pragma unroll
+            for (int ty = 0; ty < 2; ++ty) {
+                k.pre[2 * gt + ty] = ex[ty];
+                const int64_t o = 6 * gt + 3 * ty;
+                k.nbx[o] = xb[ty][0];
+                k.nby[o] = xb[ty][0] >= 0 ? yb[ty][0] : (real)0;
+                k.nbx[o + 1] = xb[ty][1];
+                k.nby[o + 1] = xb[ty][1] >= 0 ? yb[ty][1] : (real)0;
+                // the tile's own first knot, read back by this lane below
+                k.nbx[o + 2] = fp[ty];
+            }
+        }
+        c[0] = cn[0];
+        c[1] = cn[1];
+#pragma unroll
+        for (int q = 0; q < 2 * kWR; ++q) m[q] = mn[q];
+    }
+    // backward: the first knot after each tile = the least first knot of
+    // the tiles after it
+    int cmin[2] = {INF, INF};
+    for (int ch = nch - 1; ch >= 0; --ch) {
+        const int t = ch * kWave + lane;
+        const bool in = t < nt;
+        const int64_t gt = tb + (in ? t : 0);
+        int after[2];
+#pragma unroll
+        for (int ty = 0; ty < 2; ++ty) {
+            const int f = in ? k.nbx[6 * gt + 3 * ty + 2] : -1;
+            int mnv = f >= 0 ? f : INF;
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+                const int v = shfl_down(mnv, o);
+                if (lane + o < kWave) mnv = min(mnv, v);
+            }
+            const int nx = shfl_down(mnv, 1);
+            after[ty] = min(cmin[ty], lane < kWave - 1 ? nx : INF);
+            cmin[ty] = min(cmin[ty], __builtin_amdgcn_readlane(mnv, 0));
+        }
+        real ya[2];
+#pragma unroll
+        for (int ty = 0; ty < 2; ++ty) ya[ty] = hs[after[ty] == INF ? 0 : after[ty]];
+        if (in) {
+#pragma unroll
+            for (int ty = 0; ty < 2; ++ty) {
+                const int64_t o = 6 * gt + 3 * ty + 2;
+                k.nbx[o] = after[ty] == INF ? -1 : after[ty];
+                k.nby[o] = after[ty] == INF ? (real)0 : ya[ty];
+            }
+        }
+    }
+    const int n0 = carry[0], n1 = carry[1];
+    const int wmx = (n0 + K - 1) / K;
+    const int wmn = (n1 + K - 1) / K;
+    unsigned wbase = 0;
+    if (lane == 0) {
+        k.n[2 * s] = n0;
+        k.n[2 * s + 1] = n1;
+        wbase = atomicAdd(&pc.wincount[pc.epoch], (unsigned)(wmx + wmn));
+        if (dense) atomicAdd(&pc.dense[pc.epoch], dense);
+    }
+    wbase = (unsigned)__builtin_amdgcn_readfirstlane((int)wbase);
+    wave_sync();   // the window hints written by other lanes
+    for (int w = lane; w < wmx + wmn; w += kWave) {
+        const int type = w >= wmx;
+        const int wi = type ? w - wmx : w;
+        pc.winlist[kWinItem * (wbase + w)] = s;
+        pc.winlist[kWinItem * (wbase + w) + 1] = (wi << 1) | type;
+        pc.winlist[kWinItem * (wbase + w) + 2] = wi < kPWin ? s_hint[wv][type][wi] : -1;
+    }
+}
+
 // ------------------------------------------------------------- stage init
 
 template <typename real>
@@ -523,26 +736,27 @@ __device__ __forceinline__ real horner(const Coef<real>& q, real t)
 
 // ------------------------------------------------------- windowed spline solve
 
-// Row g of the spline system (cudaICEEMDAN.cu:310-343); rows outside
-// [0, n) and the two end rows are identity rows with a zero right-hand side.
-// LDS index i holds knot g0 - 1 + i.  Not-a-knot ends (nak, n >= 4): c_0 is
-// eliminated from row 1 and c_{n-1} from row n-2 (both stay diagonally
-// dominant); the end values follow from nak_end after the solve.
-template <typename real>
-__device__ __forceinline__ void spline_row(int n, long g, int i, const int* sp, const real* sy,
-                                           bool nak, real& dl, real& d, real& du, real& f)
+// Row g of the spline system (cudaICEEMDAN.cu:310-343) from the lengths
+// hl, hr of the intervals left and right of knot g and their slopes sl, sr
+// ((y_g - y_{g-1}) / hl and (y_{g+1} - y_g) / hr, each computed once per
+// interval and shared by the two rows it touches); rows outside [0, n) and
+// the two end rows are identity rows with a zero right-hand side.
+// Not-a-knot ends (NAK, n >= 4): c_0 is eliminated from row 1 and c_{n-1}
+// from row n-2 (both stay diagonally dominant); the end values follow from
+// nak_end after the solve.
+template <typename real, bool NAK>
+__device__ __forceinline__ void spline_row(int n, long g, int hl, int hr, real sl, real sr,
+                                           real& dl, real& d, real& du, real& f)
 {
     // computed on every lane and selected at the end (an early return here
-    // kept the four outputs in scratch memory); rows outside (0, n-1) read
+    // kept the four outputs in scratch memory); rows outside (0, n-1) see
     // neighbours that exist in the window's LDS but take the identity row
     const bool inner = g > 0 && g < n - 1;
-    const int hl = sp[i] - sp[i - 1];
-    const int hr = sp[i + 1] - sp[i];
     real tdl = (real)hl;
     real tdu = (real)hr;
     real td = (real)(2 * (hl + hr));
-    const real tf = (real)3 * ((sy[i + 1] - sy[i]) / (real)hr - (sy[i] - sy[i - 1]) / (real)hl);
-    if (nak && n >= 4) {
+    const real tf = (real)3 * (sr - sl);
+    if (NAK && n >= 4) {
         if (g == 1) {
             tdl = (real)0;
             td = (real)(3 * hl + 2 * hr) + (real)hl * (real)hl / (real)hr;
@@ -568,12 +782,6 @@ __device__ __forceinline__ real nak_end(real c1, real c2, int h_e, int h_next)
     return c1 * ((real)1 + r) - c2 * r;
 }
 
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
-
 // Positions of knots [glo, ghi) of (series s, type) into LDS (sp index =
 // g - lo), from the tile bitmasks (mask [tile][type][row]), knot prefixes and
 // counts.  One wavefront; tpre (64 ints) and tmask (64 x 4 words) are LDS
@@ -585,16 +793,18 @@ __device__ __forceinline__ void wave_sync()
 // knot glo, or -1 (64-ary search).  NIT >= the knots of one window / 64.
 // (inlined: as a called function it cost the solve 8 % and the adjoint
 // window solve 1 %)
-// row masks of one staged tile in LDS, padded from 4 to 5 words: the 64
-// lanes store their tiles' rows at a 40-byte stride instead of 32, which
-// spreads the stores over the banks
+// row masks of one staged tile in LDS (TW words per tile).  The adjoint
+// pads them from 4 to 5 words: the 64 lanes store their tiles' rows at a
+// 40-byte stride instead of 32, which spreads the stores over the banks.
+// The solve keeps 4: 2 KB less LDS per workgroup gives its FP32 instance a
+// sixth wave per SIMD, which is worth more than the conflicts (-2.5 %).
 constexpr int kTMaskW = kWR + 1;
 
-template <int NIT>
+template <int NIT, int TW>
 __device__ __forceinline__ void gather_positions(const SeriesDev& sd, const uint64_t* kmask,
                                                  const int* kpre, const int* kcnt, int s, int type,
                                                  long lo, int glo, int ghi, int* sp, int* tpre,
-                                                 uint64_t (*tmask)[kTMaskW], int hint)
+                                                 uint64_t (*tmask)[TW], int hint)
 {
     const int lane = threadIdx.x & 63;
     const int tb = load_scalar(sd.tile0 + s), ntl = load_scalar(sd.ntl + s);
@@ -697,23 +907,31 @@ __device__ void gather_values(const real* hs, int N, long lo, int glo, int ghi, 
     }
 }
 
+// x / y in the solve: IEEE division in FP64 (the oracle's arithmetic, bit
+// for bit); in FP32 one hardware reciprocal (1 ulp) and a product
+template <typename real>
+__device__ __forceinline__ real sdiv(real x, real y)
+{
+    if constexpr (std::is_same<real, double>::value) return x / y;
+    else return x * (real)__builtin_amdgcn_rcpf((float)y);
+}
+
 template <typename real, int B, class RowFn>
 __device__ __forceinline__ void partition_pcr(int lane, RowFn row, real (&x)[B])
 {
     real cp[B], al[B], be[B], ga[B];
-    const int rb = lane * B;
 #pragma unroll
     for (int q = 0; q <= B - 2; ++q) {
         real dl, d, du, f;
-        row(rb + q, dl, d, du, f);
+        row(q, dl, d, du, f);
         real inv;
         if (q == 0) {
             const real dl0 = (lane == 0) ? (real)0 : dl;
-            inv = (real)1 / d;
+            inv = sdiv<real>((real)1, d);
             al[0] = f * inv;
             be[0] = (-dl0) * inv;
         } else {
-            inv = (real)1 / (d - dl * cp[q - 1]);
+            inv = sdiv<real>((real)1, d - dl * cp[q - 1]);
             al[q] = (f - dl * al[q - 1]) * inv;
             be[q] = (-dl) * be[q - 1] * inv;
         }
@@ -728,7 +946,7 @@ __device__ __forceinline__ void partition_pcr(int lane, RowFn row, real (&x)[B])
     }
     // interface row q = B-1 -> reduced tridiagonal system in z_lane
     real dl, d, du, f;
-    row(rb + B - 1, dl, d, du, f);
+    row(B - 1, dl, d, du, f);
     real na0 = shfl_down(al[0], 1), nb0 = shfl_down(be[0], 1), ng0 = shfl_down(ga[0], 1);
     if (lane == kWave - 1) {
         du = (real)0;
@@ -748,13 +966,13 @@ __device__ __forceinline__ void partition_pcr(int lane, RowFn row, real (&x)[B])
                    Rp = shfl_down(R, st);
         real a2 = (real)0, c2 = (real)0, d2 = D, r2 = R;
         if (lane - st >= 0) {
-            const real k1 = A / Dm;
+            const real k1 = sdiv<real>(A, Dm);
             a2 = -(Am * k1);
             d2 = d2 - Cm * k1;
             r2 = r2 - Rm * k1;
         }
         if (lane + st < kWave) {
-            const real k2 = C / Dp;
+            const real k2 = sdiv<real>(C, Dp);
             c2 = -(Cp * k2);
             d2 = d2 - Ap * k2;
             r2 = r2 - Rp * k2;
@@ -764,25 +982,28 @@ __device__ __forceinline__ void partition_pcr(int lane, RowFn row, real (&x)[B])
         D = d2;
         R = r2;
     }
-    const real z = R / D;
+    const real z = sdiv<real>(R, D);
     real zl = shfl_up(z, 1);
     if (lane == 0) zl = (real)0;
 #pragma unroll
     for (int q = 0; q < B; ++q) x[q] = (q == B - 1) ? z : (al[q] + be[q] * zl) + ga[q] * z;
 }
 
-// (three waves per SIMD in FP64, ≈ 147 registers: held to 128 for four
-// waves, the partition + PCR spills to scratch and the solve is 7 % slower)
-template <typename real, int B, int H>
+// One wavefront per window item.  NAK = not-a-knot ends, as its own
+// instance: the natural-end instance carries no end-row divisions (FP64:
+// 113 registers, four waves per SIMD; the not-a-knot one 149, three).
+template <typename real, int B, int H, bool NAK>
 __global__ __launch_bounds__(kThreads) void k_solve(SolveArgs<real> a)
 {
     constexpr int W = kWave * B;
     constexpr int K = W - 2 * H;
     constexpr int NW = kThreads / kWave;
-    __shared__ int s_pos[NW][W + 2];
-    __shared__ real s_y[NW][W + 2];
+    // rows padded to a multiple of 16 bytes, so a lane's B + 2 entries at
+    // r = lane * B start 16-byte aligned
+    __shared__ __align__(16) int s_pos[NW][W + 4];
+    __shared__ __align__(16) real s_y[NW][W + 4];
     __shared__ int s_tpre[NW][kWave];
-    __shared__ uint64_t s_tmask[NW][kWave][kTMaskW];
+    __shared__ uint64_t s_tmask[NW][kWave][kWR];
     const int wv = wave_index(), lane = threadIdx.x & 63;
     int* sp = s_pos[wv];
     real* sy = s_y[wv];
@@ -804,7 +1025,7 @@ __global__ __launch_bounds__(kThreads) void k_solve(SolveArgs<real> a)
         const int glo = (int)max(lo, 0L);
         const int ghi = (int)min(lo + W + 2, (long)n);
         constexpr int NIT = (W + 2 + kWave - 1) / kWave;
-        gather_positions<NIT>(a.sd, a.k.mask, a.k.pre, a.k.cnt, s, type, lo, glo, ghi, sp,
+        gather_positions<NIT, kWR>(a.sd, a.k.mask, a.k.pre, a.k.cnt, s, type, lo, glo, ghi, sp,
                               s_tpre[wv], s_tmask[wv], hint);
         wave_sync();
         gather_values<real, NIT>(a.h + load_scalar(a.sd.off + s), load_scalar(a.sd.len + s), lo, glo,
@@ -852,16 +1073,36 @@ __global__ __launch_bounds__(kThreads) void k_solve(SolveArgs<real> a)
         }
         wave_sync();
 
+        // the lane's rows r = rb .. rb+B-1 (LDS index r + 1) need window
+        // knots rb .. rb+B+1: loaded once, as wide reads, and the B + 1
+        // intervals between them give every row its lengths and slopes
+        const int rb = lane * B;
+        int hx[B + 1];
+        real sl[B + 1];
+        {
+            int px[B + 2];
+            real py[B + 2];
+#pragma unroll
+            for (int j = 0; j < B + 2; ++j) {
+                px[j] = sp[rb + j];
+                py[j] = sy[rb + j];
+            }
+#pragma unroll
+            for (int j = 0; j <= B; ++j) {
+                hx[j] = px[j + 1] - px[j];
+                sl[j] = sdiv<real>(py[j + 1] - py[j], (real)hx[j]);
+            }
+        }
         real xs[B];
         partition_pcr<real, B>(
             lane,
-            [&](int r, real& dl, real& d, real& du, real& f) {
-                spline_row<real>(n, g0 + r, r + 1, sp, sy, a.nak, dl, d, du, f);
+            [&](int q, real& dl, real& d, real& du, real& f) {
+                spline_row<real, NAK>(n, g0 + rb + q, hx[q], hx[q + 1], sl[q], sl[q + 1], dl, d, du,
+                                      f);
             },
             xs);
         real* cout = a.c + kb;
-        const int rb = lane * B;
-        const bool nak = a.nak && n >= 4;
+        const bool nak = NAK && n >= 4;
 #pragma unroll
         for (int q = 0; q < B; ++q) {
             const int r = rb + q;
@@ -1368,66 +1609,99 @@ __global__ __launch_bounds__(kThreads) void k_sift_list(SiftArgs<real> a, int ns
         sift_segment<real, RILL, kWKC>(b, segs, sid, lane, s_tp[wv], s_tc[wv]);
 }
 
-// Stage start for 256-sample tiles, one wavefront per tile: forms x_i
-// (the same arithmetic as init_value) and stores its knots as bitmasks.
-// MODE is ia.mode as a compile-time constant: one straight-line body per
-// mode, so that every load of the tile goes out before the first wait.
+// Stage start for 256-sample tiles: forms x_i (the same arithmetic as
+// init_value) and stores its knots as bitmasks.  MODE is ia.mode as a
+// compile-time constant: one straight-line body per mode.
+//
+// Per-series pointers and scales of the tile being formed.
 template <typename real, int MODE>
-__device__ __forceinline__ void init_tile(const SeriesDev& sd, const InitArgs<real>& ia, real* h_out,
-                                          const KnotsDev<real>& kout, int gt, int lane)
+struct InitSeries {
+    using P = typename Promote<real>::type;
+    int tile0, tend, N;     // first tile, one past the last, samples
+    const real* src;        // INIT_COPY / INIT_KNOTS / INIT_RESID
+    const real* base;       // INIT_STAGE0 / INIT_STAGEK: the channel's signal
+    const real* E;
+    P scl0;
+    real sclk;
+    real* out;
+    real* resid;
+};
+
+template <typename real, int MODE>
+__device__ __forceinline__ InitSeries<real, MODE> init_series(const SeriesDev& sd,
+                                                             const InitArgs<real>& ia,
+                                                             real* h_out, int s)
 {
     using P = typename Promote<real>::type;
-    const int s = sd.tile_series[gt];
-    const int t0 = (gt - sd.tile0[s]) * kWT;
-    const int N = sd.len[s];
-    const int nt = min(kWT, N - t0);
-    real* out = MODE == INIT_KNOTS ? nullptr : h_out + sd.off[s];
-    const real* src = nullptr;
-    const real* base = nullptr;
-    const real* E = nullptr;
-    P scl0 = (P)0;
-    real sclk = (real)0;
+    InitSeries<real, MODE> q{};
+    q.tile0 = sd.tile0[s];
+    q.tend = q.tile0 + sd.ntl[s];
+    q.N = sd.len[s];
+    const int64_t off = sd.off[s];
+    q.out = MODE == INIT_KNOTS ? nullptr : h_out + off;
+    q.resid = MODE == INIT_RESID ? ia.resid + off : nullptr;
     if (MODE == INIT_COPY || MODE == INIT_KNOTS) {
-        src = ia.src + sd.off[s];
+        q.src = ia.src + off;
     } else if (MODE == INIT_RESID) {
-        src = ia.src + sd.off[s];
-        E = ia.E + sd.off[s];
+        q.src = ia.src + off;
+        q.E = ia.E + off;
     } else {
         const int c = sd.chan[s];
-        base = ia.chan_sig + ia.chan_off[c];
-        E = ia.E + sd.eoff[s];
-        if (MODE == INIT_STAGE0) scl0 = (P)ia.scale[sd.nidx[s]];
-        else sclk = (real)ia.scale[c];
+        q.base = ia.chan_sig + ia.chan_off[c];
+        q.E = ia.E + sd.eoff[s];
+        if (MODE == INIT_STAGE0) q.scl0 = (P)ia.scale[sd.nidx[s]];
+        else q.sclk = (real)ia.scale[c];
     }
-    auto val = [&](int p) -> real {
-        if (MODE == INIT_COPY || MODE == INIT_KNOTS) return src[p];
-        if (MODE == INIT_RESID) return src[p] - E[p];
-        const real e = E[p];
-        if (MODE == INIT_STAGE0) return (real)((P)base[p] + (P)(ia.eps0 * e) * scl0);
-        return base[p] + (ia.eps0 * e) * sclk;
-    };
-    real* resid = MODE == INIT_RESID ? ia.resid + sd.off[s] : nullptr;
-    // every load of the tile before any store (the compiler may not move a
-    // load across a store to another pointer, so interleaving them would
-    // make each row wait for the previous one)
-    // (unconditional loads at clamped indices, then selects: a load under a
-    // lane condition would sit in its own block, waited for at its end)
+    return q;
+}
+
+// x_i of one tile (lane l: samples t0 + 64 r + l) and the samples just
+// outside it; loads at clamped indices, then selects (a load under a lane
+// condition would sit in its own block, waited for at its end)
+template <typename real>
+struct InitTile {
     real x[kWR];
+    real xl, xr;
+};
+
+template <typename real, int MODE>
+__device__ __forceinline__ void init_load(const InitSeries<real, MODE>& q, const InitArgs<real>& ia,
+                                          int gt, int lane, InitTile<real>& w)
+{
+    using P = typename Promote<real>::type;
+    const int t0 = (gt - q.tile0) * kWT;
+    const int nt = min(kWT, q.N - t0);
+    auto val = [&](int p) -> real {
+        if (MODE == INIT_COPY || MODE == INIT_KNOTS) return q.src[p];
+        if (MODE == INIT_RESID) return q.src[p] - q.E[p];
+        const real e = q.E[p];
+        if (MODE == INIT_STAGE0) return (real)((P)q.base[p] + (P)(ia.eps0 * e) * q.scl0);
+        return q.base[p] + (ia.eps0 * e) * q.sclk;
+    };
 #pragma unroll
     for (int r = 0; r < kWR; ++r) {
         const int u = r * kWave + lane;
-        const real v = val(min(t0 + u, N - 1));
-        x[r] = u < nt ? v : (real)0;
+        const real v = val(min(t0 + u, q.N - 1));
+        w.x[r] = u < nt ? v : (real)0;
     }
-    const real xlv = val(max(t0 - 1, 0)), xrv = val(min(t0 + kWT, N - 1));
-    const real xl = t0 >= 1 ? xlv : (real)0;
-    const real xr = t0 + kWT < N ? xrv : (real)0;
+    const real xlv = val(max(t0 - 1, 0)), xrv = val(min(t0 + kWT, q.N - 1));
+    w.xl = t0 >= 1 ? xlv : (real)0;
+    w.xr = t0 + kWT < q.N ? xrv : (real)0;
+}
+
+// stores of one tile: x_i (and the noise residual), its knot bitmasks and counts
+template <typename real, int MODE>
+__device__ __forceinline__ void init_store(const InitSeries<real, MODE>& q, const KnotsDev<real>& kout,
+                                           int gt, int lane, const InitTile<real>& w)
+{
+    const int t0 = (gt - q.tile0) * kWT;
+    const int nt = min(kWT, q.N - t0);
 #pragma unroll
     for (int r = 0; r < kWR; ++r) {
         const int u = r * kWave + lane;
         if (u < nt) {
-            if (MODE != INIT_KNOTS) out[t0 + u] = x[r];
-            if (resid) resid[t0 + u] = x[r];
+            if (MODE != INIT_KNOTS) q.out[t0 + u] = w.x[r];
+            if (MODE == INIT_RESID) q.resid[t0 + u] = w.x[r];
         }
     }
     int run[2] = {0, 0};
@@ -1436,11 +1710,11 @@ __device__ __forceinline__ void init_tile(const SeriesDev& sd, const InitArgs<re
     for (int r = 0; r < kWR; ++r) {
         const int u = r * kWave + lane;
         const int p = t0 + u;
-        const real lft = shift_prev(x[r], r == 0 ? xl : lane_value(x[r > 0 ? r - 1 : 0], kWave - 1));
-        const real rgt = shift_next(x[r], r == kWR - 1 ? xr : lane_value(x[r < kWR - 1 ? r + 1 : r], 0));
+        const real lft = shift_prev(w.x[r], r == 0 ? w.xl : lane_value(w.x[r > 0 ? r - 1 : 0], kWave - 1));
+        const real rgt = shift_next(w.x[r], r == kWR - 1 ? w.xr : lane_value(w.x[r < kWR - 1 ? r + 1 : r], 0));
         const bool valid = u < nt;
-        const bool end = p == 0 || p == N - 1;
-        const real v = x[r];
+        const bool end = p == 0 || p == q.N - 1;
+        const real v = w.x[r];
         const bool f[2] = {valid && (end || ((v > lft) && (v > rgt))),
                            valid && (end || ((v < lft) && (v < rgt)))};
 #pragma unroll
@@ -1456,16 +1730,38 @@ __device__ __forceinline__ void init_tile(const SeriesDev& sd, const InitArgs<re
     }
 }
 
-// one wavefront per tile, grid-stride over the tiles (the grid is capped,
-// so any tile count fits one launch)
+// One wavefront per run of kInitRun consecutive tiles (grid-stride over the
+// runs: the grid is capped, so any tile count fits one launch).  The series
+// pointers are looked up once per run (again only where the run crosses
+// into the next series), and the next tile's samples are requested before
+// the current tile's stores, so each tile's loads overlap the previous
+// tile's work instead of starting a chain of dependent lookups per tile.
+constexpr int kInitRun = 8;
+
 template <typename real, int MODE>
 __global__ __launch_bounds__(kThreads) void k_init_w(SeriesDev sd, InitArgs<real> ia, real* h_out,
                                                      KnotsDev<real> kout)
 {
     constexpr int NW = kThreads / kWave;
     const int lane = threadIdx.x & 63, wv = wave_index();
-    for (int gt = blockIdx.x * NW + wv; gt < sd.ntiles; gt += gridDim.x * NW)
-        init_tile<real, MODE>(sd, ia, h_out, kout, gt, lane);
+    const int64_t nruns = ((int64_t)sd.ntiles + kInitRun - 1) / kInitRun;
+    for (int64_t run = (int64_t)blockIdx.x * NW + wv; run < nruns; run += (int64_t)gridDim.x * NW) {
+        const int g0 = (int)(run * kInitRun);
+        const int g1 = min(g0 + kInitRun, sd.ntiles);
+        InitSeries<real, MODE> q = init_series<real, MODE>(sd, ia, h_out, sd.tile_series[g0]);
+        InitTile<real> cur, nxt;
+        init_load<real, MODE>(q, ia, g0, lane, cur);
+        for (int gt = g0; gt < g1; ++gt) {
+            InitSeries<real, MODE> qn = q;
+            if (gt + 1 < g1) {
+                if (gt + 1 >= q.tend) qn = init_series<real, MODE>(sd, ia, h_out, sd.tile_series[gt + 1]);
+                init_load<real, MODE>(qn, ia, gt + 1, lane, nxt);
+            }
+            init_store<real, MODE>(q, kout, gt, lane, cur);
+            cur = nxt;
+            q = qn;
+        }
+    }
 }
 
 // ================================================================ adjoint
@@ -1772,7 +2068,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) v
         const long lo = g0 - 1;
         const int glo = (int)max(lo, 0L);
         const int ghi = (int)min(lo + W + 2, (long)n);
-        gather_positions<(W + 2 + kWave - 1) / kWave>(a.sd, a.ps.mask, a.ps.pre, a.ps.cnt, s, type,
+        gather_positions<(W + 2 + kWave - 1) / kWave, kTMaskW>(a.sd, a.ps.mask, a.ps.pre, a.ps.cnt, s, type,
                                                       lo, glo, ghi, sp, s_tpre[wv], s_tmask[wv],
                                                       hint);
         wave_sync();
@@ -1820,8 +2116,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) v
         real xs[B];
         partition_pcr<real, B>(
             lane,
-            [&](int r, real& dl, real& d, real& du, real& f) {
-                const int q = r - lane * B;
+            [&](int q, real& dl, real& d, real& du, real& f) {
                 const long g = gb + q;
                 if (g <= 0 || g >= n - 1) {
                     dl = (real)0; d = (real)1; du = (real)0; f = (real)0;
@@ -2209,14 +2504,14 @@ static int resident_groups(K kernel)
 
 template <typename real> struct Geom;
 template <> struct Geom<double> { static constexpr int B = 8, H = 32; };
-template <> struct Geom<float> { static constexpr int B = 8, H = 32; };
+template <> struct Geom<float> { static constexpr int B = 8, H = 16; };
 
 // tile size (samples per tile) is a plan parameter: sd.cap = T/2 + 2
 template <typename real>
 hipError_t launch_init(const SeriesDev& sd, const InitArgs<real>& ia, real* h_out,
                        const KnotsDev<real>& kout, const PassCtl&, hipStream_t st)
 {
-    const dim3 grid(capped_blocks(sd.ntiles, kThreads / kWave));
+    const dim3 grid(capped_blocks(((int64_t)sd.ntiles + kInitRun - 1) / kInitRun, kThreads / kWave));
     switch (ia.mode) {
     case INIT_COPY:
         hipLaunchKernelGGL((k_init_w<real, INIT_COPY>), grid, dim3(kThreads), 0, st, sd, ia, h_out, kout);
@@ -2241,7 +2536,10 @@ template <typename real>
 hipError_t launch_knot_prefix(const SeriesDev& sd, const KnotsDev<real>& k, const real* h,
                               const PassCtl& pc, const int* active, hipStream_t st)
 {
-    if (sd.max_tiles <= kPT)
+    if (sd.max_tiles <= kPWTiles)
+        hipLaunchKernelGGL((k_knot_prefix_w<real>), dim3(blocks_for(sd.nseries, kThreads / kWave)),
+                           dim3(kThreads), 0, st, sd, k, h, pc, active);
+    else if (sd.max_tiles <= kPT)
         hipLaunchKernelGGL((k_knot_prefix_1<real>), dim3(sd.nseries), dim3(kPT), 0, st, sd, k, h,
                            pc, active);
     else
@@ -2300,8 +2598,12 @@ hipError_t launch_sift(const SiftArgs<real>& a, int nsegs, hipStream_t st)
 template <typename real>
 hipError_t launch_solve(const SolveArgs<real>& a, int grid, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_solve<real, Geom<real>::B, Geom<real>::H>), dim3(grid), dim3(kThreads),
-                       0, st, a);
+    if (a.nak)
+        hipLaunchKernelGGL((k_solve<real, Geom<real>::B, Geom<real>::H, true>), dim3(grid),
+                           dim3(kThreads), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_solve<real, Geom<real>::B, Geom<real>::H, false>), dim3(grid),
+                           dim3(kThreads), 0, st, a);
     return hipGetLastError();
 }
 

@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <mutex>
+#include <type_traits>
 
 #include "../../include/iceemdan_mi355x.h"
 #include "icm_errors.h"
@@ -31,8 +32,12 @@ namespace {
 constexpr int64_t kMaxSamples = (int64_t)1 << 21;
 constexpr int kMaxSeries = 4096;
 constexpr int kTile = 256;         // samples per tile of the tile kernels
-constexpr int kSolveH = 32;        // halo rows of the windowed solve
-constexpr int kSolveK = 64 * 8 - 2 * kSolveH;   // rows kept per window
+// windowed solve: halo rows per side (FP32 16, FP64 32) and rows kept per
+// window, as the plans use them
+template <typename real>
+constexpr int solve_h() { return std::is_same<real, float>::value ? 16 : 32; }
+template <typename real>
+constexpr int solve_k() { return 64 * 8 - 2 * solve_h<real>(); }
 constexpr size_t kScratchBytes = (size_t)24 << 20;
 
 __device__ __align__(256) unsigned char scratch[kScratchBytes];
@@ -109,7 +114,7 @@ int sift_static(const real* h, int64_t N64, int64_t I64, real* h_new, int32_t* d
     const int nseg_s = (ntl + seg - 1) / seg;
     const int64_t kcap = N / 2 + 2;
     const int64_t kstride = (int64_t)I * kcap;
-    const int64_t maxwin = (int64_t)I * 2 * ((kcap + kSolveK - 1) / kSolveK);
+    const int64_t maxwin = (int64_t)I * 2 * ((kcap + solve_k<real>() - 1) / solve_k<real>());
 
     void* base = nullptr;
     int rc = scratch_ptr(&base);
@@ -169,8 +174,8 @@ int sift_static(const real* h, int64_t N64, int64_t I64, real* h_new, int32_t* d
         pc.epoch = epoch;
         pc.winlist = winlist;
         pc.wincount = ctlw;
-        pc.K = kSolveK;
-        pc.H = kSolveH;
+        pc.K = solve_k<real>();
+        pc.H = solve_h<real>();
         pc.err = err;
         pc.dense = ctlw + 4;
         pc.run = nullptr;

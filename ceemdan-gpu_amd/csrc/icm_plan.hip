@@ -57,11 +57,15 @@ const int64_t kMaxTiles = (int64_t)1 << 28;
 
 // samples per tile of the tile kernels (one wavefront per tile)
 int tile_of(int) { return 256; }
-// windowed solve geometry (both precisions): 64 lanes x 8 rows, 32 halo rows
-// on each side; the dropped coupling decays by >= 3.7x per row, so 32 rows
-// are below FP64 round-off
+// windowed solve geometry: 64 lanes x 8 rows and H halo rows on each side.
+// A row couples to its neighbours by hl and hr against 2 (hl + hr), so what
+// a window drops decays by 3.7x per row at uniform spacing (slower only
+// along runs of spacings that shrink geometrically).  FP64 keeps 32 rows
+// (its windows equal a global solve to round-off); FP32 keeps 16: against a
+// global solve 6e-11 of the largest coefficient on EEG extrema and 7e-11 at
+// uniform spacing, far below FP32 rounding.
 int solve_B(int) { return 8; }
-int solve_H(int) { return 32; }
+int solve_H(int dtype) { return dtype == ICM_DTYPE_F32 ? 16 : 32; }
 int solve_K(int dtype) { return 64 * solve_B(dtype) - 2 * solve_H(dtype); }
 
 int validate(const icm_config* c)

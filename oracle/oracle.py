@@ -16,9 +16,11 @@ STOP_RILLING = 1
 BC_NATURAL = 0
 BC_NOT_A_KNOT = 1
 
-# default windowed-solve geometry of the FP64 path: 64 lanes x B rows, H halo
+# windowed-solve geometry of the two paths: 64 lanes x B rows, H halo rows
+# on each side (FP32 keeps 16: its truncation, ~1e-10 of the largest
+# coefficient, is far below FP32 rounding)
 GEOM_F64 = (8, 32)
-GEOM_F32 = (8, 32)
+GEOM_F32 = (8, 16)
 
 _lib = None
 

@@ -127,6 +127,26 @@ def test_spline_window_truncation_negligible(orc):
         assert np.max(np.abs(c - ct)) <= 1e-14 * np.max(np.abs(ct))
 
 
+def test_spline_window_truncation_fp32_geometry(orc):
+    """The FP32 path keeps 16 halo rows per side instead of 32.  Against a
+    global Thomas solve the truncation is far below FP32 rounding (6e-8):
+    measured 5.7e-11 (EEG maxima), 5.0e-11 (minima), 7.3e-11 (uniform
+    spacing, the slowest decay of the coupling: 3.7x per row) and 5.6e-14
+    (random spacings) of the largest coefficient; held to 1e-9.  With 32
+    rows the same cases are at round-off (the test above)."""
+    h = eeg("eegSampleDataCH1_244032.f32")
+    cases = [orc.extrema(h, True), orc.extrema(h, False)]
+    rng = np.random.default_rng(5)
+    for hsp in (np.full(5000, 5), rng.integers(1, 300, 5000)):
+        pos = np.concatenate([[0], np.cumsum(hsp)]).astype(np.int32)
+        cases.append((pos, rng.standard_normal(pos.size)))
+    for pos, val in cases:
+        c = orc.spline_solve(pos, val, geom=orc.GEOM_F32)
+        ct = orc.spline_solve_thomas(pos, val)
+        err = np.max(np.abs(c - ct)) / np.max(np.abs(ct))
+        assert 0 < err <= 1e-9, err
+
+
 def test_spline_window_single_window_is_global(orc):
     rng = np.random.default_rng(9)
     pos = np.cumsum(rng.integers(1, 9, 300)).astype(np.int32)
