@@ -1738,14 +1738,17 @@ __device__ __forceinline__ void init_store(const InitSeries<real, MODE>& q, cons
 // tile's work instead of starting a chain of dependent lookups per tile.
 constexpr int kInitRun = 8;
 
-template <typename real, int MODE>
+template <typename real, int MODE, bool LOOP>
 __global__ __launch_bounds__(kThreads) void k_init_w(SeriesDev sd, InitArgs<real> ia, real* h_out,
                                                      KnotsDev<real> kout)
 {
     constexpr int NW = kThreads / kWave;
     const int lane = threadIdx.x & 63, wv = wave_index();
     const int64_t nruns = ((int64_t)sd.ntiles + kInitRun - 1) / kInitRun;
-    for (int64_t run = (int64_t)blockIdx.x * NW + wv; run < nruns; run += (int64_t)gridDim.x * NW) {
+    // one run per wavefront, or (LOOP) a grid-stride loop over the runs
+    // where that grid would exceed the capped one
+    for (int64_t run = (int64_t)blockIdx.x * NW + wv; run < nruns;
+         run = LOOP ? run + (int64_t)gridDim.x * NW : nruns) {
         const int g0 = (int)(run * kInitRun);
         const int g1 = min(g0 + kInitRun, sd.ntiles);
         InitSeries<real, MODE> q = init_series<real, MODE>(sd, ia, h_out, sd.tile_series[g0]);
@@ -1962,15 +1965,24 @@ __device__ __forceinline__ void adj_tile(const AdjTileArgs<real>& a, int gt, int
     }
 }
 
-// one wavefront per tile, grid-stride over the tiles; the wave's LDS slice
-// is reused by its next tile only after a wave barrier
-template <typename real>
+// One wavefront per tile (LOOP = false), or a grid-stride loop over the
+// tiles where one wave per tile would exceed the capped grid (the wave's
+// LDS slice is reused by its next tile only after a wave barrier).  The
+// loop keeps the arguments live across its iterations, which spilled
+// scalar registers (20, restored through v_readlane every tile) and made
+// the adjoint tile step 44 % slower, so it is used only where it must be.
+template <typename real, bool LOOP>
 __global__ __launch_bounds__(kThreads) void k_adj_tile(AdjTileArgs<real> a)
 {
     constexpr int NW = kThreads / kWave;
     __shared__ real s_delta[NW][kWT];
     __shared__ int s_tp[NW][kAdjKC];
     const int lane = threadIdx.x & 63, wv = wave_index();
+    if (!LOOP) {
+        const int gt = blockIdx.x * NW + wv;
+        if (gt < a.sd.ntiles) adj_tile<real>(a, gt, lane, s_delta[wv], s_tp[wv]);
+        return;
+    }
     for (int gt = blockIdx.x * NW + wv; gt < a.sd.ntiles; gt += gridDim.x * NW) {
         adj_tile<real>(a, gt, lane, s_delta[wv], s_tp[wv]);
         wave_sync();
@@ -2507,27 +2519,31 @@ template <> struct Geom<double> { static constexpr int B = 8, H = 32; };
 template <> struct Geom<float> { static constexpr int B = 8, H = 16; };
 
 // tile size (samples per tile) is a plan parameter: sd.cap = T/2 + 2
+template <typename real, int MODE>
+static void launch_init_mode(bool loop, dim3 grid, hipStream_t st, const SeriesDev& sd,
+                             const InitArgs<real>& ia, real* h_out, const KnotsDev<real>& kout)
+{
+    if (loop)
+        hipLaunchKernelGGL((k_init_w<real, MODE, true>), grid, dim3(kThreads), 0, st, sd, ia, h_out,
+                           kout);
+    else
+        hipLaunchKernelGGL((k_init_w<real, MODE, false>), grid, dim3(kThreads), 0, st, sd, ia, h_out,
+                           kout);
+}
+
 template <typename real>
 hipError_t launch_init(const SeriesDev& sd, const InitArgs<real>& ia, real* h_out,
                        const KnotsDev<real>& kout, const PassCtl&, hipStream_t st)
 {
-    const dim3 grid(capped_blocks(((int64_t)sd.ntiles + kInitRun - 1) / kInitRun, kThreads / kWave));
+    const int64_t runs = ((int64_t)sd.ntiles + kInitRun - 1) / kInitRun;
+    const dim3 grid(capped_blocks(runs, kThreads / kWave));
+    const bool loop = (int64_t)grid.x * (kThreads / kWave) < runs;
     switch (ia.mode) {
-    case INIT_COPY:
-        hipLaunchKernelGGL((k_init_w<real, INIT_COPY>), grid, dim3(kThreads), 0, st, sd, ia, h_out, kout);
-        break;
-    case INIT_RESID:
-        hipLaunchKernelGGL((k_init_w<real, INIT_RESID>), grid, dim3(kThreads), 0, st, sd, ia, h_out, kout);
-        break;
-    case INIT_STAGE0:
-        hipLaunchKernelGGL((k_init_w<real, INIT_STAGE0>), grid, dim3(kThreads), 0, st, sd, ia, h_out, kout);
-        break;
-    case INIT_KNOTS:
-        hipLaunchKernelGGL((k_init_w<real, INIT_KNOTS>), grid, dim3(kThreads), 0, st, sd, ia, h_out, kout);
-        break;
-    default:
-        hipLaunchKernelGGL((k_init_w<real, INIT_STAGEK>), grid, dim3(kThreads), 0, st, sd, ia, h_out, kout);
-        break;
+    case INIT_COPY: launch_init_mode<real, INIT_COPY>(loop, grid, st, sd, ia, h_out, kout); break;
+    case INIT_RESID: launch_init_mode<real, INIT_RESID>(loop, grid, st, sd, ia, h_out, kout); break;
+    case INIT_STAGE0: launch_init_mode<real, INIT_STAGE0>(loop, grid, st, sd, ia, h_out, kout); break;
+    case INIT_KNOTS: launch_init_mode<real, INIT_KNOTS>(loop, grid, st, sd, ia, h_out, kout); break;
+    default: launch_init_mode<real, INIT_STAGEK>(loop, grid, st, sd, ia, h_out, kout); break;
     }
     return hipGetLastError();
 }
@@ -2695,8 +2711,12 @@ template <typename real>
 hipError_t launch_adj_tile(const AdjTileArgs<real>& a, hipStream_t st)
 {
     if (2 * (a.sd.cap - 2) != kWT) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_adj_tile<real>), dim3(capped_blocks(a.sd.ntiles, kThreads / kWave)),
-                       dim3(kThreads), 0, st, a);
+    const int64_t need = ((int64_t)a.sd.ntiles + kThreads / kWave - 1) / (kThreads / kWave);
+    if (need == (int64_t)capped_blocks(a.sd.ntiles, kThreads / kWave))
+        hipLaunchKernelGGL((k_adj_tile<real, false>), dim3((unsigned)need), dim3(kThreads), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_adj_tile<real, true>), dim3(capped_blocks(a.sd.ntiles, kThreads / kWave)),
+                           dim3(kThreads), 0, st, a);
     return hipGetLastError();
 }
 
