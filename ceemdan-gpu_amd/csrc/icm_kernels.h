@@ -184,8 +184,10 @@ struct AdjTileArgs {
     real* edge;
 };
 
+// loop = true takes the grid-stride instance even where one wavefront per
+// tile fits a launch (for testing that instance)
 template <typename real>
-hipError_t launch_adj_tile(const AdjTileArgs<real>& a, hipStream_t st);
+hipError_t launch_adj_tile(const AdjTileArgs<real>& a, hipStream_t st, bool loop = false);
 template <typename real>
 hipError_t launch_adj_span_knot(const AdjArgs<real>& a, int solve_grid, hipStream_t st);
 template <typename real>
@@ -201,9 +203,12 @@ hipError_t launch_adj_input(int nchan, const int* chan_len, const int64_t* chan_
                             const real* gxt, real* gy, hipStream_t st);
 
 // host-side launchers (icm_kernels.hip)
+// loop = true: the grid-stride instance even where one wavefront per run of
+// tiles fits a launch (for testing that instance)
 template <typename real>
 hipError_t launch_init(const SeriesDev& sd, const InitArgs<real>& ia, real* h_out,
-                       const KnotsDev<real>& kout, const PassCtl& pc, hipStream_t st);
+                       const KnotsDev<real>& kout, const PassCtl& pc, hipStream_t st,
+                       bool loop = false);
 template <typename real>
 hipError_t launch_sift(const SiftArgs<real>& a, int nsegs, hipStream_t st);
 template <typename real>

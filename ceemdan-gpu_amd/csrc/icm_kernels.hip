@@ -1746,7 +1746,7 @@ __global__ __launch_bounds__(kThreads) void k_init_w(SeriesDev sd, InitArgs<real
     const int lane = threadIdx.x & 63, wv = wave_index();
     const int64_t nruns = ((int64_t)sd.ntiles + kInitRun - 1) / kInitRun;
     // one run per wavefront, or (LOOP) a grid-stride loop over the runs
-    // where that grid would exceed the capped one
+    // where that grid would exceed a launch's 2^32 work-items
     for (int64_t run = (int64_t)blockIdx.x * NW + wv; run < nruns;
          run = LOOP ? run + (int64_t)gridDim.x * NW : nruns) {
         const int g0 = (int)(run * kInitRun);
@@ -1966,11 +1966,11 @@ __device__ __forceinline__ void adj_tile(const AdjTileArgs<real>& a, int gt, int
 }
 
 // One wavefront per tile (LOOP = false), or a grid-stride loop over the
-// tiles where one wave per tile would exceed the capped grid (the wave's
-// LDS slice is reused by its next tile only after a wave barrier).  The
-// loop keeps the arguments live across its iterations, which spilled
-// scalar registers (20, restored through v_readlane every tile) and made
-// the adjoint tile step 44 % slower, so it is used only where it must be.
+// tiles where one wave per tile would exceed a launch's 2^32 work-items
+// (the wave's LDS slice is reused by its next tile only after a wave
+// barrier).  The loop keeps the arguments live across its iterations, which
+// spilled scalar registers (20, restored through v_readlane every tile) and
+// made the adjoint tile step 44 % slower, so it is used only where it must.
 template <typename real, bool LOOP>
 __global__ __launch_bounds__(kThreads) void k_adj_tile(AdjTileArgs<real> a)
 {
@@ -2489,6 +2489,13 @@ static inline unsigned capped_blocks(int64_t n, int b)
     const int64_t g = (n + b - 1) / b;
     return (unsigned)(g < 1 ? 1 : (g > (1 << 20) ? (1 << 20) : g));
 }
+// workgroups of kThreads for n items, b per workgroup, if that grid fits one
+// launch (fewer than 2^32 work-items); else 0 (the grid-stride instance)
+static inline unsigned full_blocks(int64_t n, int b)
+{
+    const int64_t g = std::max<int64_t>(1, (n + b - 1) / b);
+    return g * kThreads < ((int64_t)1 << 32) ? (unsigned)g : 0u;
+}
 
 // Workgroups of `kernel` (kThreads each) that the device keeps resident at
 // once, per device (the grid of a persistent launch).
@@ -2533,11 +2540,12 @@ static void launch_init_mode(bool loop, dim3 grid, hipStream_t st, const SeriesD
 
 template <typename real>
 hipError_t launch_init(const SeriesDev& sd, const InitArgs<real>& ia, real* h_out,
-                       const KnotsDev<real>& kout, const PassCtl&, hipStream_t st)
+                       const KnotsDev<real>& kout, const PassCtl&, hipStream_t st, bool force_loop)
 {
     const int64_t runs = ((int64_t)sd.ntiles + kInitRun - 1) / kInitRun;
-    const dim3 grid(capped_blocks(runs, kThreads / kWave));
-    const bool loop = (int64_t)grid.x * (kThreads / kWave) < runs;
+    const unsigned full = force_loop ? 0u : full_blocks(runs, kThreads / kWave);
+    const bool loop = full == 0;
+    const dim3 grid(loop ? capped_blocks(runs, kThreads / kWave) : full);
     switch (ia.mode) {
     case INIT_COPY: launch_init_mode<real, INIT_COPY>(loop, grid, st, sd, ia, h_out, kout); break;
     case INIT_RESID: launch_init_mode<real, INIT_RESID>(loop, grid, st, sd, ia, h_out, kout); break;
@@ -2708,12 +2716,11 @@ hipError_t launch_select_copy(const SeriesDev& sd, int max_len, const real* hA, 
 }
 
 template <typename real>
-hipError_t launch_adj_tile(const AdjTileArgs<real>& a, hipStream_t st)
+hipError_t launch_adj_tile(const AdjTileArgs<real>& a, hipStream_t st, bool force_loop)
 {
     if (2 * (a.sd.cap - 2) != kWT) return hipErrorInvalidValue;
-    const int64_t need = ((int64_t)a.sd.ntiles + kThreads / kWave - 1) / (kThreads / kWave);
-    if (need == (int64_t)capped_blocks(a.sd.ntiles, kThreads / kWave))
-        hipLaunchKernelGGL((k_adj_tile<real, false>), dim3((unsigned)need), dim3(kThreads), 0, st, a);
+    if (const unsigned full = force_loop ? 0u : full_blocks(a.sd.ntiles, kThreads / kWave))
+        hipLaunchKernelGGL((k_adj_tile<real, false>), dim3(full), dim3(kThreads), 0, st, a);
     else
         hipLaunchKernelGGL((k_adj_tile<real, true>), dim3(capped_blocks(a.sd.ntiles, kThreads / kWave)),
                            dim3(kThreads), 0, st, a);
@@ -2800,7 +2807,8 @@ hipError_t launch_check_index(const int* d_x, int64_t N, int* err, hipStream_t s
 
 #define ICM_INSTANTIATE(real)                                                                      \
     template hipError_t launch_init<real>(const SeriesDev&, const InitArgs<real>&, real*,          \
-                                          const KnotsDev<real>&, const PassCtl&, hipStream_t);     \
+                                          const KnotsDev<real>&, const PassCtl&, hipStream_t,      \
+                                          bool);                                                   \
     template hipError_t launch_sift<real>(const SiftArgs<real>&, int, hipStream_t);                \
     template hipError_t launch_knot_prefix<real>(const SeriesDev&, const KnotsDev<real>&,          \
                                                  const real*, const PassCtl&, const int*,          \
@@ -2827,7 +2835,7 @@ hipError_t launch_check_index(const int* d_x, int64_t N, int* err, hipStream_t s
                                                  const int*, real*, hipStream_t);
 
 #define ICM_INSTANTIATE_ADJ(real)                                                                  \
-    template hipError_t launch_adj_tile<real>(const AdjTileArgs<real>&, hipStream_t);              \
+    template hipError_t launch_adj_tile<real>(const AdjTileArgs<real>&, hipStream_t, bool);        \
     template hipError_t launch_adj_span_knot<real>(const AdjArgs<real>&, int, hipStream_t);        \
     template hipError_t launch_adj_seed<real>(const real*, int, int64_t, int64_t, real*,           \
                                               hipStream_t);                                        \

@@ -259,6 +259,7 @@ struct PlanBase {
     hipEvent_t ev_fork = nullptr, ev_sig = nullptr;
     bool overlap_on = true;
     bool sift_queue = false;          // every sift pass from a work queue (ICM_SIFT_QUEUE=1)
+    bool grid_loop = false;           // grid-stride start / adjoint tile (ICM_GRID_LOOP=1)
     int* h_err = nullptr;             // pinned
     // rilling
     int *active = nullptr, *stop_pass = nullptr, *subs = nullptr, *sx_cnt = nullptr,
@@ -766,6 +767,10 @@ struct Plan : PlanBase {
         // ICM_SIFT_QUEUE=1: every sift pass from the work queue on a resident
         // grid (else only passes whose one-wave-per-segment grid is too large)
         if (const char* env = std::getenv("ICM_SIFT_QUEUE")) sift_queue = std::atoi(env) != 0;
+        // ICM_GRID_LOOP=1: the stage start and the adjoint tile step as
+        // grid-stride loops (else only where one wavefront per item would
+        // exceed a launch's 2^32 work-items)
+        if (const char* env = std::getenv("ICM_GRID_LOOP")) grid_loop = std::atoi(env) != 0;
         tune_init();
         return ICM_OK;
     }
@@ -911,7 +916,7 @@ struct Plan : PlanBase {
         const int64_t pass_bytes = 2 * (int64_t)sizeof(real) * d.samples;
         uint32_t e_emit = next_epoch(L);
         if ((rc = prof_begin(0, pass_bytes, st))) return rc;
-        ICM_TRY(launch_init<real>(sd, ia, L.hA, knots(L, 0), ctl(L, e_emit), st));
+        ICM_TRY(launch_init<real>(sd, ia, L.hA, knots(L, 0), ctl(L, e_emit), st, grid_loop));
         if ((rc = prof_end(st))) return rc;
         ICM_TRY(launch_knot_prefix<real>(sd, knots(L, 0), L.hA, ctl(L, e_emit), nullptr, st));
         if ((rc = trace_copy(phase, stage, 0, d, 0, st))) return rc;
@@ -995,7 +1000,7 @@ struct Plan : PlanBase {
             if (rc) return rc;
         }
         uint32_t e_emit = next_epoch(L);
-        ICM_TRY(launch_init<real>(sd, ia, L.hA, knots(L, 0), ctl(L, e_emit), st));
+        ICM_TRY(launch_init<real>(sd, ia, L.hA, knots(L, 0), ctl(L, e_emit), st, grid_loop));
         ICM_TRY(launch_knot_prefix<real>(sd, knots(L, 0), L.hA, ctl(L, e_emit), nullptr, st));
         int rc;
         if ((rc = trace_copy(phase, stage, 0, d, 0, st))) return rc;
@@ -1626,7 +1631,7 @@ struct Plan : PlanBase {
                 ta.mo = rec_of(j);
                 ta.mom = mom;
                 ta.edge = edge;
-                ICM_TRY(launch_adj_tile<real>(ta, st));
+                ICM_TRY(launch_adj_tile<real>(ta, st, grid_loop));
                 if (j >= 0) ICM_TRY(launch_adj_span_knot<real>(adj_args(j), solve_grid(), st));
             }
             ICM_TRY(launch_adj_stage<real>(k, C, I, max_chan_len, d_chan_len, d_chan_off, sd, lam,

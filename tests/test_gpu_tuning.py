@@ -1,4 +1,5 @@
-"""The sift autotuner and the work-queue sift on the GPU.
+"""The sift autotuner, the work-queue sift and the grid-stride tile kernels
+on the GPU.
 
 Every tuning candidate (segment length x knot-table size) must compute the
 same bits as the default geometry, per stage and mixed across stages, in
@@ -104,3 +105,49 @@ def test_autotune_run_and_table(icm, dev):
     again.close()
     for a, b in zip(out, ref):
         assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("stop", [0, 1])
+def test_queue_and_grid_stride_instances_match(icm, orc, dev, monkeypatch, stop):
+    """The instances a plan uses only past a launch's 2^32 work-items (the
+    work-queue sift over a resident grid, the grid-stride stage start and
+    adjoint tile step), forced on a small plan with ICM_SIFT_QUEUE=1 and
+    ICM_GRID_LOOP=1: the same forward bits as the default instances, equal
+    to the oracle per channel (FP64), and in FIXED mode the same adjoint
+    bits."""
+    import torch
+    lens, I, M, S = [70000, 3000], 3, 4, 5
+    sig = _sig()
+    xs = [sig.eeg_like(n, seed=320 + c) for c, n in enumerate(lens)]
+    xd = [torch.tensor(x, device=dev) for x in xs]
+    rng = np.random.default_rng(7)
+    grads = [torch.tensor(rng.standard_normal((M, n)), device=dev) for n in lens]
+    print(f"case: queue + grid-stride instances stop={stop} lens={lens}", flush=True)
+
+    def run(forced):
+        for k in ("ICM_SIFT_QUEUE", "ICM_GRID_LOOP"):
+            if forced:
+                monkeypatch.setenv(k, "1")
+            else:
+                monkeypatch.delenv(k, raising=False)
+        plan = icm.Plan(lens, I, M, dtype=torch.float64, stop_mode=stop, max_sifts=S,
+                        noise_max_sifts=S if stop == 0 else 50, use_graph=False, device=dev,
+                        keep_records=stop == 0)
+        out = [o.clone() for o in plan.forward(xd)]
+        g = [t.clone() for t in plan.backward(grads)] if stop == 0 else []
+        plan.close()
+        return out, g
+
+    ref_out, ref_g = run(False)
+    out, g = run(True)
+    monkeypatch.delenv("ICM_SIFT_QUEUE", raising=False)
+    monkeypatch.delenv("ICM_GRID_LOOP", raising=False)
+    for a, b in zip(out, ref_out):
+        assert torch.equal(a, b)
+    for a, b in zip(g, ref_g):
+        assert torch.equal(a, b)
+    for c, (n, x) in enumerate(zip(lens, xs)):
+        w = icm.noise(I, n, seed=1234, dtype=torch.float64, device=dev).cpu().numpy()
+        ref = orc.iceemdan(x, w, M, stop_mode=stop, S_or_cap=S,
+                           noise_cap=S if stop == 0 else 50)["imfs"]
+        assert np.array_equal(out[c].cpu().numpy(), ref), c
