@@ -66,6 +66,27 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_wave_barrier();
 }
 
+// Bitmasks of the strict maxima and minima among one row of 64 samples
+// (v against its neighbours lft, rgt).  The conditions are combined with
+// bitwise operators, so they stay lane masks combined by scalar
+// instructions: with short-circuit && and || the compiler turned each flag
+// into a 0/1 vector value and compared it again (two more vector
+// instructions per flag, and more scalar registers held, which spilled).
+// `valid` and `end` are the lanes inside the series and at its two ends
+// (ends are knots of both types); FULL rows have every lane valid, no end.
+template <bool FULL, typename real>
+__device__ __forceinline__ void row_extrema(real v, real lft, real rgt, bool valid, bool end,
+                                            uint64_t& mx, uint64_t& mn)
+{
+    if (FULL) {
+        mx = wave_ballot((v > lft) & (v > rgt));
+        mn = wave_ballot((v < lft) & (v < rgt));
+    } else {
+        mx = wave_ballot(valid & (end | ((v > lft) & (v > rgt))));
+        mn = wave_ballot(valid & (end | ((v < lft) & (v < rgt))));
+    }
+}
+
 // tile offset of the first set bit, -1 if none
 __device__ __forceinline__ int mask_first(const uint64_t (&m)[kWR])
 {
@@ -343,11 +364,12 @@ __global__ __launch_bounds__(kPT) void k_knot_prefix_1(SeriesDev sd, KnotsDev<re
     // both types' counts, then both types' bitmasks, in one round trip each
     c[0] = t < nt ? k.cnt[2 * (tb + t)] : 0;
     c[1] = t < nt ? k.cnt[2 * (tb + t) + 1] : 0;
+    // the bitmasks loaded with the counts (not after them, for the tiles
+    // with knots only): one round trip fewer per workgroup
     uint64_t mm[2][kWR];
-    if (c[0] > 0 || c[1] > 0) {
 #pragma unroll
-        for (int q = 0; q < 2 * kWR; ++q) mm[q / kWR][q % kWR] = k.mask[8 * (int64_t)(tb + t) + q];
-    }
+    for (int q = 0; q < 2 * kWR; ++q)
+        mm[q / kWR][q % kWR] = t < nt ? k.mask[8 * (int64_t)(tb + min(t, nt - 1)) + q] : 0ull;
 #pragma unroll
     for (int ty = 0; ty < 2; ++ty) {
         sc[ty] = lt_none();
@@ -448,6 +470,14 @@ __global__ __launch_bounds__(kPT) void k_knot_prefix_1(SeriesDev sd, KnotsDev<re
         for (int ty = 0; ty < 2; ++ty)
 #pragma unroll
             for (int w = 0; w < 3; ++w) ys[ty][w] = hs[max(xs[ty][w], 0)];
+        if (tid == 0) {
+            // the window base is requested while the values are in flight
+            const int n0 = s_all[0], n1 = s_all[1];
+            k.n[2 * s] = n0;
+            k.n[2 * s + 1] = n1;
+            s_wbase = atomicAdd(&pc.wincount[pc.epoch],
+                                (unsigned)((n0 + pc.K - 1) / pc.K + (n1 + pc.K - 1) / pc.K));
+        }
 #pragma unroll
         for (int ty = 0; ty < 2; ++ty) {
             k.pre[2 * (tb + t) + ty] = ex[ty];
@@ -463,10 +493,12 @@ __global__ __launch_bounds__(kPT) void k_knot_prefix_1(SeriesDev sd, KnotsDev<re
     const int n0 = s_all[0], n1 = s_all[1];
     const int wmx = (n0 + pc.K - 1) / pc.K;
     const int wmn = (n1 + pc.K - 1) / pc.K;
-    if (tid == 0) {
+    if (tid == 0 && nt == 0) {
         k.n[2 * s] = n0;
         k.n[2 * s + 1] = n1;
         s_wbase = atomicAdd(&pc.wincount[pc.epoch], (unsigned)(wmx + wmn));
+    }
+    if (tid == 0) {
         unsigned d = 0;
         for (int w = 0; w < NWV; ++w) d += s_dense[w];
         if (d) atomicAdd(&pc.dense[pc.epoch], d);
@@ -836,7 +868,7 @@ __device__ __forceinline__ void gather_positions(const SeriesDev& sd, const uint
         const int pend = __shfl(p + c, last, kWave);
         const int gend = min(ghi, pend);
         // chunk tiles holding the first and the last knot of [g, gend)
-        const uint64_t ua = wave_ballot(valid && p <= g), ub = wave_ballot(valid && p <= gend - 1);
+        const uint64_t ua = wave_ballot(valid & (p <= g)), ub = wave_ballot(valid & (p <= gend - 1));
         const int xa = ua ? 63 - __builtin_clzll(ua) : 0;
         const int xb = ub ? 63 - __builtin_clzll(ub) : 0;
         wave_sync();
@@ -1461,7 +1493,7 @@ __device__ __forceinline__ void sift_segment(const SiftArgs<real>& a, const Segm
                     if (RILL) {
                         const real sx = fabs((U + L) / (U - L));
                         sxc += (sx > (real)0.05) ? 1 : 0;
-                        sxm = sxm || (sx > (real)0.5);
+                        sxm |= sx > (real)0.5;
                     }
                 }
             }
@@ -1502,16 +1534,10 @@ __device__ __forceinline__ void sift_segment(const SiftArgs<real>& a, const Segm
                     const int p = t0 + u;
                     const real lft = shift_prev(hn[r], r == 0 ? hnl : lane_value(hn[r > 0 ? r - 1 : 0], kWave - 1));
                     const real rgt = shift_next(hn[r], r == kWR - 1 ? hnr : lane_value(hn[r < kWR - 1 ? r + 1 : r], 0));
-                    const bool valid = FULL || u < nt;
-                    const bool end = !FULL && (p == 0 || p == N - 1);
-                    const real v = hn[r];
-                    const bool f[2] = {valid && (end || ((v > lft) && (v > rgt))),
-                                       valid && (end || ((v < lft) && (v < rgt)))};
+                    row_extrema<FULL>(hn[r], lft, rgt, FULL || u < nt,
+                                      !FULL && (p == 0 || p == N - 1), em[0][r], em[1][r]);
 #pragma unroll
-                    for (int ty = 0; ty < 2; ++ty) {
-                        em[ty][r] = wave_ballot(f[ty]);
-                        run[ty] += __popcll(em[ty][r]);
-                    }
+                    for (int ty = 0; ty < 2; ++ty) run[ty] += __popcll(em[ty][r]);
                 }
                 store_masks(a.kout.mask + 8 * (int64_t)gt, em, lane);
                 if (lane == 0) {
@@ -1712,16 +1738,9 @@ __device__ __forceinline__ void init_store(const InitSeries<real, MODE>& q, cons
         const int p = t0 + u;
         const real lft = shift_prev(w.x[r], r == 0 ? w.xl : lane_value(w.x[r > 0 ? r - 1 : 0], kWave - 1));
         const real rgt = shift_next(w.x[r], r == kWR - 1 ? w.xr : lane_value(w.x[r < kWR - 1 ? r + 1 : r], 0));
-        const bool valid = u < nt;
-        const bool end = p == 0 || p == q.N - 1;
-        const real v = w.x[r];
-        const bool f[2] = {valid && (end || ((v > lft) && (v > rgt))),
-                           valid && (end || ((v < lft) && (v < rgt)))};
+        row_extrema<false>(w.x[r], lft, rgt, u < nt, p == 0 || p == q.N - 1, em[0][r], em[1][r]);
 #pragma unroll
-        for (int ty = 0; ty < 2; ++ty) {
-            em[ty][r] = wave_ballot(f[ty]);
-            run[ty] += __popcll(em[ty][r]);
-        }
+        for (int ty = 0; ty < 2; ++ty) run[ty] += __popcll(em[ty][r]);
     }
     store_masks(kout.mask + 8 * (int64_t)gt, em, lane);
     if (lane == 0) {
