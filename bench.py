@@ -364,6 +364,9 @@ def main():
     ap.add_argument("--c5", action="store_true", help="also time the C5 share forward + adjoint")
     ap.add_argument("--forward-only", action="store_true",
                     help="only the timed forwards of the workload (for profiler runs)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the forward + adjoint timing of the workload, one JSON line "
+                         "(for profiler runs)")
     args = ap.parse_args()
     if args.forward_only:
         args.no_cpu = args.no_per_signal = args.no_extra = True
@@ -383,6 +386,10 @@ def main():
     W = WORKLOADS[name]
     xs = signals_for(name, sig)
     lengths = [x.size for x in xs]
+    if args.adjoint_only:
+        print(json.dumps({"workload": name, **backward_workload(icm, torch, dev, name, xs,
+                                                                args.steps)}))
+        return
 
     main_res = forward_workload(icm, torch, dev, name, xs, args.steps, args.warmup)
     main_res["plan"].close()
