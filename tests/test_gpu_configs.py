@@ -88,6 +88,38 @@ def test_c3_channels_fp64_bitexact(icm, orc, dev):
         assert np.array_equal(got, ref["imfs"]), (c, relerr(got, ref["imfs"]))
 
 
+@pytest.mark.parametrize("stop", [0, 1])
+def test_short_series_one_wave_prefix_bitexact(icm, orc, dev, stop):
+    """Series of at most 128 tiles take the one-wavefront knot prefix (one
+    or two chunks of 64 tiles per wave; 20, 79, 118 and 128 tiles here):
+    modes equal the oracle's per channel, bit for bit, in FIXED mode with
+    the knot counts of every pass too, and in RILLING mode (where the
+    prefix skips the series that stopped)."""
+    import torch
+    sig = _sig()
+    lens, I, M, S = [5000, 20000, 30000, 32768], 3, 5, 6
+    xs = [sig.eeg_like(n, seed=700 + c) for c, n in enumerate(lens)]
+    print(f"case: one-wave prefix stop={stop} lens={lens}", flush=True)
+    cap, ncap = 40, 60
+    plan = icm.Plan(lens, I, M, dtype=torch.float64, stop_mode=stop,
+                    max_sifts=S if stop == 0 else cap, noise_max_sifts=S if stop == 0 else ncap,
+                    use_graph=False, device=dev)
+    if stop == 0:
+        plan.enable_trace(S)
+    out = plan.forward([torch.tensor(x, device=dev) for x in xs])
+    tr = plan.trace(icm.PHASE_SIGNAL) if stop == 0 else None
+    plan.close()
+    for c, (n, x) in enumerate(zip(lens, xs)):
+        w = _gpu_noise(icm, dev, I, n)
+        if stop == 0:
+            ref = orc.iceemdan(x, w, M, S_or_cap=S, trace_passes=S)
+            assert np.array_equal(tr[:, :, c * I:(c + 1) * I], ref["trace"][1]), c
+        else:
+            ref = orc.iceemdan(x, w, M, stop_mode=orc.STOP_RILLING, S_or_cap=cap, noise_cap=ncap)
+        got = out[c].cpu().numpy()
+        assert np.array_equal(got, ref["imfs"]), (c, relerr(got, ref["imfs"]))
+
+
 def test_c3_channels_fp32_vs_oracle(icm, orc, dev):
     """The same channels in FP32.  The oracle is fed the same float32 input
     and the plan's float32 noise: the first pass of every noise mode sees
