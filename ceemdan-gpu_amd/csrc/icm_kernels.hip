@@ -1272,6 +1272,33 @@ __device__ __forceinline__ void store_masks(uint64_t* dst, const uint64_t (&m)[2
     }
 }
 
+// Per-wave knot table of a tile (KC entries per type): each knot's position
+// and the coefficients of the interval that starts at it.  The FP32 small
+// table keeps one 32-byte record per knot (coefficients, then the
+// position), so that a knot's position and coefficients are read at one
+// LDS address; the others keep two arrays (records would cost the full
+// tables waves per SIMD: their LDS would grow from 22 to 35 KB per
+// workgroup in FP32, to 52 KB in FP64).
+template <typename real, int KC, bool REC>
+struct KnotTableOf {
+    int pos[2][KC];
+    Coef<real> q[2][KC];
+    __device__ __forceinline__ int& P(int ty, int j) { return pos[ty][j]; }
+    __device__ __forceinline__ Coef<real>& Q(int ty, int j) { return q[ty][j]; }
+};
+template <typename real, int KC>
+struct KnotTableOf<real, KC, true> {
+    struct alignas(32) Rec {
+        Coef<real> q;
+        int pos;
+    };
+    Rec r[2][KC];
+    __device__ __forceinline__ int& P(int ty, int j) { return r[ty][j].pos; }
+    __device__ __forceinline__ Coef<real>& Q(int ty, int j) { return r[ty][j].q; }
+};
+template <typename real, int KC>
+using KnotTable = KnotTableOf<real, KC, std::is_same<real, float>::value && KC == kWKS>;
+
 template <typename real, int KC>
 struct WTile {
     static constexpr int NC = (KC + kWave - 1) / kWave;
@@ -1291,7 +1318,7 @@ struct WTile {
 // tile after it takes h_new of its left neighbour from its own knots.
 template <typename real, bool RILL, int KC>
 __device__ __forceinline__ void sift_segment(const SiftArgs<real>& a, const Segment* segs, int sid,
-                                             int lane, int (*tp)[KC], Coef<real> (*tc)[KC])
+                                             int lane, KnotTable<real, KC>& T)
 {
     const SeriesDev& sd = a.sd;
     const Segment seg = segs[sid];
@@ -1419,8 +1446,8 @@ __device__ __forceinline__ void sift_segment(const SiftArgs<real>& a, const Segm
                             if (g == 0) y = fx[ty][0];
                             if (g == nn[ty] - 1) y = fx[ty][1];
                         }
-                        tp[ty][j] = p;
-                        tc[ty][j].a = y;
+                        T.P(ty, j) = p;
+                        T.Q(ty, j).a = y;
                     }
                     rb[ty] += __popcll(bm[ty][r]);
                 }
@@ -1433,8 +1460,8 @@ __device__ __forceinline__ void sift_segment(const SiftArgs<real>& a, const Segm
                     real y = cur.nby;
                     if (g == 0) y = fx[ty][0];
                     if (g == nn[ty] - 1) y = fx[ty][1];
-                    tp[ty][j] = cur.nbx;
-                    tc[ty][j].a = y;
+                    T.P(ty, j) = cur.nbx;
+                    T.Q(ty, j).a = y;
                 }
             }
 #pragma unroll
@@ -1442,7 +1469,7 @@ __device__ __forceinline__ void sift_segment(const SiftArgs<real>& a, const Segm
 #pragma unroll
                 for (int k = 0; k < WTile<real, KC>::NC; ++k) {
                     const int j = lane + k * kWave;
-                    if (j < cnt[ty] + 3) tc[ty][j].c = cur.c[ty][k];
+                    if (j < cnt[ty] + 3) T.Q(ty, j).c = cur.c[ty][k];
                 }
             wave_sync();
             // ---- interval coefficients (spline_coefficients, cudaICEEMDAN.cu:394-398)
@@ -1452,14 +1479,14 @@ __device__ __forceinline__ void sift_segment(const SiftArgs<real>& a, const Segm
                 const int j = ty ? q - (cnt[0] + 2) : q;
                 const int g = pre[ty] - 2 + j;
                 if (g < 0 || g + 1 > nn[ty] - 1) continue;
-                const real hk = (real)(tp[ty][j + 1] - tp[ty][j]);
-                const real a0 = tc[ty][j].a, a1 = tc[ty][j + 1].a;
-                const real c0 = tc[ty][j].c, c1 = tc[ty][j + 1].c;
+                const real hk = (real)(T.P(ty, j + 1) - T.P(ty, j));
+                const real a0 = T.Q(ty, j).a, a1 = T.Q(ty, j + 1).a;
+                const real c0 = T.Q(ty, j).c, c1 = T.Q(ty, j + 1).c;
                 if (std::is_same<real, double>::value) {
                     // the reference's expressions, correctly rounded divisions:
                     // FP64 equals the oracle bit for bit
-                    tc[ty][j].b = (a1 - a0) / hk - hk * ((real)2 * c0 + c1) / (real)3;
-                    tc[ty][j].d = (c1 - c0) / ((real)3 * hk);
+                    T.Q(ty, j).b = (a1 - a0) / hk - hk * ((real)2 * c0 + c1) / (real)3;
+                    T.Q(ty, j).d = (c1 - c0) / ((real)3 * hk);
                 } else {
                     // FP32 (compared with the FP64 oracle by its error, not
                     // bit for bit): the hardware reciprocal of the spacing
@@ -1468,8 +1495,8 @@ __device__ __forceinline__ void sift_segment(const SiftArgs<real>& a, const Segm
                     // 9 % of the sift pass's time at the 64-channel FP32 job
                     const real rh = (real)__builtin_amdgcn_rcpf((float)hk);
                     const real third = (real)(1.0 / 3.0);
-                    tc[ty][j].b = (a1 - a0) * rh - hk * ((real)2 * c0 + c1) * third;
-                    tc[ty][j].d = (c1 - c0) * third * rh;
+                    T.Q(ty, j).b = (a1 - a0) * rh - hk * ((real)2 * c0 + c1) * third;
+                    T.Q(ty, j).d = (c1 - c0) * third * rh;
                 }
             }
             wave_sync();
@@ -1503,8 +1530,8 @@ __device__ __forceinline__ void sift_segment(const SiftArgs<real>& a, const Segm
                         const bool p0 = !FULL && p == 0;
                         const int ju = p0 ? 2 : ru;
                         const int jl = p0 ? 2 : rl;
-                        const real U = horner(tc[0][ju], (real)(p - tp[0][ju]));
-                        const real L = horner(tc[1][jl], (real)(p - tp[1][jl]));
+                        const real U = horner(T.Q(0, ju), (real)(p - T.P(0, ju)));
+                        const real L = horner(T.Q(1, jl), (real)(p - T.P(1, jl)));
                         if (MEAN == 1) hn[r] = cur.h[r] - (U + L) / (real)2;
                         if (MEAN == 2) hn[r] = cur.h[r] - (use_mean ? (U + L) / (real)2 : (real)0);
                         hout[p] = hn[r];
@@ -1533,17 +1560,17 @@ __device__ __forceinline__ void sift_segment(const SiftArgs<real>& a, const Segm
                 real hnl = hcarry, hnr = (real)0;
                 if (!carry && (FULL || t0 >= 1)) {
                     const int p = t0 - 1;
-                    const int ju = tp[0][1] == p ? 0 : 1;
-                    const int jl = tp[1][1] == p ? 0 : 1;
-                    const real U = horner(tc[0][ju], (real)(p - tp[0][ju]));
-                    const real L = horner(tc[1][jl], (real)(p - tp[1][jl]));
+                    const int ju = T.P(0, 1) == p ? 0 : 1;
+                    const int jl = T.P(1, 1) == p ? 0 : 1;
+                    const real U = horner(T.Q(0, ju), (real)(p - T.P(0, ju)));
+                    const real L = horner(T.Q(1, jl), (real)(p - T.P(1, jl)));
                     hnl = cur.hl - (use_mean ? (U + L) / (real)2 : (real)0);
                 }
                 if (FULL || t0 + kWT < N) {
                     const int p = t0 + kWT;
                     const int ju = cnt[0] + 1, jl = cnt[1] + 1;
-                    const real U = horner(tc[0][ju], (real)(p - tp[0][ju]));
-                    const real L = horner(tc[1][jl], (real)(p - tp[1][jl]));
+                    const real U = horner(T.Q(0, ju), (real)(p - T.P(0, ju)));
+                    const real L = horner(T.Q(1, jl), (real)(p - T.P(1, jl)));
                     hnr = cur.hr - (use_mean ? (U + L) / (real)2 : (real)0);
                 }
                 // ---- knots of h_new (select_extrema_*, :150-207): their row
@@ -1614,8 +1641,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(KC == 
 void k_sift_w(SiftArgs<real> a, int nsegs)
 {
     constexpr int NW = kThreads / kWave;
-    __shared__ int s_tp[NW][2][KC];
-    __shared__ Coef<real> s_tc[NW][2][KC];
+    __shared__ KnotTable<real, KC> s_tab[NW];
     const int lane = threadIdx.x & 63, wv = wave_index();
     if (a.pc.run && load_scalar(a.pc.run) == 0u) return;
     // a small-table pass whose input has too many over-full tiles is left
@@ -1623,14 +1649,14 @@ void k_sift_w(SiftArgs<real> a, int nsegs)
     if (KC == kWKS && load_scalar(a.dense) > a.dense_max) return;
     if (!QUEUE) {
         const int sid = blockIdx.x * NW + wv;
-        if (sid < nsegs) sift_segment<real, RILL, KC>(a, a.segs, sid, lane, s_tp[wv], s_tc[wv]);
+        if (sid < nsegs) sift_segment<real, RILL, KC>(a, a.segs, sid, lane, s_tab[wv]);
         return;
     }
     unsigned got = lane == 0 ? atomicAdd(a.queue, 1u) : 0u;
     int sid = __builtin_amdgcn_readlane((int)got, 0);
     while (sid < nsegs) {
         const unsigned next = lane == 0 ? atomicAdd(a.queue, 1u) : 0u;
-        sift_segment<real, RILL, KC>(a, a.segs, sid, lane, s_tp[wv], s_tc[wv]);
+        sift_segment<real, RILL, KC>(a, a.segs, sid, lane, s_tab[wv]);
         wave_sync();
         sid = __builtin_amdgcn_readlane((int)next, 0);
     }
@@ -1644,8 +1670,7 @@ template <typename real, bool RILL>
 __global__ __launch_bounds__(kThreads) void k_sift_list(SiftArgs<real> a, int nsegs)
 {
     constexpr int NW = kThreads / kWave;
-    __shared__ int s_tp[NW][2][kWKC];
-    __shared__ Coef<real> s_tc[NW][2][kWKC];
+    __shared__ KnotTable<real, kWKC> s_tab[NW];
     const int lane = threadIdx.x & 63, wv = wave_index();
     if (a.pc.run && load_scalar(a.pc.run) == 0u) return;
     const bool all = load_scalar(a.dense) > a.dense_max;
@@ -1654,7 +1679,7 @@ __global__ __launch_bounds__(kThreads) void k_sift_list(SiftArgs<real> a, int ns
     SiftArgs<real> b = a;
     b.ovf_segs = nullptr;   // full tables hold every tile
     for (int sid = blockIdx.x * NW + wv; sid < ns; sid += gridDim.x * NW)
-        sift_segment<real, RILL, kWKC>(b, segs, sid, lane, s_tp[wv], s_tc[wv]);
+        sift_segment<real, RILL, kWKC>(b, segs, sid, lane, s_tab[wv]);
 }
 
 // Stage start for 256-sample tiles: forms x_i (the same arithmetic as
