@@ -1273,12 +1273,13 @@ __device__ __forceinline__ void store_masks(uint64_t* dst, const uint64_t (&m)[2
 }
 
 // Per-wave knot table of a tile (KC entries per type): each knot's position
-// and the coefficients of the interval that starts at it.  The FP32 small
-// table keeps one 32-byte record per knot (coefficients, then the
-// position), so that a knot's position and coefficients are read at one
-// LDS address; the others keep two arrays (records would cost the full
-// tables waves per SIMD: their LDS would grow from 22 to 35 KB per
-// workgroup in FP32, to 52 KB in FP64).
+// and the coefficients of the interval that starts at it.  The small
+// tables keep one record per knot (coefficients, then the position: 32
+// bytes in FP32, 48 in FP64), so that a knot's position and coefficients
+// are read at one LDS address (-2.5 % on the FP32 sift, -1 % on FP64); the
+// full tables keep two arrays (records would cost them waves per SIMD:
+// their LDS would grow from 22 to 35 KB per workgroup in FP32, to 52 KB in
+// FP64).
 template <typename real, int KC, bool REC>
 struct KnotTableOf {
     int pos[2][KC];
@@ -1288,7 +1289,7 @@ struct KnotTableOf {
 };
 template <typename real, int KC>
 struct KnotTableOf<real, KC, true> {
-    struct alignas(32) Rec {
+    struct alignas(sizeof(real) == 4 ? 32 : 16) Rec {
         Coef<real> q;
         int pos;
     };
@@ -1297,7 +1298,7 @@ struct KnotTableOf<real, KC, true> {
     __device__ __forceinline__ Coef<real>& Q(int ty, int j) { return r[ty][j].q; }
 };
 template <typename real, int KC>
-using KnotTable = KnotTableOf<real, KC, std::is_same<real, float>::value && KC == kWKS>;
+using KnotTable = KnotTableOf<real, KC, KC == kWKS>;
 
 template <typename real, int KC>
 struct WTile {
