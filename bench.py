@@ -193,7 +193,11 @@ def forward_workload(icm, torch, dev, name, xs, steps, warmup, graph=False):
                            if measured_traffic(name, W["dtype"]) else None,
                            "kernel": "k_sift_w (+ k_sift_list on small-table passes)",
                            "avg_launch_ms": avg_ms, "alg_bytes_per_launch": bpl,
-                           "launches": int(sift_n)}
+                           "launches": int(sift_n),
+                           "note": "timed steps run the noise-mode and CEEMDAN phases side by "
+                                   "side, so each sift launch shares the CUs with the other "
+                                   "phase's kernels; roofline_alone is the same kernel with "
+                                   "the GPU to itself"}
     return res
 
 
