@@ -180,6 +180,22 @@ __device__ __forceinline__ LastTwo lt_none()
     return r;
 }
 
+// Window item w of a series with wmx windows of maxima and wmn of minima:
+// the two types alternate (maxima window i, then minima window i), so the
+// two windows over one stretch of samples go to neighbouring waves of one
+// solve workgroup and read the same cache lines at about the same time.
+__device__ __forceinline__ void window_of_item(int w, int wmx, int wmn, int& type, int& wi)
+{
+    const int m = min(wmx, wmn);
+    if (w < 2 * m) {
+        type = w & 1;
+        wi = w >> 1;
+    } else {
+        type = wmx > wmn ? 0 : 1;
+        wi = m + (w - 2 * m);
+    }
+}
+
 template <typename real>
 __global__ __launch_bounds__(kPT) void k_knot_prefix_b(SeriesDev sd, KnotsDev<real> k, const real* h,
                                                        PassCtl pc, const int* active)
@@ -326,8 +342,8 @@ __global__ __launch_bounds__(kPT) void k_knot_prefix_b(SeriesDev sd, KnotsDev<re
     __syncthreads();
     const unsigned wbase = s_wbase;
     for (int w = tid; w < wmx + wmn; w += kPT) {
-        const int type = w >= wmx;
-        const int wi = type ? w - wmx : w;
+        int type, wi;
+        window_of_item(w, wmx, wmn, type, wi);
         pc.winlist[kWinItem * (wbase + w)] = s;
         pc.winlist[kWinItem * (wbase + w) + 1] = (wi << 1) | type;
         pc.winlist[kWinItem * (wbase + w) + 2] = -1;   // no tile hint: the solve searches
@@ -506,8 +522,8 @@ __global__ __launch_bounds__(kPT) void k_knot_prefix_1(SeriesDev sd, KnotsDev<re
     __syncthreads();
     const unsigned wbase = s_wbase;
     for (int w = tid; w < wmx + wmn; w += kPT) {
-        const int type = w >= wmx;
-        const int wi = type ? w - wmx : w;
+        int type, wi;
+        window_of_item(w, wmx, wmn, type, wi);
         // the last tile whose knots start at or before the window's first
         // gathered knot, so the solve needs no search
         const int glo = max(wi * pc.K - pc.H - 1, 0);
@@ -721,8 +737,8 @@ __global__ __launch_bounds__(kThreads) void k_knot_prefix_w(SeriesDev sd, KnotsD
     wbase = (unsigned)__builtin_amdgcn_readfirstlane((int)wbase);
     wave_sync();   // the window hints written by other lanes
     for (int w = lane; w < wmx + wmn; w += kWave) {
-        const int type = w >= wmx;
-        const int wi = type ? w - wmx : w;
+        int type, wi;
+        window_of_item(w, wmx, wmn, type, wi);
         pc.winlist[kWinItem * (wbase + w)] = s;
         pc.winlist[kWinItem * (wbase + w) + 1] = (wi << 1) | type;
         pc.winlist[kWinItem * (wbase + w) + 2] = wi < kPWin ? s_hint[wv][type][wi] : -1;
