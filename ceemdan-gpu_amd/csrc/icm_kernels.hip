@@ -96,38 +96,6 @@ __device__ __forceinline__ int mask_first(const uint64_t (&m)[kWR])
     return first;
 }
 
-// tile offset of the k-th set bit (k = 0 the first); k must be below the
-// number of set bits (any other k gives some offset in [0, 256))
-__device__ __forceinline__ int mask_select(const uint64_t (&m)[kWR], int k)
-{
-    static_assert(kWR == 4, "four rows per tile");
-    const int c0 = __popcll(m[0]), c1 = __popcll(m[1]), c2 = __popcll(m[2]);
-    int r = 0;
-    uint64_t w = m[0];
-    if (k >= c0) {
-        k -= c0; r = 1; w = m[1];
-        if (k >= c1) {
-            k -= c1; r = 2; w = m[2];
-            if (k >= c2) { k -= c2; r = 3; w = m[3]; }
-        }
-    }
-    // k-th set bit of w by halving
-    int pos = 0;
-    uint32_t v = (uint32_t)w;
-    int c = __popc(v);
-    if (k >= c) { k -= c; pos = 32; v = (uint32_t)(w >> 32); }
-    c = __popc(v & 0xFFFFu);
-    if (k >= c) { k -= c; pos += 16; v >>= 16; }
-    c = __popc(v & 0xFFu);
-    if (k >= c) { k -= c; pos += 8; v >>= 8; }
-    c = __popc(v & 0xFu);
-    if (k >= c) { k -= c; pos += 4; v >>= 4; }
-    c = __popc(v & 0x3u);
-    if (k >= c) { k -= c; pos += 2; v >>= 2; }
-    if (k >= (int)(v & 1u)) pos += 1;
-    return kWave * r + pos;
-}
-
 template <typename real>
 __device__ __forceinline__ void load_masks(const KnotsDev<real>& k, int64_t gt, int type,
                                            uint64_t (&m)[kWR])
@@ -848,7 +816,7 @@ __device__ __forceinline__ real nak_end(real c1, real c2, int h_e, int h_next)
 // sixth wave per SIMD, which is worth more than the conflicts (-2.5 %).
 constexpr int kTMaskW = kWR + 1;
 
-template <int NIT, int TW>
+template <int TW>
 __device__ __forceinline__ void gather_positions(const SeriesDev& sd, const uint64_t* kmask,
                                                  const int* kpre, const int* kcnt, int s, int type,
                                                  long lo, int glo, int ghi, int* sp, int* tpre,
@@ -873,11 +841,12 @@ __device__ __forceinline__ void gather_positions(const SeriesDev& sd, const uint
         const bool valid = tl < ntl;
         const int p = valid ? kpre[2 * (tb + tl) + type] : 0x7FFFFFFF;
         const int c = valid ? kcnt[2 * (tb + tl) + type] : 0;
-        tpre[lane] = p;
-        if (valid) {
+        uint64_t mk[kWR];
 #pragma unroll
-            for (int r = 0; r < kWR; ++r) tmask[lane][r] = kmask[8 * (int64_t)(tb + tl) + kWR * type + r];
-        }
+        for (int r = 0; r < kWR; ++r) mk[r] = valid ? kmask[8 * (int64_t)(tb + tl) + kWR * type + r] : 0ull;
+        tpre[lane] = p;
+#pragma unroll
+        for (int r = 0; r < kWR; ++r) tmask[lane][r] = mk[r];
         const uint64_t vm = wave_ballot(valid);
         if (vm == 0ull) break;   // counts inconsistent with n: leave the rest
         const int last = 63 - __builtin_clzll(vm);
@@ -889,6 +858,8 @@ __device__ __forceinline__ void gather_positions(const SeriesDev& sd, const uint
         const int xb = ub ? 63 - __builtin_clzll(ub) : 0;
         wave_sync();
         if (xb - xa < 16) {
+            // few tiles with many knots each: the wave goes through the
+            // tiles, one lane per sample of a row
             for (int x = xa; x <= xb; ++x) {
                 // the tile's prefix and four row masks in one round of LDS reads
                 int cb = tpre[x];
@@ -904,23 +875,18 @@ __device__ __forceinline__ void gather_positions(const SeriesDev& sd, const uint
                     cb += __popcll(mr[r]);
                 }
             }
-        } else {
-            for (int gb = g; gb < gend; gb += kWave * NIT) {
+        } else if (lane >= xa && lane <= xb) {
+            // many tiles with few knots each: every lane walks the set bits
+            // of its own tile (prefix and row masks still in its registers)
+            int gi = p;
 #pragma unroll
-                for (int j = 0; j < NIT; ++j) {
-                    const int gi = gb + lane + kWave * j;
-                    if (gi < gend) {
-                        int x = 0, y = last;
-                        while (x < y) {
-                            const int mid = (x + y + 1) >> 1;
-                            if (tpre[mid] <= gi) x = mid;
-                            else y = mid - 1;
-                        }
-                        uint64_t m[kWR];
-#pragma unroll
-                        for (int r = 0; r < kWR; ++r) m[r] = tmask[x][r];
-                        sp[gi - lo] = (tile + x) * kWT + mask_select(m, gi - tpre[x]);
-                    }
+            for (int r = 0; r < kWR; ++r) {
+                uint64_t m = mk[r];
+                while (m != 0ull && gi < gend) {
+                    const int bit = __builtin_ctzll(m);
+                    m &= m - 1ull;
+                    if (gi >= g) sp[gi - lo] = (tile + lane) * kWT + kWave * r + bit;
+                    ++gi;
                 }
             }
         }
@@ -1073,7 +1039,7 @@ __global__ __launch_bounds__(kThreads) void k_solve(SolveArgs<real> a)
         const int glo = (int)max(lo, 0L);
         const int ghi = (int)min(lo + W + 2, (long)n);
         constexpr int NIT = (W + 2 + kWave - 1) / kWave;
-        gather_positions<NIT, kWR>(a.sd, a.k.mask, a.k.pre, a.k.cnt, s, type, lo, glo, ghi, sp,
+        gather_positions<kWR>(a.sd, a.k.mask, a.k.pre, a.k.cnt, s, type, lo, glo, ghi, sp,
                               s_tpre[wv], s_tmask[wv], hint);
         wave_sync();
         gather_values<real, NIT>(a.h + load_scalar(a.sd.off + s), load_scalar(a.sd.len + s), lo, glo,
@@ -2163,9 +2129,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) v
         const long lo = g0 - 1;
         const int glo = (int)max(lo, 0L);
         const int ghi = (int)min(lo + W + 2, (long)n);
-        gather_positions<(W + 2 + kWave - 1) / kWave, kTMaskW>(a.sd, a.ps.mask, a.ps.pre, a.ps.cnt, s, type,
-                                                      lo, glo, ghi, sp, s_tpre[wv], s_tmask[wv],
-                                                      hint);
+        gather_positions<kTMaskW>(a.sd, a.ps.mask, a.ps.pre, a.ps.cnt, s, type, lo, glo, ghi, sp,
+                                  s_tpre[wv], s_tmask[wv], hint);
         wave_sync();
         // lane l owns window rows r = l*B + q (knot gb + q); everything a row
         // needs comes from its own lane's registers or a neighbour lane's
