@@ -78,8 +78,9 @@ struct PassCtl {
 
 enum InitMode : int {
     INIT_COPY = 0,     // h0 = src[off + p]                       (noise residual)
-    INIT_RESID = 3,    // h0 = src[off + p] - E[off + p], also stored in resid[off + p]
-                       //  (the next noise residual, formed where it is first read)
+    INIT_RESID = 3,    // h0 = src[off + p] - E[off + p], stored in resid[off + p] (the
+                       //  next noise residual, formed where it is first read) and in
+                       //  h_out unless h_out is null (the first pass then reads resid)
     INIT_STAGE0 = 1,   // h0 = xt[c][p] + (eps0 * E[p]) * inv[s]  (cudaICEEMDAN.cu:68-78)
     INIT_STAGEK = 2,   // h0 = r[c][p] + (eps0 * E[p]) * sq[c]    (cudaICEEMDAN.cu:80-90)
     INIT_KNOTS = 4,    // knots of src only (no samples written: the pass reads src itself)

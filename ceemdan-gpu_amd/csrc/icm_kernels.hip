@@ -1694,7 +1694,7 @@ __device__ __forceinline__ InitSeries<real, MODE> init_series(const SeriesDev& s
     q.tend = q.tile0 + sd.ntl[s];
     q.N = sd.len[s];
     const int64_t off = sd.off[s];
-    q.out = MODE == INIT_KNOTS ? nullptr : h_out + off;
+    q.out = (MODE == INIT_KNOTS || !h_out) ? nullptr : h_out + off;
     q.resid = MODE == INIT_RESID ? ia.resid + off : nullptr;
     if (MODE == INIT_COPY || MODE == INIT_KNOTS) {
         q.src = ia.src + off;
@@ -1756,7 +1756,7 @@ __device__ __forceinline__ void init_store(const InitSeries<real, MODE>& q, cons
     for (int r = 0; r < kWR; ++r) {
         const int u = r * kWave + lane;
         if (u < nt) {
-            if (MODE != INIT_KNOTS) q.out[t0 + u] = w.x[r];
+            if (MODE == INIT_RESID ? q.out != nullptr : MODE != INIT_KNOTS) q.out[t0 + u] = w.x[r];
             if (MODE == INIT_RESID) q.resid[t0 + u] = w.x[r];
         }
     }

@@ -908,26 +908,32 @@ struct Plan : PlanBase {
     // Sift one stage of a phase, FIXED mode: init + S x (solve, sift).
     // The stage output (h after S subtractions) ends in `out` if given
     // (series offsets), else in buffer S % 2.
+    // h0: the buffer that holds the stage's first pass input when the stage
+    // start writes no samples to L.hA (INIT_KNOTS, or INIT_RESID with no
+    // h_out: the first pass reads the noise residual itself); null = L.hA.
     int stage_fixed(int phase, int stage, const DevSeries& d, const InitArgs<real>& ia, int S,
-                    real* out, hipStream_t st, Rec* record = nullptr, bool full = false)
+                    real* out, hipStream_t st, Rec* record = nullptr, bool full = false,
+                    real* h0 = nullptr)
     {
         Lane& L = ln[phase];
         const SeriesDev sd = d.view(L.kstride, cap);
         int rc;
         if (L.epoch + (uint32_t)S + 8 >= kEpochs - 1 && (rc = reset_control(L, st))) return rc;
         const int64_t pass_bytes = 2 * (int64_t)sizeof(real) * d.samples;
+        real* in0 = h0 ? h0 : L.hA;
         uint32_t e_emit = next_epoch(L);
         if ((rc = prof_begin(0, pass_bytes, st))) return rc;
-        ICM_TRY(launch_init<real>(sd, ia, L.hA, knots(L, 0), ctl(L, e_emit), st, grid_loop));
+        ICM_TRY(launch_init<real>(sd, ia, h0 ? nullptr : L.hA, knots(L, 0), ctl(L, e_emit), st,
+                                  grid_loop));
         if ((rc = prof_end(st))) return rc;
-        ICM_TRY(launch_knot_prefix<real>(sd, knots(L, 0), L.hA, ctl(L, e_emit), nullptr, st));
+        ICM_TRY(launch_knot_prefix<real>(sd, knots(L, 0), in0, ctl(L, e_emit), nullptr, st));
         if ((rc = trace_copy(phase, stage, 0, d, 0, st))) return rc;
         for (int j = 0; j < S; ++j) {
             const int b = j & 1;
             SolveArgs<real> sa{};
             sa.sd = sd;
             sa.k = knots(L, b);
-            sa.h = b ? L.hB : L.hA;
+            sa.h = j == 0 ? in0 : (b ? L.hB : L.hA);
             sa.c = L.kc;
             sa.winlist = L.winlist;
             sa.wincount = L.wincount + e_emit;
@@ -954,7 +960,7 @@ struct Plan : PlanBase {
             if (record && !full && j == S - 1) break;
             SiftArgs<real> a{};
             a.sd = sd;
-            a.h_in = b ? L.hB : L.hA;
+            a.h_in = j == 0 ? in0 : (b ? L.hB : L.hA);
             a.h_out = (j == S - 1 && out) ? out : (b ? L.hA : L.hB);
             a.kin = knots(L, b);
             a.c = L.kc;
@@ -1079,11 +1085,15 @@ struct Plan : PlanBase {
         // stage k starts from the residual w - E_0 - ... - E_{k-1}; stage
         // k >= 1 forms it from the previous residual and E_{k-1} while it
         // reads it (k_init_w, INIT_RESID), alternating two buffers
+        // FIXED: the stage's first pass reads the residual buffer itself, so
+        // the stage start writes no copy of it (RILLING keeps the copy in
+        // the sample buffer: a series that stops at once exports it)
         real* res[2] = {wres, wres2};
+        const bool fixed = cfg.stop_mode == ICM_STOP_FIXED;
         for (int k = 0; k < M - 1; ++k) {
             InitArgs<real> ia{};
             if (k == 0) {
-                ia.mode = INIT_COPY;
+                ia.mode = fixed ? INIT_KNOTS : INIT_COPY;
                 ia.src = wres;
             } else {
                 ia.mode = INIT_RESID;
@@ -1092,8 +1102,9 @@ struct Plan : PlanBase {
                 ia.resid = res[k & 1];
             }
             real* Ek = modes + (int64_t)k * hn.samples;
-            if (cfg.stop_mode == ICM_STOP_FIXED) {
-                if ((rc = stage_fixed(ICM_PHASE_NOISE, k, dn, ia, cfg.noise_max_sifts, Ek, st)))
+            if (fixed) {
+                if ((rc = stage_fixed(ICM_PHASE_NOISE, k, dn, ia, cfg.noise_max_sifts, Ek, st,
+                                      nullptr, false, k == 0 ? wres : res[k & 1])))
                     return rc;
             } else {
                 if ((rc = stage_rilling(ICM_PHASE_NOISE, k, dn, ia, cfg.noise_max_sifts, st)))
