@@ -2,7 +2,8 @@
 
 FP64: bit-identical results are expected (same arithmetic order, no FMA
 contraction on either side); the assertions use exact equality for knot
-counts and a 1e-12 relative bound for samples (the north-star bar is 1e-9).
+counts and a 1e-12 relative bound for samples (the required bar for FP64
+IMFs is 1e-9).
 """
 import os
 
