@@ -74,6 +74,37 @@ __device__ __forceinline__ double shift_next(double v, double fill)
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// Left and right neighbours of the samples of row v of a tile, where row p
+// (q) is the row before (after) it: lane i gets lane i-1 (i+1) of v, and
+// lane 0 (63) gets lane 63 (0) of p (q).  The other row's lane comes from a
+// whole-wave rotation by one lane (DPP wave_ror:1 / wave_rol:1), which is
+// then the fill of the shift: two moves, where a shift with a wave-uniform
+// fill takes a lane read, a move and the shift.
+__device__ __forceinline__ int dpp_rot_prev_i(int v) { return __builtin_amdgcn_mov_dpp(v, 0x13C, 0xF, 0xF, false); }
+__device__ __forceinline__ int dpp_rot_next_i(int v) { return __builtin_amdgcn_mov_dpp(v, 0x134, 0xF, 0xF, false); }
+__device__ __forceinline__ float row_prev(float v, float p)
+{
+    return __int_as_float(dpp_prev_i(__float_as_int(v), dpp_rot_prev_i(__float_as_int(p))));
+}
+__device__ __forceinline__ float row_next(float v, float q)
+{
+    return __int_as_float(dpp_next_i(__float_as_int(v), dpp_rot_next_i(__float_as_int(q))));
+}
+__device__ __forceinline__ double row_prev(double v, double p)
+{
+    const long long b = __double_as_longlong(v), f = __double_as_longlong(p);
+    const int lo = dpp_prev_i((int)b, dpp_rot_prev_i((int)f));
+    const int hi = dpp_prev_i((int)(b >> 32), dpp_rot_prev_i((int)(f >> 32)));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double row_next(double v, double q)
+{
+    const long long b = __double_as_longlong(v), f = __double_as_longlong(q);
+    const int lo = dpp_next_i((int)b, dpp_rot_next_i((int)f));
+    const int hi = dpp_next_i((int)(b >> 32), dpp_rot_next_i((int)(f >> 32)));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // DPP row moves (GFX9 encodings): row_shr:n = 0x110 + n shifts inside each
 // 16-lane row, row_bcast:15 = 0x142 sends lane 15 of a row to the next row,
 // row_bcast:31 = 0x143 sends lane 31 to rows 2 and 3.  Lanes with no source

@@ -67,24 +67,30 @@ __device__ __forceinline__ void wave_sync()
 }
 
 // Bitmasks of the strict maxima and minima among one row of 64 samples
-// (v against its neighbours lft, rgt).  The conditions are combined with
-// bitwise operators, so they stay lane masks combined by scalar
-// instructions: with short-circuit && and || the compiler turned each flag
-// into a 0/1 vector value and compared it again (two more vector
-// instructions per flag, and more scalar registers held, which spilled).
-// `valid` and `end` are the lanes inside the series and at its two ends
-// (ends are knots of both types); FULL rows have every lane valid, no end.
+// (v against its neighbours lft, rgt).  Each comparison is balloted on its
+// own and the masks are combined by scalar instructions: the ballot of one
+// comparison is that comparison's lane mask, while the ballot of a
+// combination of flags costs a v_cndmask and a v_cmp (the compiler turns
+// the flag into a 0/1 vector value and compares it again).  The empty asm
+// statement makes the two row masks scalar registers at this point, so the
+// four comparison masks of every row are not all held until the masks are
+// stored (held together they spilled).  `valid` and `end` are the masks of
+// the lanes inside the series and at its two ends (ends are knots of both
+// types); FULL rows have every lane valid and no end, and ignore both.
 template <bool FULL, typename real>
-__device__ __forceinline__ void row_extrema(real v, real lft, real rgt, bool valid, bool end,
+__device__ __forceinline__ void row_extrema(real v, real lft, real rgt, uint64_t valid, uint64_t end,
                                             uint64_t& mx, uint64_t& mn)
 {
+    const uint64_t gl = wave_ballot(v > lft), gr = wave_ballot(v > rgt);
+    const uint64_t ll = wave_ballot(v < lft), lr = wave_ballot(v < rgt);
     if (FULL) {
-        mx = wave_ballot((v > lft) & (v > rgt));
-        mn = wave_ballot((v < lft) & (v < rgt));
+        mx = gl & gr;
+        mn = ll & lr;
     } else {
-        mx = wave_ballot(valid & (end | ((v > lft) & (v > rgt))));
-        mn = wave_ballot(valid & (end | ((v < lft) & (v < rgt))));
+        mx = valid & (end | (gl & gr));
+        mn = valid & (end | (ll & lr));
     }
+    asm volatile("" : "+s"(mx), "+s"(mn));
 }
 
 // tile offset of the first set bit, -1 if none
@@ -853,7 +859,7 @@ __device__ __forceinline__ void gather_positions(const SeriesDev& sd, const uint
         const int pend = __shfl(p + c, last, kWave);
         const int gend = min(ghi, pend);
         // chunk tiles holding the first and the last knot of [g, gend)
-        const uint64_t ua = wave_ballot(valid & (p <= g)), ub = wave_ballot(valid & (p <= gend - 1));
+        const uint64_t ua = vm & wave_ballot(p <= g), ub = vm & wave_ballot(p <= gend - 1);
         const int xa = ua ? 63 - __builtin_clzll(ua) : 0;
         const int xb = ub ? 63 - __builtin_clzll(ub) : 0;
         wave_sync();
@@ -1312,6 +1318,11 @@ __device__ __forceinline__ void sift_segment(const SiftArgs<real>& a, const Segm
     const int64_t kb0 = sd.kbase[s], kb1 = sd.kstride + sd.kbase[s];
     const real* h = a.h_in + sd.off[s];
     real* hout = a.h_out + sd.off[s];
+    // FP32: the two sample pointers in vector registers (the empty asm makes
+    // them non-uniform), which frees four scalar registers through the loop;
+    // the small-table instance then spills 8 instead of 21 (FP64 has no
+    // vector registers to spare)
+    if (std::is_same<real, float>::value) asm volatile("" : "+v"(h), "+v"(hout));
     const int nn[2] = {a.kin.n[2 * s], a.kin.n[2 * s + 1]};
     const real fx[2][2] = {{a.kin.fix[4 * s], a.kin.fix[4 * s + 1]},
                            {a.kin.fix[4 * s + 2], a.kin.fix[4 * s + 3]}};
@@ -1564,10 +1575,12 @@ __device__ __forceinline__ void sift_segment(const SiftArgs<real>& a, const Segm
                 for (int r = 0; r < kWR; ++r) {
                     const int u = r * kWave + lane;
                     const int p = t0 + u;
-                    const real lft = shift_prev(hn[r], r == 0 ? hnl : lane_value(hn[r > 0 ? r - 1 : 0], kWave - 1));
-                    const real rgt = shift_next(hn[r], r == kWR - 1 ? hnr : lane_value(hn[r < kWR - 1 ? r + 1 : r], 0));
-                    row_extrema<FULL>(hn[r], lft, rgt, FULL || u < nt,
-                                      !FULL && (p == 0 || p == N - 1), em[0][r], em[1][r]);
+                    const real lft = r == 0 ? shift_prev(hn[0], hnl) : row_prev(hn[r], hn[r > 0 ? r - 1 : 0]);
+                    const real rgt = r == kWR - 1 ? shift_next(hn[r], hnr)
+                                                  : row_next(hn[r], hn[r < kWR - 1 ? r + 1 : r]);
+                    row_extrema<FULL>(hn[r], lft, rgt, FULL ? 0ull : wave_ballot(u < nt),
+                                      FULL ? 0ull : wave_ballot(p == 0) | wave_ballot(p == N - 1),
+                                      em[0][r], em[1][r]);
 #pragma unroll
                     for (int ty = 0; ty < 2; ++ty) run[ty] += __popcll(em[ty][r]);
                 }
@@ -1766,9 +1779,10 @@ __device__ __forceinline__ void init_store(const InitSeries<real, MODE>& q, cons
     for (int r = 0; r < kWR; ++r) {
         const int u = r * kWave + lane;
         const int p = t0 + u;
-        const real lft = shift_prev(w.x[r], r == 0 ? w.xl : lane_value(w.x[r > 0 ? r - 1 : 0], kWave - 1));
-        const real rgt = shift_next(w.x[r], r == kWR - 1 ? w.xr : lane_value(w.x[r < kWR - 1 ? r + 1 : r], 0));
-        row_extrema<false>(w.x[r], lft, rgt, u < nt, p == 0 || p == q.N - 1, em[0][r], em[1][r]);
+        const real lft = r == 0 ? shift_prev(w.x[0], w.xl) : row_prev(w.x[r], w.x[r > 0 ? r - 1 : 0]);
+        const real rgt = r == kWR - 1 ? shift_next(w.x[r], w.xr) : row_next(w.x[r], w.x[r < kWR - 1 ? r + 1 : r]);
+        row_extrema<false>(w.x[r], lft, rgt, wave_ballot(u < nt),
+                           wave_ballot(p == 0) | wave_ballot(p == q.N - 1), em[0][r], em[1][r]);
 #pragma unroll
         for (int ty = 0; ty < 2; ++ty) run[ty] += __popcll(em[ty][r]);
     }
