@@ -867,15 +867,22 @@ __device__ __forceinline__ void gather_positions(const SeriesDev& sd, const uint
             // few tiles with many knots each: the wave goes through the
             // tiles, one lane per sample of a row
             for (int x = xa; x <= xb; ++x) {
-                // the tile's prefix and four row masks in one round of LDS reads
-                int cb = tpre[x];
+                // the tile's prefix and four row masks in one round of LDS
+                // reads, moved to scalar registers (the same in every lane):
+                // the lane test is then the exec mask, the counts scalar
+                int cb = __builtin_amdgcn_readfirstlane(tpre[x]);
                 uint64_t mr[kWR];
 #pragma unroll
-                for (int r = 0; r < kWR; ++r) mr[r] = tmask[x][r];
+                for (int r = 0; r < kWR; ++r) {
+                    const uint64_t m = tmask[x][r];
+                    mr[r] = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(m >> 32)) << 32) |
+                            (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)m);
+                }
 #pragma unroll
                 for (int r = 0; r < kWR; ++r) {
-                    if ((mr[r] >> lane) & 1ull) {
-                        const int gi = cb + lanes_below(mr[r]);
+                    if (__builtin_amdgcn_inverse_ballot_w64(mr[r])) {
+                        const int gi = __builtin_amdgcn_mbcnt_hi(
+                            (uint32_t)(mr[r] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mr[r], (uint32_t)cb));
                         if (gi >= g && gi < gend) sp[gi - lo] = (tile + x) * kWT + kWave * r + lane;
                     }
                     cb += __popcll(mr[r]);
