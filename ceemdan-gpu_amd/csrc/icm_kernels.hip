@@ -943,6 +943,34 @@ __device__ __forceinline__ real sdiv(real x, real y)
     else return x * (real)__builtin_amdgcn_rcpf((float)y);
 }
 
+// Shuffles of the reduction by ST lanes: lane i gets lane i - ST (lane_up)
+// or i + ST (lane_down); what a lane with no source gets is unspecified (the
+// callers do not use it).  By one lane a DPP whole-wave shift (one vector
+// move per 32 bits); farther a ds_bpermute.
+template <int CTRL>
+__device__ __forceinline__ int dpp_move(int v) { return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, true); }
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) { return __int_as_float(dpp_move<CTRL>(__float_as_int(v))); }
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_move<CTRL>((int)b), hi = dpp_move<CTRL>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <int ST, typename real>
+__device__ __forceinline__ real lane_up(real v)
+{
+    if constexpr (ST == 1) return dpp_move<0x138>(v);
+    else return shfl_up(v, ST);
+}
+template <int ST, typename real>
+__device__ __forceinline__ real lane_down(real v)
+{
+    if constexpr (ST == 1) return dpp_move<0x130>(v);
+    else return shfl_down(v, ST);
+}
+
 template <typename real, int B, class RowFn>
 __device__ __forceinline__ void partition_pcr(int lane, RowFn row, real (&x)[B])
 {
@@ -974,7 +1002,7 @@ __device__ __forceinline__ void partition_pcr(int lane, RowFn row, real (&x)[B])
     // interface row q = B-1 -> reduced tridiagonal system in z_lane
     real dl, d, du, f;
     row(B - 1, dl, d, du, f);
-    real na0 = shfl_down(al[0], 1), nb0 = shfl_down(be[0], 1), ng0 = shfl_down(ga[0], 1);
+    real na0 = lane_down<1>(al[0]), nb0 = lane_down<1>(be[0]), ng0 = lane_down<1>(ga[0]);
     if (lane == kWave - 1) {
         du = (real)0;
         na0 = (real)0;
@@ -985,12 +1013,12 @@ __device__ __forceinline__ void partition_pcr(int lane, RowFn row, real (&x)[B])
     real D = (d + dl * ga[B - 2]) + du * nb0;
     real C = du * ng0;
     real R = (f - dl * al[B - 2]) - du * na0;
-#pragma unroll
-    for (int st = 1; st < kWave; st <<= 1) {
-        const real Am = shfl_up(A, st), Dm = shfl_up(D, st), Cm = shfl_up(C, st),
-                   Rm = shfl_up(R, st);
-        const real Ap = shfl_down(A, st), Dp = shfl_down(D, st), Cp = shfl_down(C, st),
-                   Rp = shfl_down(R, st);
+    auto level = [&](auto st_c) {
+        constexpr int st = decltype(st_c)::value;
+        const real Am = lane_up<st>(A), Dm = lane_up<st>(D), Cm = lane_up<st>(C),
+                   Rm = lane_up<st>(R);
+        const real Ap = lane_down<st>(A), Dp = lane_down<st>(D), Cp = lane_down<st>(C),
+                   Rp = lane_down<st>(R);
         real a2 = (real)0, c2 = (real)0, d2 = D, r2 = R;
         if (lane - st >= 0) {
             const real k1 = sdiv<real>(A, Dm);
@@ -1008,9 +1036,15 @@ __device__ __forceinline__ void partition_pcr(int lane, RowFn row, real (&x)[B])
         C = c2;
         D = d2;
         R = r2;
-    }
+    };
+    level(std::integral_constant<int, 1>{});
+    level(std::integral_constant<int, 2>{});
+    level(std::integral_constant<int, 4>{});
+    level(std::integral_constant<int, 8>{});
+    level(std::integral_constant<int, 16>{});
+    level(std::integral_constant<int, 32>{});
     const real z = sdiv<real>(R, D);
-    real zl = shfl_up(z, 1);
+    real zl = lane_up<1>(z);
     if (lane == 0) zl = (real)0;
 #pragma unroll
     for (int q = 0; q < B; ++q) x[q] = (q == B - 1) ? z : (al[q] + be[q] * zl) + ga[q] * z;
